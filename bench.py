@@ -1,0 +1,170 @@
+"""Benchmark: SPH particle-steps/s of the 1M-particle FP32 dam break on one GPU.
+
+Workload: a 100 x 100 x 100 particle column (spacing h/2, +-1 % seeded jitter,
+zero velocity) in a closed 40 m tank, dt = 0.01, the FP32 forward step of
+FluidWorld::step (crates/phy-fluid/src/sph/world.rs:183-192) on the GPU.
+Default: 20 untimed warm-up steps, then the remaining 980 steps of the
+1000-step run timed. Inputs are resident in HBM before the timed region.
+
+Prints one JSON line. The timed steps run in chunks, each bracketed by a device
+synchronisation; state snapshots for the CPU baseline are taken between chunks,
+outside the timed windows. Per-stage kernel times come from HIP events recorded
+on the world's own stream (phy_fluid_profile_*).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+# algorithmic HBM bytes per particle-step (compulsory FP32 traffic): sort
+# 60 B, density 20 B, fused force + integrate 56 B (read x, v, rho, p; write x', v')
+ALG_BYTES = {"sort": 60, "density": 20, "force": 56}
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=980)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n-side", type=int, default=100)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--chunks", type=int, default=4)
+    ap.add_argument("--cpu-budget", type=float, default=20.0,
+                    help="seconds of oracle CPU work for the baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": a.gpus}), flush=True)
+        sys.exit(2)
+    from physics_amd import ffi, scenes
+
+    n_side = a.n_side
+    sc = scenes.dam_break((n_side, n_side, n_side), seed=a.seed)
+    n = len(sc.pos)
+    gp = scenes.apply_scene(ffi.PhySphParams.defaults(), sc)
+    g = ffi.FluidWorld(gp)
+    g.set_particles_f32(sc.pos, None, np.full(n, sc.mass, dtype=np.float32))
+    dt = sc.dt
+
+    for _ in range(a.warmup):
+        g.step(dt)
+    g.sync()
+
+    K = a.steps
+    chunks = max(1, min(a.chunks, K))
+    sizes = [K // chunks + (1 if i < K % chunks else 0) for i in range(chunks)]
+    wall = 0.0
+    stage = np.zeros(4)
+    timed_steps = 0
+    snaps = []
+    for c, k in enumerate(sizes):
+        if k == 0:
+            continue
+        g.sync()
+        g.profile_begin()
+        t0 = time.perf_counter()
+        g.step(dt, k)
+        g.sync()
+        wall += time.perf_counter() - t0
+        ms, s = g.profile_end()
+        stage += ms
+        timed_steps += s
+        if not a.no_cpu:
+            snaps.append((g.get("pos"), g.get("vel")))
+    assert timed_steps == K
+
+    value = n * K / wall
+    per = stage / K  # ms per step per stage
+    names = ["sort", "density", "force"]
+    dom = names[int(np.argmax(per[:3]))]
+    t_dom = per[names.index(dom)] * 1e-3
+    achieved = ALG_BYTES[dom] * n / t_dom / 1e9
+    x = g.get("pos")
+    assert np.isfinite(x).all(), "non-finite state in the benchmark run"
+
+    out = {
+        "metric": "SPH particle-steps/s at 1M (FP32 fwd) + adjoint/forward time ratio, 1 GPU",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": a.warmup,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"dam-break column {n_side}^3 = {n} particles, spacing h/2 (0.1f), "
+                        f"+-1% SplitMix64 jitter seed {a.seed}, zero velocity, closed "
+                        f"{sc.bounds_max[0]:.0f} m tank, dt {dt}, steps {a.warmup + 1}.."
+                        f"{a.warmup + K} of the run",
+            "particles": n,
+            "h": sc.h,
+            "parallelism": "single GPU",
+        },
+        "stages_ms_per_step": {"sort": per[0], "density": per[1], "force": per[2]},
+        "adjoint_forward_ratio": None,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": f"k_{dom}" if dom != "sort" else "sort kernels",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "alg_bytes_per_particle": ALG_BYTES[dom],
+        },
+    }
+
+    if not a.no_cpu and snaps:
+        import oracle as O
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        O.set_threads(threads)
+        op = O.Params.defaults()
+        op.h = sc.h
+        for ax in range(3):
+            op.bounds_min[ax] = sc.bounds_min[ax]
+            op.bounds_max[ax] = sc.bounds_max[ax]
+        w = O.World(op)
+        mass = np.full(n, float(np.float32(sc.mass)))
+        t_cpu, done, used = 0.0, 0, []
+        # one oracle step from each snapshot, spread over the timed trajectory
+        for idx in range(len(snaps)):
+            pos, vel = snaps[idx]
+            w.set_particles(pos, vel, mass)
+            t_cpu += w.time_steps(dt, 1)
+            done += 1
+            used.append(idx)
+            if t_cpu >= a.cpu_budget:
+                break
+        out["cpu_baseline"] = {
+            "value": n * done / t_cpu,
+            "unit": "particle-steps/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"f64 oracle (C++ restatement of FluidWorld::step, OpenMP) one step "
+                      f"from each of {done} GPU states taken at the end of timed chunks "
+                      f"{used} of {len(sizes)}; {t_cpu:.1f} s of CPU work",
+        }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

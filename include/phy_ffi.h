@@ -1,0 +1,224 @@
+/* phy_ffi.h — C ABI of the MI355X phy-fluid library (libphy_ffi.so).
+ *
+ * Drop-in for the fluid subset of the reference ABI, crates/phy-ffi/phy_ffi.h
+ * (ABI version 2): same symbol names, signatures, ownership and error codes.
+ * The world's particle state lives on the GPU in FP32; the f64 readback channel
+ * returns the exact (double) of the FP32 value, so the f32 channel is always
+ * the exact (float) cast of the f64 channel (crates/phy-ffi/src/lib.rs:978-1001).
+ *
+ * Symbols below the "additive" line have no counterpart in the reference; they
+ * are a backwards-compatible minor-version extension (policy lib.rs:19-41).
+ *
+ * Handles are not thread-safe: serialise calls on one handle. Distinct handles
+ * are independent (each owns its own HIP stream and buffers).
+ */
+#ifndef PHY_FFI_H
+#define PHY_FFI_H
+
+#include <stdbool.h>
+#include <stdint.h>
+#include <stdlib.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* phy_ffi.h:15-24 */
+#define PHY_FFI_ABI_VERSION 2
+#define PHY_FFI_VERSION_MAJOR 0
+#define PHY_FFI_VERSION_MINOR 2
+#define PHY_FFI_VERSION_PATCH 0
+
+/* Opaque world handle (phy_ffi.h:28-30). */
+typedef struct PhyWorldHandle PhyWorldHandle;
+
+/* ---- reference ABI v2 ------------------------------------------------------ */
+
+/* Returns PHY_FFI_ABI_VERSION (2). Replaces lib.rs:106-109 / phy_ffi.h:36. */
+uint32_t phy_ffi_abi_version(void);
+
+/* Dam-break fluid world: SphParams::defaults() and
+ * fill_box((-4,-4,-1), (0,4,1), 0.18, 0.05) = 10,648 particles.
+ * NULL on failure (e.g. no GPU; see phy_last_error). Replaces lib.rs:113-126,
+ * 209-214 / phy_ffi.h:55. */
+PhyWorldHandle *phy_world_create_fluid(void);
+
+/* Non-fluid factories: outside this library (rigid, granular, coupled, soft,
+ * field, optics, solid worlds). Exported so ABI v2 stays link-compatible; they
+ * return NULL, the documented failure value (phy_ffi.h:57-68, lib.rs:216-328). */
+PhyWorldHandle *phy_world_create_rigid(void);
+PhyWorldHandle *phy_world_create_rigid_empty(void);
+PhyWorldHandle *phy_world_create_granular(void);
+PhyWorldHandle *phy_world_create_coupled(void);
+PhyWorldHandle *phy_world_create_soft_empty(void);
+PhyWorldHandle *phy_world_create_field_empty(void);
+PhyWorldHandle *phy_world_create_optics_empty(void);
+PhyWorldHandle *phy_world_create_solid_empty(void);
+
+/* Step dt seconds: 0 ok, -1 null handle or internal error.
+ * Replaces lib.rs:330-338 / phy_ffi.h:71. */
+int32_t phy_world_step(PhyWorldHandle *w, double dt);
+
+/* Step with the NaN/Inf watchdog: 0 ok, -1 null/error, 2 non-finite value
+ * (world stops at that frame), 3 stalled (never produced), 4 other.
+ * Replaces lib.rs:350-363 / phy_ffi.h:83. */
+int32_t phy_world_step_checked(PhyWorldHandle *w, double dt);
+
+/* Simulation time (f64 accumulation of dt); NaN for a null handle.
+ * Replaces lib.rs:365-372 / phy_ffi.h:87. */
+double phy_world_time(PhyWorldHandle *w);
+
+/* Number of subsystems; 0 for null. Replaces lib.rs:374-378 / phy_ffi.h:90. */
+uintptr_t phy_world_sub_count(PhyWorldHandle *w);
+
+/* Total particles over all fluid subsystems; 0 for null.
+ * Replaces lib.rs:380-397 / phy_ffi.h:93. */
+uintptr_t phy_world_fluid_count(PhyWorldHandle *w);
+
+/* Positions / velocities (x,y,z interleaved, index order) into buf of len
+ * doubles; returns particles written, truncating at len/3.
+ * Replaces lib.rs:399-460 / phy_ffi.h:97, 102. */
+uintptr_t phy_world_get_fluid_positions(PhyWorldHandle *w, double *buf, uintptr_t len);
+uintptr_t phy_world_get_fluid_velocities(PhyWorldHandle *w, double *buf, uintptr_t len);
+
+/* Rigid-body readbacks: no rigid subsystem exists in this library; 0 bodies.
+ * Replaces lib.rs:462-515 / phy_ffi.h:105, 109. */
+uintptr_t phy_world_rigid_count(PhyWorldHandle *w);
+uintptr_t phy_world_get_rigid_transforms(PhyWorldHandle *w, double *buf, uintptr_t len);
+
+/* JSON save / load (phy-io archive). Not provided by this library: -1 / NULL,
+ * the documented failure values. Replaces lib.rs:517-542 / phy_ffi.h:112, 115. */
+int32_t phy_world_save(PhyWorldHandle *w, const char *path);
+PhyWorldHandle *phy_world_load(const char *path);
+
+/* Free a handle; null-safe. A second destroy of the same non-null handle is a
+ * caller error (as in the reference). Replaces lib.rs:544-553 / phy_ffi.h:118. */
+void phy_world_destroy(PhyWorldHandle *w);
+
+/* f32 channel. Replaces lib.rs:582-653, 655-689 / phy_ffi.h:129-141. */
+int32_t phy_world_step_f32(PhyWorldHandle *w, float dt);
+uintptr_t phy_world_get_fluid_positions_f32(PhyWorldHandle *w, float *buf, uintptr_t len);
+uintptr_t phy_world_get_fluid_velocities_f32(PhyWorldHandle *w, float *buf, uintptr_t len);
+uintptr_t phy_world_get_rigid_transforms_f32(PhyWorldHandle *w, float *buf, uintptr_t len);
+
+/* Rigid-body control: no rigid subsystem here; -1 (documented failure).
+ * Replaces lib.rs:712-899 / phy_ffi.h:149-175. */
+int64_t phy_world_rigid_add_body(PhyWorldHandle *w, int32_t shape_kind, double mass,
+                                 const double *pos7, const double *inertia3);
+int32_t phy_world_rigid_apply_force(PhyWorldHandle *w, int64_t id, const double *f3, double dt,
+                                    int32_t _mode);
+int32_t phy_world_rigid_apply_torque(PhyWorldHandle *w, int64_t id, const double *t3, double dt,
+                                     int32_t _mode);
+int32_t phy_world_rigid_get_velocity(PhyWorldHandle *w, int64_t id, double *out3);
+int32_t phy_world_rigid_get_angular_velocity(PhyWorldHandle *w, int64_t id, double *out3);
+
+/* ---- additive (no counterpart in the reference) ------------------------------ */
+
+#define PHY_MAX_MATERIALS 16
+
+/* SphParams<f64> as a POD (crates/phy-fluid/src/sph/params.rs:12-50); the
+ * material tables hold n_visc_k / n_visc_n valid entries. */
+typedef struct PhySphParams {
+    double rest_density;
+    double stiffness;
+    double viscosity;
+    double visc_k[PHY_MAX_MATERIALS];
+    double visc_n[PHY_MAX_MATERIALS];
+    uint32_t n_visc_k;
+    uint32_t n_visc_n;
+    double shear_min;
+    double mass;
+    double h;
+    double gravity[3];
+    double bounds_min[3];
+    double bounds_max[3];
+    double boundary_damp;
+    double xsph_eps;
+} PhySphParams;
+
+/* Particle fields for phy_fluid_get_field*: vector fields give 3 values per
+ * particle, scalar fields 1. ACC, XSPH and MU_EFF hold the last step's values
+ * and need phy_fluid_set_keep_aux(w, 1) before that step. */
+enum {
+    PHY_FIELD_POS = 0,
+    PHY_FIELD_VEL = 1,
+    PHY_FIELD_ACC = 2,
+    PHY_FIELD_XSPH = 3,
+    PHY_FIELD_RHO = 4,
+    PHY_FIELD_P = 5,
+    PHY_FIELD_MASS = 6,
+    PHY_FIELD_MU_EFF = 7
+};
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *phy_last_error(void);
+
+/* Fill out with SphParams::defaults() (params.rs:54-85). */
+void phy_sph_params_defaults(PhySphParams *out);
+
+/* World with one empty fluid subsystem of the given parameters; NULL on error. */
+PhyWorldHandle *phy_world_create_fluid_with(const PhySphParams *params);
+
+/* Replace / read the parameters of the first fluid subsystem. 0 ok, -1 error. */
+int32_t phy_fluid_set_params(PhyWorldHandle *w, const PhySphParams *params);
+int32_t phy_fluid_get_params(PhyWorldHandle *w, PhySphParams *out);
+
+/* FluidWorld::fill_box / lattice_mass (world.rs:124-169) on the first fluid
+ * subsystem. fill_box returns 0 ok / -1; lattice_mass NaN on error. */
+int32_t phy_fluid_fill_box(PhyWorldHandle *w, const double *min3, const double *max3,
+                           double spacing, double margin);
+double phy_fluid_lattice_mass(PhyWorldHandle *w, double spacing);
+
+/* Replace all particles of the first fluid subsystem (index order). vel, mass
+ * and material may be NULL (zero velocity, SphParams::mass, material 0).
+ * Values are rounded to FP32. 0 ok, -1 error. */
+int32_t phy_fluid_set_particles(PhyWorldHandle *w, uintptr_t n, const double *pos3,
+                                const double *vel3, const double *mass,
+                                const uint32_t *material);
+int32_t phy_fluid_set_particles_f32(PhyWorldHandle *w, uintptr_t n, const float *pos3,
+                                    const float *vel3, const float *mass,
+                                    const uint32_t *material);
+
+/* Read a PHY_FIELD_* field in index order into buf (len values); returns the
+ * particles written (truncating like the position readback), 0 on error. */
+uintptr_t phy_fluid_get_field(PhyWorldHandle *w, int32_t field, double *buf, uintptr_t len);
+uintptr_t phy_fluid_get_field_f32(PhyWorldHandle *w, int32_t field, float *buf, uintptr_t len);
+
+/* Keep acc / xsph / mu_eff of each step (costs 36 B per particle-step of HBM
+ * writes). 0 ok, -1 error. */
+int32_t phy_fluid_set_keep_aux(PhyWorldHandle *w, int32_t on);
+
+/* body_acc accumulator (3 per particle, index order), consumed by the next step
+ * (world.rs:356, 361). 0 ok, -1 error. */
+int32_t phy_fluid_set_body_acc(PhyWorldHandle *w, const double *b3, uintptr_t len);
+
+/* FP64 device reductions over the current state: out5 = total mass, momentum
+ * x, y, z (sum m v) and kinetic energy (sum |v|^2 m / 2) (subsystem.rs:149-163).
+ * 0 ok, -1 error. */
+int32_t phy_fluid_diagnostics_f64(PhyWorldHandle *w, double *out5);
+
+/* Density-predicate neighbour statistics of the current state (index order):
+ * neighbour count (self included), sum of neighbour indices and xor of their
+ * SplitMix64 hashes. For exact neighbour-set comparison. 0 ok, -1 error. */
+int32_t phy_fluid_neighbor_stats(PhyWorldHandle *w, uint32_t *counts, uint64_t *sums,
+                                 uint64_t *xors, uintptr_t len);
+
+/* n steps of dt in one call (same result as n phy_world_step calls). */
+int32_t phy_world_step_n(PhyWorldHandle *w, double dt, uint32_t n);
+
+/* Steps are asynchronous on the world's HIP stream; block until the GPU has
+ * finished every step issued so far. 0 ok, -1 error. */
+int32_t phy_fluid_sync(PhyWorldHandle *w);
+
+/* Per-stage timing with HIP events on the world's stream. Between begin and
+ * end every step records its stages; end synchronises and writes the summed
+ * milliseconds {sort, density, force, whole step} to ms4 and the number of
+ * timed steps to *steps (may be NULL). 0 ok, -1 error. */
+int32_t phy_fluid_profile_begin(PhyWorldHandle *w);
+int32_t phy_fluid_profile_end(PhyWorldHandle *w, double *ms4, uint32_t *steps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PHY_FFI_H */

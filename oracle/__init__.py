@@ -1,0 +1,263 @@
+"""ctypes binding of the f64 CPU oracle (test infrastructure only).
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg may
+import this package; it is the checker, never the product path.  The oracle
+restates ``FluidWorld<f64>::step`` of crates/phy-fluid/src/sph/world.rs:183-397
+(see sph_oracle.cpp for the per-function citations).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_MAX_MAT = 16
+
+MUTATIONS = {
+    "none": 0,
+    "drop_xsph": 1,
+    "visc_sign": 2,
+    "no_pressure_clamp": 3,
+    "self_in_force": 4,
+    "density_no_self": 5,
+    "bounds_no_damp": 6,
+    "poly6_h2": 7,
+    "dt_in_xsph": 8,
+}
+
+
+class Params(C.Structure):
+    """Mirror of ``SphParams<f64>`` (crates/phy-fluid/src/sph/params.rs:12-50)."""
+
+    _fields_ = [
+        ("rest_density", C.c_double),
+        ("stiffness", C.c_double),
+        ("viscosity", C.c_double),
+        ("visc_k", C.c_double * _MAX_MAT),
+        ("visc_n", C.c_double * _MAX_MAT),
+        ("n_visc_k", C.c_int32),
+        ("n_visc_n", C.c_int32),
+        ("shear_min", C.c_double),
+        ("mass", C.c_double),
+        ("h", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("bounds_min", C.c_double * 3),
+        ("bounds_max", C.c_double * 3),
+        ("boundary_damp", C.c_double),
+        ("xsph_eps", C.c_double),
+    ]
+
+    @classmethod
+    def defaults(cls) -> "Params":
+        p = cls()
+        lib().or_params_defaults(C.byref(p))
+        return p
+
+    def set_visc(self, k, n) -> None:
+        k = list(k)
+        n = list(n)
+        assert 1 <= len(k) <= _MAX_MAT and 1 <= len(n) <= _MAX_MAT
+        for i, v in enumerate(k):
+            self.visc_k[i] = v
+        for i, v in enumerate(n):
+            self.visc_n[i] = v
+        self.n_visc_k = len(k)
+        self.n_visc_n = len(n)
+
+    def copy(self) -> "Params":
+        q = Params()
+        C.memmove(C.byref(q), C.byref(self), C.sizeof(Params))
+        return q
+
+
+_lib = None
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile (g++, OpenMP, -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    L = C.CDLL(_LIB_PATH)
+    vp, dp, sz = C.c_void_p, C.POINTER(C.c_double), C.c_size_t
+    L.or_params_size.restype = C.c_int
+    L.or_params_defaults.argtypes = [C.POINTER(Params)]
+    L.or_set_mutation.argtypes = [C.c_int]
+    L.or_set_threads.argtypes = [C.c_int]
+    L.or_get_threads.restype = C.c_int
+    L.or_kernels.argtypes = [C.c_double, dp, C.c_double, dp]
+    L.or_world_new.argtypes = [C.POINTER(Params)]
+    L.or_world_new.restype = vp
+    L.or_world_free.argtypes = [vp]
+    L.or_world_clone.argtypes = [vp]
+    L.or_world_clone.restype = vp
+    L.or_world_set_params.argtypes = [vp, C.POINTER(Params)]
+    L.or_world_len.argtypes = [vp]
+    L.or_world_len.restype = sz
+    L.or_world_lattice_mass.argtypes = [vp, C.c_double]
+    L.or_world_lattice_mass.restype = C.c_double
+    L.or_world_fill_box.argtypes = [vp, dp, dp, C.c_double, C.c_double]
+    L.or_world_set_particles.argtypes = [vp, sz, dp, dp, dp, C.POINTER(C.c_uint64), dp]
+    L.or_world_get.argtypes = [vp, C.c_int, dp]
+    L.or_world_step.argtypes = [vp, C.c_double]
+    L.or_world_step_n.argtypes = [vp, C.c_double, C.c_int]
+    L.or_world_density.argtypes = [vp]
+    L.or_world_forces.argtypes = [vp]
+    L.or_world_validate.argtypes = [vp]
+    L.or_world_validate.restype = C.c_int
+    L.or_world_momentum.argtypes = [vp, dp]
+    L.or_world_kinetic_energy.argtypes = [vp]
+    L.or_world_kinetic_energy.restype = C.c_double
+    L.or_world_neighbors.argtypes = [vp, sz, C.POINTER(C.c_uint64), sz]
+    L.or_world_neighbors.restype = sz
+    L.or_world_neighbor_stats.argtypes = [
+        vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.or_world_time_steps.argtypes = [vp, C.c_double, C.c_int]
+    L.or_world_time_steps.restype = C.c_double
+    assert L.or_params_size() == C.sizeof(Params), "Params layout mismatch"
+    _lib = L
+    return L
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def kernels(h: float, r: float = 0.0):
+    """(poly6_coeff, spiky_grad_coeff, visc_lap_coeff), (poly6(r), spiky(r), lap(r))."""
+    c = np.zeros(3)
+    v = np.zeros(3)
+    lib().or_kernels(h, _dptr(c), r, _dptr(v))
+    return c, v
+
+
+def set_mutation(name: str) -> None:
+    lib().or_set_mutation(MUTATIONS[name])
+
+
+def set_threads(n: int) -> None:
+    lib().or_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return lib().or_get_threads()
+
+
+_VEC_FIELDS = {"pos": 0, "vel": 1, "acc": 2, "body_acc": 3, "xsph": 4}
+_SCALAR_FIELDS = {"rho": 5, "p": 6, "mass": 7, "mu_eff": 8}
+
+
+class World:
+    """The oracle's ``FluidWorld<f64>``."""
+
+    def __init__(self, params: Params | None = None, _handle=None):
+        self.params = (params or Params.defaults()).copy()
+        L = lib()
+        self._h = _handle if _handle is not None else L.or_world_new(C.byref(self.params))
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            lib().or_world_free(h)
+            self._h = None
+
+    def clone(self) -> "World":
+        return World(self.params, _handle=lib().or_world_clone(self._h))
+
+    def set_params(self, params: Params) -> None:
+        self.params = params.copy()
+        lib().or_world_set_params(self._h, C.byref(self.params))
+
+    def __len__(self) -> int:
+        return int(lib().or_world_len(self._h))
+
+    def lattice_mass(self, s: float) -> float:
+        return lib().or_world_lattice_mass(self._h, s)
+
+    def fill_box(self, mn, mx, spacing: float, margin: float) -> None:
+        a = np.ascontiguousarray(mn, dtype=np.float64)
+        b = np.ascontiguousarray(mx, dtype=np.float64)
+        lib().or_world_fill_box(self._h, _dptr(a), _dptr(b), spacing, margin)
+
+    def set_particles(self, pos, vel=None, mass=None, material=None, body_acc=None) -> None:
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        n = pos.shape[0]
+        args = [pos]
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(n, 3)
+        mass = None if mass is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(mass, dtype=np.float64), (n,)))
+        material = None if material is None else np.ascontiguousarray(material, dtype=np.uint64)
+        body_acc = None if body_acc is None else np.ascontiguousarray(
+            body_acc, dtype=np.float64).reshape(n, 3)
+        self._keep = (args, vel, mass, material, body_acc)
+        lib().or_world_set_particles(
+            self._h, n, _dptr(pos),
+            _dptr(vel) if vel is not None else None,
+            _dptr(mass) if mass is not None else None,
+            material.ctypes.data_as(C.POINTER(C.c_uint64)) if material is not None else None,
+            _dptr(body_acc) if body_acc is not None else None)
+
+    def get(self, field: str) -> np.ndarray:
+        n = len(self)
+        if field in _VEC_FIELDS:
+            out = np.empty((n, 3))
+            lib().or_world_get(self._h, _VEC_FIELDS[field], _dptr(out))
+        else:
+            out = np.empty(n)
+            lib().or_world_get(self._h, _SCALAR_FIELDS[field], _dptr(out))
+        return out
+
+    def step(self, dt: float, n: int = 1) -> None:
+        if n == 1:
+            lib().or_world_step(self._h, dt)
+        else:
+            lib().or_world_step_n(self._h, dt, int(n))
+
+    def density(self) -> None:
+        lib().or_world_density(self._h)
+
+    def forces(self) -> None:
+        lib().or_world_forces(self._h)
+
+    def validate(self) -> int:
+        return lib().or_world_validate(self._h)
+
+    def total_momentum(self) -> np.ndarray:
+        out = np.zeros(3)
+        lib().or_world_momentum(self._h, _dptr(out))
+        return out
+
+    def kinetic_energy(self) -> float:
+        return lib().or_world_kinetic_energy(self._h)
+
+    def neighbors(self, i: int) -> np.ndarray:
+        cap = 4096
+        out = np.zeros(cap, dtype=np.uint64)
+        n = lib().or_world_neighbors(self._h, i, out.ctypes.data_as(C.POINTER(C.c_uint64)), cap)
+        assert n <= cap
+        return out[:n].astype(np.int64)
+
+    def neighbor_stats(self):
+        n = len(self)
+        c = np.zeros(n, dtype=np.uint32)
+        s = np.zeros(n, dtype=np.uint64)
+        x = np.zeros(n, dtype=np.uint64)
+        lib().or_world_neighbor_stats(
+            self._h, c.ctypes.data_as(C.POINTER(C.c_uint32)),
+            s.ctypes.data_as(C.POINTER(C.c_uint64)), x.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return c, s, x
+
+    def time_steps(self, dt: float, n: int) -> float:
+        return lib().or_world_time_steps(self._h, dt, int(n))
+

@@ -1,0 +1,551 @@
+// sph_oracle.cpp — f64 CPU oracle for the phy-fluid SPH step.
+//
+// TEST INFRASTRUCTURE ONLY. This file is the checker, never the product: only
+// tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+// The shipped library (physics_amd/csrc) never links or calls it.
+//
+// It restates, in plain C++ f64, the reference solver of jojo240607/physics:
+//   crates/phy-fluid/src/sph/world.rs   FluidWorld::{new, fill_box, lattice_mass,
+//                                        step, compute_density_pressure,
+//                                        compute_forces, integrate, enforce_bounds}
+//   crates/phy-fluid/src/sph/grid.rs    Grid::{key_of, build, for_each_neighbor}
+//   crates/phy-fluid/src/kernels.rs     Kernels::{new, poly6, spiky_grad_mag, visc_lap}
+//   crates/phy-fluid/src/sph/params.rs  SphParams::defaults
+//   crates/phy-fluid/src/particle.rs    Particle::{new, with_material}
+//   crates/phy-fluid/src/subsystem.rs   FluidSubsystem::{validate, total_momentum,
+//                                        kinetic_energy}
+// with the reference's operation order (left-associative products, nalgebra's
+// (x*x + y*y) + z*z norm) and no FMA contraction (built with -ffp-contract=off).
+// The two parallel passes use OpenMP over particles where the reference uses
+// rayon par_iter + an order-preserving collect: every particle writes only its
+// own output slot from snapshot inputs, so results do not depend on the thread
+// count.
+//
+// Parity of the restatement itself is pinned by tests/test_oracle_*.py against
+// closed forms, brute-force neighbour sets, symmetry and the reference's own
+// test assertions; the ulp-level agreement with the Rust binary (nalgebra 0.33.3
+// norm order, libm powf) cannot be executed here (no Rust toolchain).
+//
+// A run-time "mutation" switch plants known mistakes so the tests can show that
+// each check actually fails on a plausible bug.
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <unordered_map>
+#include <vector>
+#include <algorithm>
+#include <chrono>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+struct V3 {
+    double x, y, z;
+};
+inline V3 v3(double x, double y, double z) { return V3{x, y, z}; }
+inline V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline V3 operator*(V3 a, double s) { return v3(a.x * s, a.y * s, a.z * s); }
+inline V3 operator/(V3 a, double s) { return v3(a.x / s, a.y / s, a.z / s); }
+inline V3& operator+=(V3& a, V3 b) { a = a + b; return a; }
+// nalgebra Vector3::norm_squared / norm: (x*x + y*y) + z*z, then sqrt.
+inline double norm_sq(V3 a) { return (a.x * a.x + a.y * a.y) + a.z * a.z; }
+inline double norm(V3 a) { return std::sqrt(norm_sq(a)); }
+
+// Planted-mistake switch used only by the oracle's own sensitivity tests.
+enum Mutation : int {
+    MUT_NONE = 0,
+    MUT_DROP_XSPH = 1,        // integrate without the XSPH velocity correction
+    MUT_VISC_SIGN = 2,        // (v_i - v_j) instead of (v_j - v_i) in viscosity
+    MUT_NO_PRESSURE_CLAMP = 3,// p = k (rho - rho0) without max(0, .)
+    MUT_SELF_IN_FORCE = 4,    // forget the j == i skip (adds nothing: r = 0 is
+                              // skipped anyway — kept to show a benign change)
+    MUT_DENSITY_NO_SELF = 5,  // drop the self term from the density sum
+    MUT_BOUNDS_NO_DAMP = 6,   // reflect without the damping factor
+    MUT_POLY6_H2 = 7,         // poly6 with (h - r) instead of (h^2 - r^2)
+    MUT_DT_IN_XSPH = 8,       // v += xsph * eps * dt instead of v += xsph * eps
+};
+int g_mutation = MUT_NONE;
+
+// crates/phy-fluid/src/sph/params.rs:12-50
+constexpr int kMaxMat = 16;
+struct Params {
+    double rest_density, stiffness, viscosity;
+    double visc_k[kMaxMat];
+    double visc_n[kMaxMat];
+    int32_t n_visc_k, n_visc_n;
+    double shear_min, mass, h;
+    double gravity[3], bounds_min[3], bounds_max[3];
+    double boundary_damp, xsph_eps;
+};
+
+// crates/phy-fluid/src/particle.rs:17-46
+struct Particle {
+    V3 pos, vel, acc, body_acc;
+    double rho, p, mass;
+    uint64_t material;
+    double mu_eff;
+    V3 xsph;
+};
+
+// crates/phy-fluid/src/kernels.rs:22-65
+struct Kernels {
+    double h, h2, poly6_coeff, spiky_grad_coeff, visc_lap_coeff;
+    explicit Kernels(double h_) {
+        const double pi = 3.141592653589793;  // std::f64::consts::PI
+        h = h_;
+        h2 = h * h;
+        double h3 = h2 * h;
+        double h6 = h3 * h3;
+        double h9 = h6 * h3;
+        poly6_coeff = 315.0 / (64.0 * pi * h9);
+        spiky_grad_coeff = -45.0 / (pi * h6);
+        visc_lap_coeff = 45.0 / (pi * h6);
+    }
+    double poly6(double r) const {
+        if (r >= h) return 0.0;
+        double diff = (g_mutation == MUT_POLY6_H2) ? (h - r) : (h2 - r * r);
+        return poly6_coeff * diff * diff * diff;
+    }
+    double spiky_grad_mag(double r) const {
+        if (r >= h || r <= 0.0) return 0.0;
+        double diff = h - r;
+        return spiky_grad_coeff * diff * diff;
+    }
+    double visc_lap(double r) const {
+        if (r >= h) return 0.0;
+        return visc_lap_coeff * (h - r);
+    }
+};
+
+// num-traits ToPrimitive::to_i64 for f64: None (-> unwrap_or(0)) for NaN and
+// for values outside [-2^63, 2^63).
+inline int64_t to_i64_or0(double v) {
+    if (v >= -9223372036854775808.0 && v < 9223372036854775808.0) return (int64_t)v;
+    return 0;
+}
+
+struct Key {
+    int64_t i, j, k;
+    bool operator==(const Key& o) const { return i == o.i && j == o.j && k == o.k; }
+};
+struct KeyHash {
+    size_t operator()(const Key& k) const {
+        uint64_t h = (uint64_t)k.i * 0x9E3779B97F4A7C15ull;
+        h ^= (uint64_t)k.j + 0xBF58476D1CE4E5B9ull + (h << 6) + (h >> 2);
+        h ^= (uint64_t)k.k + 0x94D049BB133111EBull + (h << 6) + (h >> 2);
+        return (size_t)h;
+    }
+};
+
+// crates/phy-fluid/src/sph/grid.rs:10-56
+struct Grid {
+    double cell;
+    std::unordered_map<Key, std::vector<size_t>, KeyHash> map;
+    explicit Grid(double c) : cell(c) {}
+    Key key_of(V3 p) const {
+        double inv = 1.0 / cell;  // T::one() / self.cell
+        return Key{to_i64_or0(std::floor(p.x * inv)), to_i64_or0(std::floor(p.y * inv)),
+                   to_i64_or0(std::floor(p.z * inv))};
+    }
+    void build(const std::vector<Particle>& ps) {
+        map.clear();
+        for (size_t i = 0; i < ps.size(); ++i) map[key_of(ps[i].pos)].push_back(i);
+    }
+    // di outermost, dk innermost; ascending index inside a cell.
+    template <class F>
+    void for_each_neighbor(V3 p, F&& f) const {
+        Key c = key_of(p);
+        for (int64_t di = -1; di <= 1; ++di)
+            for (int64_t dj = -1; dj <= 1; ++dj)
+                for (int64_t dk = -1; dk <= 1; ++dk) {
+                    Key q{(int64_t)((uint64_t)c.i + (uint64_t)di), (int64_t)((uint64_t)c.j + (uint64_t)dj),
+                          (int64_t)((uint64_t)c.k + (uint64_t)dk)};
+                    auto it = map.find(q);
+                    if (it == map.end()) continue;
+                    for (size_t id : it->second) f(id);
+                }
+    }
+};
+
+struct World {
+    Params params;
+    std::vector<Particle> particles;
+    Kernels kernels;
+    Grid grid;
+    explicit World(const Params& p) : params(p), kernels(p.h), grid(p.h) {}
+
+    // world.rs:148-169
+    double lattice_mass(double s) const {
+        double h = params.h;
+        int32_t n_shell = (int32_t)std::ceil(h / s) + 1;
+        double sum_w = 0.0;
+        for (int32_t i = -n_shell; i <= n_shell; ++i)
+            for (int32_t j = -n_shell; j <= n_shell; ++j)
+                for (int32_t l = -n_shell; l <= n_shell; ++l) {
+                    double rr = std::sqrt((double)(i * i + j * j + l * l));
+                    double r = s * rr;
+                    if (r < h) sum_w += kernels.poly6(r);
+                }
+        if (sum_w > 0.0) return params.rest_density / sum_w;
+        return params.mass;
+    }
+
+    // world.rs:124-144
+    void fill_box(V3 mn, V3 mx, double spacing, double margin) {
+        V3 lo = mn + v3(margin, margin, margin);
+        V3 hi = mx - v3(margin, margin, margin);
+        double d = spacing;
+        double mass = lattice_mass(d);
+        double x = lo.x;
+        while (x <= hi.x) {
+            double y = lo.y;
+            while (y <= hi.y) {
+                double z = lo.z;
+                while (z <= hi.z) {
+                    Particle pt{};
+                    pt.pos = v3(x, y, z);
+                    pt.mass = mass;
+                    particles.push_back(pt);
+                    z += d;
+                }
+                y += d;
+            }
+            x += d;
+        }
+    }
+
+    // world.rs:183-192
+    void step(double dt) {
+        if (particles.empty()) return;
+        grid.build(particles);
+        compute_density_pressure();
+        compute_forces();
+        integrate(dt);
+        enforce_bounds();
+    }
+
+    // world.rs:204-246
+    void compute_density_pressure() {
+        const size_t n = particles.size();
+        const double h = params.h, rest = params.rest_density, stiff = params.stiffness;
+        std::vector<V3> pos(n);
+        std::vector<double> mass(n), rho_out(n), p_out(n);
+        for (size_t i = 0; i < n; ++i) { pos[i] = particles[i].pos; mass[i] = particles[i].mass; }
+#pragma omp parallel for schedule(dynamic, 256)
+        for (long long ii = 0; ii < (long long)n; ++ii) {
+            size_t i = (size_t)ii;
+            V3 p_i = pos[i];
+            double rho = 0.0;
+            grid.for_each_neighbor(p_i, [&](size_t j) {
+                if (g_mutation == MUT_DENSITY_NO_SELF && j == i) return;
+                double r = norm(p_i - pos[j]);  // na_distance(&p_i, &pos[j])
+                if (r < h) rho += mass[j] * kernels.poly6(r);
+            });
+            double over = rho - rest;
+            double p = (over > 0.0) ? stiff * over : 0.0;
+            if (g_mutation == MUT_NO_PRESSURE_CLAMP) p = stiff * over;
+            rho_out[i] = rho;
+            p_out[i] = p;
+        }
+        for (size_t i = 0; i < n; ++i) { particles[i].rho = rho_out[i]; particles[i].p = p_out[i]; }
+    }
+
+    // world.rs:252-347
+    void compute_forces() {
+        const size_t n = particles.size();
+        const double h = params.h;
+        const double r_eps = 1e-4;
+        const double shear_min = params.shear_min;
+        std::vector<V3> pos(n), vel(n);
+        std::vector<double> rho(n), pr(n), mass(n);
+        std::vector<uint64_t> mat(n);
+        for (size_t i = 0; i < n; ++i) {
+            pos[i] = particles[i].pos; vel[i] = particles[i].vel; rho[i] = particles[i].rho;
+            pr[i] = particles[i].p; mass[i] = particles[i].mass; mat[i] = particles[i].material;
+        }
+        std::vector<V3> acc_out(n), xsph_out(n);
+        std::vector<double> mu_out(n);
+#pragma omp parallel for schedule(dynamic, 256)
+        for (long long ii = 0; ii < (long long)n; ++ii) {
+            size_t i = (size_t)ii;
+            V3 p_i = pos[i], v_i = vel[i];
+            double rho_i = rho[i], p_i_p = pr[i], m_i = mass[i];
+            V3 f_press{0, 0, 0}, f_visc_raw{0, 0, 0}, xsph{0, 0, 0};
+            double shear = 0.0;
+            grid.for_each_neighbor(p_i, [&](size_t j) {
+                if (j == i && g_mutation != MUT_SELF_IN_FORCE) return;
+                V3 d = pos[j] - p_i;
+                double r = norm(d);
+                if (r >= h || r <= 0.0) return;  // NaN r falls through, as in the reference
+                double rho_j = rho[j], p_j = pr[j];
+                double grad = kernels.spiky_grad_mag(r);
+                V3 dir = d / r;
+                double coef = m_i * mass[j] * (p_i_p / (rho_i * rho_i) + p_j / (rho_j * rho_j));
+                f_press += dir * (coef * grad);
+                double lap = kernels.visc_lap(r);
+                V3 dvel = (g_mutation == MUT_VISC_SIGN) ? (v_i - vel[j]) : (vel[j] - v_i);
+                f_visc_raw += dvel * (m_i * mass[j] / rho_j * lap);
+                double w = kernels.poly6(r);
+                xsph += (vel[j] - v_i) * (mass[j] / rho_j * w);
+                double dv = norm(vel[j] - v_i);
+                shear += dv / (r + r_eps) * (mass[j] / rho_j);
+            });
+            int n_k = params.n_visc_k;
+            size_t m_idx = (mat[i] < (uint64_t)n_k) ? (size_t)mat[i] : 0;
+            double kc = params.visc_k[m_idx];
+            double nn = params.visc_n[m_idx];
+            double sreg = (shear > shear_min) ? shear : shear_min;
+            double mu_eff = kc * std::pow(sreg, nn - 1.0);
+            V3 f_visc = f_visc_raw * mu_eff;
+            V3 acc = (rho_i > 0.0) ? (f_press + f_visc) / rho_i : v3(0, 0, 0);
+            acc_out[i] = acc;
+            mu_out[i] = mu_eff;
+            xsph_out[i] = xsph;
+        }
+        for (size_t i = 0; i < n; ++i) {
+            particles[i].acc = acc_out[i];
+            particles[i].mu_eff = mu_out[i];
+            particles[i].xsph = xsph_out[i];
+        }
+    }
+
+    // world.rs:352-363
+    void integrate(double dt) {
+        V3 g = v3(params.gravity[0], params.gravity[1], params.gravity[2]);
+        double eps = params.xsph_eps;
+        for (auto& pt : particles) {
+            V3 a = pt.acc + g + pt.body_acc;
+            pt.vel += a * dt;
+            if (g_mutation == MUT_DT_IN_XSPH) pt.vel += pt.xsph * eps * dt;
+            else if (g_mutation != MUT_DROP_XSPH) pt.vel += pt.xsph * eps;
+            pt.pos += pt.vel * dt;
+            pt.body_acc = v3(0, 0, 0);
+        }
+    }
+
+    // world.rs:366-397
+    void enforce_bounds() {
+        const double* lo = params.bounds_min;
+        const double* hi = params.bounds_max;
+        double damp = (g_mutation == MUT_BOUNDS_NO_DAMP) ? 1.0 : params.boundary_damp;
+        for (auto& pt : particles) {
+            double* p = &pt.pos.x;
+            double* v = &pt.vel.x;
+            for (int a = 0; a < 3; ++a) {
+                if (p[a] < lo[a]) { p[a] = lo[a]; v[a] = -v[a] * damp; }
+                else if (p[a] > hi[a]) { p[a] = hi[a]; v[a] = -v[a] * damp; }
+            }
+        }
+    }
+};
+
+// params.rs:54-85
+void params_defaults(Params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->rest_density = 1000.0;
+    p->stiffness = 250.0;
+    p->viscosity = 3.5;
+    p->visc_k[0] = 3.5;
+    p->visc_n[0] = 1.0;
+    p->n_visc_k = 1;
+    p->n_visc_n = 1;
+    p->shear_min = 0.01;
+    p->mass = 0.02;
+    p->h = 0.2;
+    p->gravity[0] = 0.0; p->gravity[1] = -9.81; p->gravity[2] = 0.0;
+    for (int a = 0; a < 3; ++a) { p->bounds_min[a] = -5.0; p->bounds_max[a] = 5.0; }
+    p->boundary_damp = 0.4;
+    p->xsph_eps = 0.5;
+}
+
+}  // namespace
+
+extern "C" {
+
+typedef Params OrParams;
+
+int or_params_size(void) { return (int)sizeof(Params); }
+void or_params_defaults(OrParams* p) { params_defaults(p); }
+void or_set_mutation(int m) { g_mutation = m; }
+int or_get_mutation(void) { return g_mutation; }
+
+void or_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+int or_get_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void or_kernels(double h, double* out3, double r, double* vals3) {
+    Kernels k(h);
+    out3[0] = k.poly6_coeff;
+    out3[1] = k.spiky_grad_coeff;
+    out3[2] = k.visc_lap_coeff;
+    if (vals3) {
+        vals3[0] = k.poly6(r);
+        vals3[1] = k.spiky_grad_mag(r);
+        vals3[2] = k.visc_lap(r);
+    }
+}
+
+void* or_world_new(const OrParams* p) { return new World(*p); }
+void or_world_free(void* w) { delete (World*)w; }
+void* or_world_clone(void* w) { return new World(*(World*)w); }
+
+void or_world_set_params(void* w, const OrParams* p) {
+    World* W = (World*)w;
+    W->params = *p;
+    W->kernels = Kernels(p->h);
+    W->grid = Grid(p->h);
+}
+
+size_t or_world_len(void* w) { return ((World*)w)->particles.size(); }
+double or_world_lattice_mass(void* w, double s) { return ((World*)w)->lattice_mass(s); }
+void or_world_fill_box(void* w, const double* mn, const double* mx, double spacing, double margin) {
+    ((World*)w)->fill_box(v3(mn[0], mn[1], mn[2]), v3(mx[0], mx[1], mx[2]), spacing, margin);
+}
+
+// Replace the particle list. Null arrays leave the field at Particle::new's
+// default (zero velocity, zero body_acc, material 0).
+void or_world_set_particles(void* w, size_t n, const double* pos, const double* vel,
+                            const double* mass, const uint64_t* material, const double* body_acc) {
+    World* W = (World*)w;
+    W->particles.assign(n, Particle{});
+    for (size_t i = 0; i < n; ++i) {
+        Particle& p = W->particles[i];
+        p.pos = v3(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        if (vel) p.vel = v3(vel[3 * i], vel[3 * i + 1], vel[3 * i + 2]);
+        p.mass = mass ? mass[i] : W->params.mass;
+        if (material) p.material = material[i];
+        if (body_acc) p.body_acc = v3(body_acc[3 * i], body_acc[3 * i + 1], body_acc[3 * i + 2]);
+    }
+}
+
+// field: 0 pos, 1 vel, 2 acc, 3 body_acc, 4 xsph (3 per particle);
+//        5 rho, 6 p, 7 mass, 8 mu_eff (1 per particle)
+void or_world_get(void* w, int field, double* out) {
+    World* W = (World*)w;
+    size_t n = W->particles.size();
+    for (size_t i = 0; i < n; ++i) {
+        const Particle& p = W->particles[i];
+        const V3* v = nullptr;
+        switch (field) {
+            case 0: v = &p.pos; break;
+            case 1: v = &p.vel; break;
+            case 2: v = &p.acc; break;
+            case 3: v = &p.body_acc; break;
+            case 4: v = &p.xsph; break;
+            case 5: out[i] = p.rho; break;
+            case 6: out[i] = p.p; break;
+            case 7: out[i] = p.mass; break;
+            case 8: out[i] = p.mu_eff; break;
+            default: break;
+        }
+        if (v) { out[3 * i] = v->x; out[3 * i + 1] = v->y; out[3 * i + 2] = v->z; }
+    }
+}
+
+void or_world_step(void* w, double dt) { ((World*)w)->step(dt); }
+
+void or_world_step_n(void* w, double dt, int n) {
+    for (int i = 0; i < n; ++i) ((World*)w)->step(dt);
+}
+
+// Individual stages, for stage-level parity (the grid is rebuilt by density).
+void or_world_density(void* w) {
+    World* W = (World*)w;
+    W->grid.build(W->particles);
+    W->compute_density_pressure();
+}
+void or_world_forces(void* w) { ((World*)w)->compute_forces(); }
+
+// subsystem.rs:74-90 — 0 ok, 1 NonFinite{pos}, 2 NonFinite{vel}
+int or_world_validate(void* w) {
+    for (const auto& p : ((World*)w)->particles) {
+        if (!std::isfinite(p.pos.x) || !std::isfinite(p.pos.y) || !std::isfinite(p.pos.z)) return 1;
+        if (!std::isfinite(p.vel.x) || !std::isfinite(p.vel.y) || !std::isfinite(p.vel.z)) return 2;
+    }
+    return 0;
+}
+
+// subsystem.rs:149-155
+void or_world_momentum(void* w, double* out3) {
+    V3 s{0, 0, 0};
+    for (const auto& p : ((World*)w)->particles) s += p.vel * p.mass;
+    out3[0] = s.x; out3[1] = s.y; out3[2] = s.z;
+}
+// subsystem.rs:157-163
+double or_world_kinetic_energy(void* w) {
+    double e = 0.0;
+    for (const auto& p : ((World*)w)->particles) e += (norm_sq(p.vel) * p.mass) / (1.0 + 1.0);
+    return e;
+}
+
+// Density-pass neighbour set of particle i (j with r < h, self included), in the
+// reference's visiting order. Returns the count; writes at most cap indices.
+size_t or_world_neighbors(void* w, size_t i, uint64_t* out, size_t cap) {
+    World* W = (World*)w;
+    W->grid.build(W->particles);
+    size_t cnt = 0;
+    V3 p_i = W->particles[i].pos;
+    W->grid.for_each_neighbor(p_i, [&](size_t j) {
+        double r = norm(p_i - W->particles[j].pos);
+        if (r < W->params.h) {
+            if (cnt < cap) out[cnt] = j;
+            ++cnt;
+        }
+    });
+    return cnt;
+}
+
+// Per-particle neighbour statistics over the whole world (density predicate
+// r < h, self included): count and two order-independent checksums of the
+// neighbours' indices, for exact set comparison without storing the sets.
+void or_world_neighbor_stats(void* w, uint32_t* counts, uint64_t* sum_ids, uint64_t* xor_hash) {
+    World* W = (World*)w;
+    W->grid.build(W->particles);
+    const size_t n = W->particles.size();
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)n; ++ii) {
+        size_t i = (size_t)ii;
+        V3 p_i = W->particles[i].pos;
+        uint32_t c = 0;
+        uint64_t s = 0, x = 0;
+        W->grid.for_each_neighbor(p_i, [&](size_t j) {
+            double r = norm(p_i - W->particles[j].pos);
+            if (r < W->params.h) {
+                ++c;
+                s += (uint64_t)j;
+                uint64_t z = (uint64_t)j + 0x9E3779B97F4A7C15ull;
+                z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+                z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+                x ^= z ^ (z >> 31);
+            }
+        });
+        counts[i] = c;
+        sum_ids[i] = s;
+        xor_hash[i] = x;
+    }
+}
+
+// Wall-clock of n steps (seconds), for the CPU baseline.
+double or_world_time_steps(void* w, double dt, int n) {
+    auto t0 = std::chrono::steady_clock::now();
+    for (int i = 0; i < n; ++i) ((World*)w)->step(dt);
+    auto t1 = std::chrono::steady_clock::now();
+    return std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // extern "C"

@@ -1,0 +1,423 @@
+// fluid_device.cpp — DeviceFluid: buffers, uploads and the step sequence.
+#include "fluid_device.h"
+
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+
+namespace phy {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// to_i64 of the floored key, as in the kernels (grid.rs:26-33)
+static long long host_key(float x, double inv) {
+    double f = std::floor((double)x * inv);
+    if (f >= -9223372036854775808.0 && f < 9223372036854775808.0) return (long long)f;
+    return 0;
+}
+
+DevParams make_dev_params(const HostParams& hp) {
+    DevParams P;
+    memset(&P, 0, sizeof(P));
+    const double h = hp.h;
+    P.h64 = h;
+    P.inv_cell = 1.0 / h;
+    // smallest f64 s with sqrt(s) >= h: the reference's r < h is s < s_thresh
+    double s = h * h;
+    if (std::isfinite(s) && h > 0.0) {
+        for (int it = 0; it < 64 && std::sqrt(std::nextafter(s, 0.0)) >= h; ++it)
+            s = std::nextafter(s, 0.0);
+        for (int it = 0; it < 64 && std::sqrt(s) < h; ++it) s = std::nextafter(s, INFINITY);
+    }
+    P.s_thresh = s;
+    const double marg = std::ldexp(1.0, -19);
+    P.s_lo = (float)(s * (1.0 - marg));
+    P.s_hi = (float)(s * (1.0 + marg));
+    if ((double)P.s_lo >= s * (1.0 - marg)) P.s_lo = std::nextafter(P.s_lo, 0.0f);
+    if ((double)P.s_hi <= s * (1.0 + marg)) P.s_hi = std::nextafter(P.s_hi, INFINITY);
+    if (!(h > 0.0)) {  // degenerate support radius: only the exact path decides
+        P.s_lo = -INFINITY;
+        P.s_hi = INFINITY;
+    }
+    // kernels.rs:22-35 in f64
+    const double pi = 3.141592653589793;
+    double h2 = h * h, h3 = h2 * h, h6 = h3 * h3, h9 = h6 * h3;
+    P.h = (float)h;
+    P.h2 = (float)h2;
+    P.c6 = (float)(315.0 / (64.0 * pi * h9));
+    P.cs = (float)(-45.0 / (pi * h6));
+    P.cv = (float)(45.0 / (pi * h6));
+    P.rest = (float)hp.rest_density;
+    P.stiff = (float)hp.stiffness;
+    P.shear_min = (float)hp.shear_min;
+    P.r_eps = (float)1e-4;
+    for (int a = 0; a < 3; ++a) {
+        P.g[a] = (float)hp.gravity[a];
+        P.lo[a] = (float)hp.bounds_min[a];
+        P.hi[a] = (float)hp.bounds_max[a];
+    }
+    P.damp = (float)hp.boundary_damp;
+    P.xsph_eps = (float)hp.xsph_eps;
+    if (hp.visc_k.empty() || hp.visc_n.empty())
+        throw std::invalid_argument("SphParams: visc_k and visc_n must be non-empty");
+    if (hp.visc_k.size() > (size_t)kMaxMaterials || hp.visc_n.size() > (size_t)kMaxMaterials)
+        throw std::invalid_argument("SphParams: at most 16 materials");
+    // world.rs:319-325: m < visc_k.len() ? m : 0, then visc_n[m] (panics in the
+    // reference when visc_n is shorter; rejected here up front)
+    if (hp.visc_n.size() < hp.visc_k.size())
+        throw std::invalid_argument("SphParams: visc_n shorter than visc_k");
+    P.n_mat = (int32_t)hp.visc_k.size();
+    P.newtonian = 1;
+    for (size_t m = 0; m < hp.visc_k.size(); ++m) {
+        P.visc_k[m] = (float)hp.visc_k[m];
+        P.visc_n[m] = (float)hp.visc_n[m];
+        if (hp.visc_n[m] != 1.0) P.newtonian = 0;
+    }
+    return P;
+}
+
+DeviceFluid::DeviceFluid(const HostParams& hp) : hp_(hp) {
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count <= 0)
+        throw HipError("phy-fluid: no HIP device available (this library runs on gfx950 only)");
+    hip_check(hipGetDevice(&dev_), "hipGetDevice");
+    hip_check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate");
+    P_ = make_dev_params(hp_);
+    dom_.ensure(1);
+    diag_part_.ensure((size_t)kDiagBlocks * 5);
+    diag_out_.ensure(5);
+    partial_.ensure(kScanBlocks);
+    for (int a = 0; a < 3; ++a) {
+        bkey_[a] = host_key(P_.lo[a], P_.inv_cell);
+        bkey_[3 + a] = host_key(P_.hi[a], P_.inv_cell);
+    }
+}
+
+DeviceFluid::~DeviceFluid() {
+    for (hipEvent_t e : prof_ev_) (void)hipEventDestroy(e);
+    if (st_) {
+        (void)hipStreamSynchronize(st_);
+        (void)hipStreamDestroy(st_);
+    }
+}
+
+void DeviceFluid::set_params(const HostParams& hp) {
+    P_ = make_dev_params(hp);
+    hp_ = hp;
+    for (int a = 0; a < 3; ++a) {
+        bkey_[a] = host_key(P_.lo[a], P_.inv_cell);
+        bkey_[3 + a] = host_key(P_.hi[a], P_.inv_cell);
+    }
+    // the dense table may need to grow for a new bounds box
+    double vol = 1.0;
+    for (int a = 0; a < 3; ++a)
+        vol *= (double)((unsigned long long)bkey_[3 + a] - (unsigned long long)bkey_[a]) + 1.0;
+    unsigned long long need = vol >= (double)kMaxDenseCells ? kMaxDenseCells : (unsigned long long)vol;
+    if (need > cell_cap_) {
+        cell_cap_ = std::min<unsigned long long>(need, kMaxDenseCells);
+        counts_.release();
+        start_.release();
+        counts_.ensure(cell_cap_ + 1);
+        start_.ensure(cell_cap_ + 2);
+        hip_check(hipMemsetAsync(counts_.get(), 0, (cell_cap_ + 1) * sizeof(uint32_t), st_),
+                  "memset counts");
+    }
+    dens_valid_ = aux_valid_ = false;
+}
+
+void DeviceFluid::sync() { hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize"); }
+
+void DeviceFluid::ensure_capacity(size_t n) {
+    if (n <= cap_n_) return;
+    size_t c = std::max<size_t>(n, 1);
+    posm_.ensure(c); vel_.ensure(c); id_.ensure(c); mat_.ensure(c); body_.ensure(c);
+    posm_s_.ensure(c); velm_s_.ensure(c); body_s_.ensure(c);
+    id_s_.ensure(c); mat_s_.ensure(c); cell_s_.ensure(c);
+    rho_.ensure(c); prs_.ensure(c); mp_.ensure(c);
+    cell_.ensure(c); rank_.ensure(c); tmp_.ensure(c); idtmp_.ensure(c);
+    stage4_.ensure(c); stage1_.ensure(c);
+    if (keep_aux_) { acc_.ensure(c); xsph_.ensure(c); mu_.ensure(c); }
+    cap_n_ = c;
+}
+
+// Dense table capacity: cells of the bounds key box (every finite particle is
+// inside it after one step) and of the uploaded particles' key box.
+void DeviceFluid::compute_cell_cap(const float* pos3, size_t n) {
+    long long lo[3], hi[3];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = bkey_[a];
+        hi[a] = bkey_[3 + a];
+    }
+    unsigned long long need_b = 1, need_p = 1;
+    {
+        double v = 1.0;
+        for (int a = 0; a < 3; ++a)
+            v *= (double)((unsigned long long)hi[a] - (unsigned long long)lo[a]) + 1.0;
+        need_b = v >= (double)kMaxDenseCells ? kMaxDenseCells : (unsigned long long)v;
+    }
+    long long plo[3] = {LLONG_MAX, LLONG_MAX, LLONG_MAX}, phi[3] = {LLONG_MIN, LLONG_MIN, LLONG_MIN};
+    bool any = false;
+    for (size_t i = 0; i < n; ++i) {
+        const float* p = pos3 + 3 * i;
+        if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) continue;
+        any = true;
+        for (int a = 0; a < 3; ++a) {
+            long long k = host_key(p[a], P_.inv_cell);
+            plo[a] = std::min(plo[a], k);
+            phi[a] = std::max(phi[a], k);
+        }
+    }
+    if (any) {
+        double v = 1.0;
+        for (int a = 0; a < 3; ++a) v *= (double)((unsigned long long)phi[a] - (unsigned long long)plo[a]) + 1.0;
+        need_p = v >= (double)kMaxDenseCells ? kMaxDenseCells : (unsigned long long)v;
+    }
+    unsigned long long need = std::max(need_b, need_p);
+    need = std::min<unsigned long long>(std::max<unsigned long long>(need, 1), kMaxDenseCells);
+    if (need > cell_cap_ || !counts_.get()) {
+        cell_cap_ = std::max(need, cell_cap_);
+        counts_.release();
+        start_.release();
+        counts_.ensure(cell_cap_ + 1);
+        start_.ensure(cell_cap_ + 2);
+        hip_check(hipMemsetAsync(counts_.get(), 0, (cell_cap_ + 1) * sizeof(uint32_t), st_),
+                  "memset counts");
+    }
+}
+
+void DeviceFluid::upload(size_t n, const float* pos3, const float* vel3, const float* mass,
+                         const uint32_t* material, const float* body3) {
+    if (n >= 0x7fffffffull) throw std::invalid_argument("phy-fluid: at most 2^31-1 particles");
+    ensure_capacity(n);
+    compute_cell_cap(pos3, n);
+    std::vector<float4> p4(n), v4(n);
+    std::vector<uint32_t> ids(n), mats(n);
+    for (size_t i = 0; i < n; ++i) {
+        p4[i] = make_float4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], mass ? mass[i] : (float)hp_.mass);
+        v4[i] = vel3 ? make_float4(vel3[3 * i], vel3[3 * i + 1], vel3[3 * i + 2], 0.f)
+                     : make_float4(0.f, 0.f, 0.f, 0.f);
+        ids[i] = (uint32_t)i;
+        mats[i] = material ? material[i] : 0u;
+    }
+    n_ = n;
+    if (n) {
+        hip_check(hipMemcpyAsync(posm_.get(), p4.data(), n * sizeof(float4), hipMemcpyHostToDevice, st_), "upload pos");
+        hip_check(hipMemcpyAsync(vel_.get(), v4.data(), n * sizeof(float4), hipMemcpyHostToDevice, st_), "upload vel");
+        hip_check(hipMemcpyAsync(id_.get(), ids.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st_), "upload id");
+        hip_check(hipMemcpyAsync(mat_.get(), mats.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st_), "upload mat");
+    }
+    has_body_ = false;
+    if (body3 && n) {
+        std::vector<float4> b4(n);
+        for (size_t i = 0; i < n; ++i) {
+            b4[i] = make_float4(body3[3 * i], body3[3 * i + 1], body3[3 * i + 2], 0.f);
+            if (b4[i].x != 0.f || b4[i].y != 0.f || b4[i].z != 0.f) has_body_ = true;
+        }
+        if (has_body_)
+            hip_check(hipMemcpyAsync(body_.get(), b4.data(), n * sizeof(float4), hipMemcpyHostToDevice, st_), "upload body");
+    }
+    sync();  // host vectors go out of scope
+    dens_valid_ = aux_valid_ = false;
+}
+
+void DeviceFluid::set_keep_aux(bool on) {
+    keep_aux_ = on;
+    if (on && cap_n_) {
+        acc_.ensure(cap_n_);
+        xsph_.ensure(cap_n_);
+        mu_.ensure(cap_n_);
+    }
+    if (!on) aux_valid_ = false;
+}
+
+void DeviceFluid::run_domain_and_sort() {
+    const int n = (int)n_;
+    launch_domain(st_, n, posm_.get(), P_.inv_cell, dom_.get(), cell_cap_, bkey_);
+    SortBuffers b;
+    b.posm_in = posm_.get();
+    b.vel_in = vel_.get();
+    b.id_in = id_.get();
+    b.mat_in = P_.n_mat > 1 ? mat_.get() : nullptr;
+    b.body_in = has_body_ ? body_.get() : nullptr;
+    b.cell = cell_.get();
+    b.rank = rank_.get();
+    b.counts = counts_.get();
+    b.partial = partial_.get();
+    b.start = start_.get();
+    b.tmp = tmp_.get();
+    b.idtmp = idtmp_.get();
+    b.posm_out = posm_s_.get();
+    b.velm_out = velm_s_.get();
+    b.id_out = id_s_.get();
+    b.mat_out = mat_s_.get();
+    b.body_out = body_s_.get();
+    b.cell_out = cell_s_.get();
+    launch_sort(st_, n, b, dom_.get(), P_.inv_cell);
+}
+
+hipEvent_t DeviceFluid::ev() {
+    hipEvent_t e;
+    hip_check(hipEventCreate(&e), "hipEventCreate");
+    hip_check(hipEventRecord(e, st_), "hipEventRecord");
+    prof_ev_.push_back(e);
+    return e;
+}
+
+void DeviceFluid::profile_begin() {
+    sync();
+    for (hipEvent_t e : prof_ev_) (void)hipEventDestroy(e);
+    prof_ev_.clear();
+    prof_ = true;
+}
+
+uint32_t DeviceFluid::profile_end(double ms4[4]) {
+    sync();
+    for (int k = 0; k < 4; ++k) ms4[k] = 0.0;
+    uint32_t steps = (uint32_t)(prof_ev_.size() / 4);
+    for (uint32_t s = 0; s < steps; ++s) {
+        hipEvent_t* e = &prof_ev_[4 * s];
+        float a = 0, b = 0, c = 0;
+        hip_check(hipEventElapsedTime(&a, e[0], e[1]), "elapsed");
+        hip_check(hipEventElapsedTime(&b, e[1], e[2]), "elapsed");
+        hip_check(hipEventElapsedTime(&c, e[2], e[3]), "elapsed");
+        ms4[0] += a;
+        ms4[1] += b;
+        ms4[2] += c;
+        ms4[3] += (double)a + b + c;
+    }
+    for (hipEvent_t e : prof_ev_) (void)hipEventDestroy(e);
+    prof_ev_.clear();
+    prof_ = false;
+    return steps;
+}
+
+void DeviceFluid::step(float dt) {
+    if (n_ == 0) return;  // world.rs:184-186
+    const int n = (int)n_;
+    if (prof_) ev();
+    run_domain_and_sort();
+    if (prof_) ev();
+    launch_density(st_, n, posm_s_.get(), cell_s_.get(), start_.get(), dom_.get(), P_, rho_.get(),
+                   prs_.get(), velm_s_.get(), mp_.get());
+    ForceArgs a;
+    a.posm = posm_s_.get();
+    a.velm = velm_s_.get();
+    a.rho = rho_.get();
+    a.prs = prs_.get();
+    a.mp = mp_.get();
+    a.cell = cell_s_.get();
+    a.start = start_.get();
+    a.id = id_s_.get();
+    a.mat = P_.n_mat > 1 ? mat_s_.get() : nullptr;
+    a.body = has_body_ ? body_s_.get() : nullptr;
+    a.posm_out = posm_.get();
+    a.vel_out = vel_.get();
+    a.acc_out = keep_aux_ ? acc_.get() : nullptr;
+    a.mu_out = keep_aux_ ? mu_.get() : nullptr;
+    a.xsph_out = keep_aux_ ? xsph_.get() : nullptr;
+    if (prof_) ev();
+    launch_force(st_, n, dt, a, dom_.get(), P_);
+    if (prof_) ev();
+    hip_check(hipGetLastError(), "step launch");
+    // the new state is in cell order: ids and materials follow the sort
+    id_.swap(id_s_);
+    mat_.swap(mat_s_);
+    has_body_ = false;  // integrate zeroes body_acc (world.rs:361)
+    dens_valid_ = true;
+    aux_valid_ = keep_aux_;
+}
+
+void DeviceFluid::download(Field f, float* out) {
+    const int n = (int)n_;
+    if (n == 0) return;
+    const uint32_t* id = id_.get();
+    // rho/p/aux belong to the sorted order of the last step, which equals the
+    // current order (the force pass writes the new state in that order)
+    switch (f) {
+        case F_POS:
+        case F_VEL:
+        case F_ACC:
+        case F_XSPH:
+        case F_MASS: {
+            const float4* src = f == F_POS || f == F_MASS ? posm_.get()
+                                : f == F_VEL              ? vel_.get()
+                                : f == F_ACC              ? acc_.get()
+                                                          : xsph_.get();
+            if ((f == F_ACC || f == F_XSPH) && !aux_valid_)
+                throw std::logic_error("phy-fluid: acc/xsph need keep_aux before the step");
+            launch_unsort4(st_, n, id, src, stage4_.get());
+            std::vector<float4> h(n);
+            hip_check(hipMemcpyAsync(h.data(), stage4_.get(), n * sizeof(float4), hipMemcpyDeviceToHost, st_), "download");
+            sync();
+            for (int i = 0; i < n; ++i) {
+                if (f == F_MASS) {
+                    out[i] = h[i].w;
+                } else {
+                    out[3 * i] = h[i].x;
+                    out[3 * i + 1] = h[i].y;
+                    out[3 * i + 2] = h[i].z;
+                }
+            }
+            return;
+        }
+        case F_RHO:
+        case F_P:
+        case F_MU_EFF: {
+            if (f == F_MU_EFF && !aux_valid_)
+                throw std::logic_error("phy-fluid: mu_eff needs keep_aux before the step");
+            if ((f == F_RHO || f == F_P) && !dens_valid_)
+                throw std::logic_error("phy-fluid: rho/p are defined after a step");
+            const float* src = f == F_RHO ? rho_.get() : f == F_P ? prs_.get() : mu_.get();
+            launch_unsort1(st_, n, id, src, stage1_.get());
+            hip_check(hipMemcpyAsync(out, stage1_.get(), n * sizeof(float), hipMemcpyDeviceToHost, st_), "download");
+            sync();
+            return;
+        }
+    }
+    throw std::invalid_argument("phy-fluid: unknown field");
+}
+
+int DeviceFluid::last_validate() {
+    if (n_ == 0) return 0;
+    Domain d;
+    hip_check(hipMemcpyAsync(&d, dom_.get(), sizeof(Domain), hipMemcpyDeviceToHost, st_), "watchdog");
+    sync();
+    if (d.bad_pos == 0xffffffffu && d.bad_vel == 0xffffffffu) return 0;
+    // first particle in index order with a bad field; pos is checked before vel
+    return d.bad_pos <= d.bad_vel ? 1 : 2;
+}
+
+void DeviceFluid::diagnostics(double out5[5]) {
+    for (int k = 0; k < 5; ++k) out5[k] = 0.0;
+    if (n_ == 0) return;
+    launch_diag(st_, (int)n_, posm_.get(), vel_.get(), diag_part_.get(), diag_out_.get());
+    hip_check(hipMemcpyAsync(out5, diag_out_.get(), 5 * sizeof(double), hipMemcpyDeviceToHost, st_), "diag");
+    sync();
+}
+
+void DeviceFluid::neighbor_stats(uint32_t* cnt, uint64_t* sum, uint64_t* xr) {
+    const int n = (int)n_;
+    if (n == 0) return;
+    // sort the current state (the same binning as a step), then count
+    run_domain_and_sort();
+    DevBuf<uint32_t> c;
+    DevBuf<unsigned long long> s, x;
+    c.ensure(n);
+    s.ensure(n);
+    x.ensure(n);
+    launch_neighbor_stats(st_, n, posm_s_.get(), cell_s_.get(), start_.get(), id_s_.get(), dom_.get(),
+                          P_, c.get(), s.get(), x.get());
+    hip_check(hipMemcpyAsync(cnt, c.get(), n * sizeof(uint32_t), hipMemcpyDeviceToHost, st_), "nstats");
+    hip_check(hipMemcpyAsync(sum, s.get(), n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_), "nstats");
+    hip_check(hipMemcpyAsync(xr, x.get(), n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_), "nstats");
+    sync();
+}
+
+}  // namespace phy

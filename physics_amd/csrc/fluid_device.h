@@ -1,0 +1,175 @@
+// fluid_device.h — device state and step sequencing of one FluidWorld on one GPU.
+//
+// Owns every HBM buffer of the world, one HIP stream (default priority), and
+// runs FluidWorld::step (crates/phy-fluid/src/sph/world.rs:183-192) as
+// domain -> counting sort -> density -> fused force/integrate/bounds.
+// Particle data stays resident in cell order between steps; host transfers
+// happen only on explicit upload / download calls.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <utility>
+#include <string>
+#include <vector>
+
+#include "sph_kernels.h"
+#include "sph_types.h"
+
+namespace phy {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what);
+
+// Device buffer of T elements (hipMalloc / hipFree).
+template <class T>
+class DevBuf {
+   public:
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void ensure(size_t n) {
+        if (n <= cap_) return;
+        release();
+        void* p = nullptr;
+        hip_check(hipMalloc(&p, (n ? n : 1) * sizeof(T)), "hipMalloc");
+        p_ = (T*)p;
+        cap_ = n;
+    }
+    void release() {
+        if (p_) (void)hipFree(p_);
+        p_ = nullptr;
+        cap_ = 0;
+    }
+    T* get() const { return p_; }
+    size_t cap() const { return cap_; }
+    void swap(DevBuf& o) {
+        std::swap(p_, o.p_);
+        std::swap(cap_, o.cap_);
+    }
+
+   private:
+    T* p_ = nullptr;
+    size_t cap_ = 0;
+};
+
+// Host-side f64 parameters, the SphParams<f64> of the reference
+// (crates/phy-fluid/src/sph/params.rs:12-50), flattened.
+struct HostParams {
+    double rest_density, stiffness, viscosity;
+    std::vector<double> visc_k, visc_n;
+    double shear_min, mass, h;
+    double gravity[3], bounds_min[3], bounds_max[3];
+    double boundary_damp, xsph_eps;
+};
+
+// Derives the device constants: f64 neighbour threshold and FP32 coefficients
+// (kernels.rs:22-35 computed in f64, rounded once).
+DevParams make_dev_params(const HostParams& hp);
+
+// Particle fields the device can hand back in original index order.
+enum Field : int {
+    F_POS = 0,
+    F_VEL = 1,
+    F_ACC = 2,
+    F_XSPH = 3,
+    F_RHO = 4,
+    F_P = 5,
+    F_MASS = 6,
+    F_MU_EFF = 7,
+};
+
+class DeviceFluid {
+   public:
+    explicit DeviceFluid(const HostParams& hp);
+    ~DeviceFluid();
+    DeviceFluid(const DeviceFluid&) = delete;
+    DeviceFluid& operator=(const DeviceFluid&) = delete;
+
+    void set_params(const HostParams& hp);
+    const HostParams& params() const { return hp_; }
+    size_t size() const { return n_; }
+
+    // Replace all particles (original index order). Null vel / material /
+    // body_acc mean zero / 0 / zero. Positions and velocities are rounded to FP32.
+    void upload(size_t n, const float* pos3, const float* vel3, const float* mass,
+                const uint32_t* material, const float* body3);
+
+    // Advance one step of dt (FP32 on the device).
+    void step(float dt);
+
+    // Copy a field to host memory in original index order (3 floats per
+    // particle for vector fields, 1 otherwise). Auxiliary fields (acc, xsph,
+    // mu_eff) hold the last step's values only while keep_aux is on.
+    void download(Field f, float* out);
+
+    void set_keep_aux(bool on);
+    bool keep_aux() const { return keep_aux_; }
+    bool aux_valid() const { return aux_valid_; }
+    bool density_valid() const { return dens_valid_; }
+
+    // Watchdog result of the last step (subsystem.rs:74-90): 0 ok, 1 pos, 2 vel.
+    int last_validate();
+    // FP64 mass, momentum (3) and kinetic energy of the current state.
+    void diagnostics(double out5[5]);
+    // Neighbour statistics of the density predicate on the current state
+    // (original order): count, sum of neighbour indices, xor of hashed indices.
+    void neighbor_stats(uint32_t* cnt, uint64_t* sum, uint64_t* xr);
+
+    hipStream_t stream() const { return st_; }
+    void sync();
+
+    // Per-stage HIP-event timing on this world's stream: while enabled, every
+    // step records events around sort (domain + counting sort), density and
+    // force; profile_end() synchronises and returns the summed milliseconds
+    // {sort, density, force, whole step} and the number of steps timed.
+    void profile_begin();
+    uint32_t profile_end(double ms4[4]);
+    int device() const { return dev_; }
+
+   private:
+    void ensure_capacity(size_t n);
+    void compute_cell_cap(const float* pos3, size_t n);
+    void run_domain_and_sort();
+
+    HostParams hp_;
+    DevParams P_;
+    int dev_ = 0;
+    hipStream_t st_ = nullptr;
+    size_t n_ = 0;
+    size_t cap_n_ = 0;
+    unsigned long long cell_cap_ = 1;
+    long long bkey_[6] = {0, 0, 0, 0, 0, 0};
+    bool keep_aux_ = false, aux_valid_ = false, dens_valid_ = false;
+    bool has_body_ = false;
+
+    // current state (cell order after the first step)
+    DevBuf<float4> posm_, vel_;
+    DevBuf<uint32_t> id_, mat_;
+    DevBuf<float4> body_;
+    // sorted copies for the step
+    DevBuf<float4> posm_s_, velm_s_, body_s_;
+    DevBuf<uint32_t> id_s_, mat_s_, cell_s_;
+    DevBuf<float> rho_, prs_, mp_;
+    // sort scratch
+    DevBuf<uint32_t> cell_, rank_, counts_, partial_, start_, tmp_, idtmp_;
+    // aux outputs
+    DevBuf<float4> acc_, xsph_;
+    DevBuf<float> mu_;
+    // staging for readback
+    DevBuf<float4> stage4_;
+    DevBuf<float> stage1_;
+    DevBuf<Domain> dom_;
+    DevBuf<double> diag_part_, diag_out_;
+
+    bool prof_ = false;
+    std::vector<hipEvent_t> prof_ev_;  // 4 per profiled step
+    hipEvent_t ev();
+};
+
+}  // namespace phy

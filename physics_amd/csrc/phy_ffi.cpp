@@ -1,0 +1,463 @@
+// phy_ffi.cpp — extern "C" boundary (include/phy_ffi.h).
+//
+// Every entry point catches all C++ exceptions: none may cross the C boundary,
+// as the reference's guard() = catch_unwind (crates/phy-ffi/src/lib.rs:73-75).
+// The reason of the last failure is kept per thread for phy_last_error().
+#include "../../include/phy_ffi.h"
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "world.h"
+
+using phy::FluidSubsystem;
+using phy::FluidWorld;
+using phy::World;
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char* what) { g_err = what ? what : "unknown error"; }
+
+template <class R, class F>
+R guard(R fail, F&& f) {
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        set_err(e.what());
+    } catch (...) {
+        set_err("unknown exception");
+    }
+    return fail;
+}
+
+World* as_world(PhyWorldHandle* w) { return reinterpret_cast<World*>(w); }
+PhyWorldHandle* box_world(World* w) { return reinterpret_cast<PhyWorldHandle*>(w); }
+
+// First fluid subsystem of the world, or null.
+FluidSubsystem* first_fluid(World* w) {
+    for (size_t i = 0; i < w->subsystem_count(); ++i)
+        if (auto* f = dynamic_cast<FluidSubsystem*>(w->get(i))) return f;
+    return nullptr;
+}
+
+FluidSubsystem* need_fluid(PhyWorldHandle* h) {
+    if (!h) throw std::invalid_argument("null world handle");
+    FluidSubsystem* f = first_fluid(as_world(h));
+    if (!f) throw std::invalid_argument("world has no fluid subsystem");
+    return f;
+}
+
+phy::SphParams from_c(const PhySphParams& c) {
+    phy::SphParams p;
+    p.rest_density = c.rest_density;
+    p.stiffness = c.stiffness;
+    p.viscosity = c.viscosity;
+    if (c.n_visc_k == 0 || c.n_visc_k > PHY_MAX_MATERIALS || c.n_visc_n == 0 ||
+        c.n_visc_n > PHY_MAX_MATERIALS)
+        throw std::invalid_argument("PhySphParams: material table sizes must be in 1..16");
+    p.visc_k.assign(c.visc_k, c.visc_k + c.n_visc_k);
+    p.visc_n.assign(c.visc_n, c.visc_n + c.n_visc_n);
+    p.shear_min = c.shear_min;
+    p.mass = c.mass;
+    p.h = c.h;
+    p.gravity = phy::Vec3{c.gravity[0], c.gravity[1], c.gravity[2]};
+    p.bounds_min = phy::Vec3{c.bounds_min[0], c.bounds_min[1], c.bounds_min[2]};
+    p.bounds_max = phy::Vec3{c.bounds_max[0], c.bounds_max[1], c.bounds_max[2]};
+    p.boundary_damp = c.boundary_damp;
+    p.xsph_eps = c.xsph_eps;
+    return p;
+}
+
+void to_c(const phy::SphParams& p, PhySphParams* c) {
+    std::memset(c, 0, sizeof(*c));
+    c->rest_density = p.rest_density;
+    c->stiffness = p.stiffness;
+    c->viscosity = p.viscosity;
+    c->n_visc_k = (uint32_t)p.visc_k.size();
+    c->n_visc_n = (uint32_t)p.visc_n.size();
+    for (size_t i = 0; i < p.visc_k.size() && i < PHY_MAX_MATERIALS; ++i) c->visc_k[i] = p.visc_k[i];
+    for (size_t i = 0; i < p.visc_n.size() && i < PHY_MAX_MATERIALS; ++i) c->visc_n[i] = p.visc_n[i];
+    c->shear_min = p.shear_min;
+    c->mass = p.mass;
+    c->h = p.h;
+    const phy::Vec3* v[3] = {&p.gravity, &p.bounds_min, &p.bounds_max};
+    double* o[3] = {c->gravity, c->bounds_min, c->bounds_max};
+    for (int k = 0; k < 3; ++k) {
+        o[k][0] = v[k]->x;
+        o[k][1] = v[k]->y;
+        o[k][2] = v[k]->z;
+    }
+    c->boundary_damp = p.boundary_damp;
+    c->xsph_eps = p.xsph_eps;
+}
+
+// lib.rs:113-126
+World* build_fluid_world() {
+    auto fw = std::make_unique<FluidWorld>(phy::SphParams::defaults());
+    fw->fill_box(phy::Vec3{-4.0, -4.0, -1.0}, phy::Vec3{0.0, 4.0, 1.0}, 0.18, 0.05);
+    auto* w = new World();
+    w->add_subsystem(std::make_unique<FluidSubsystem>(std::move(fw)));
+    return w;
+}
+
+// Concatenated readback over every fluid subsystem (lib.rs:399-429): writes
+// whole particles while 3 more values fit; returns particles written.
+template <class T, class Read>
+uintptr_t read_all(PhyWorldHandle* h, T* buf, uintptr_t len, int comps, Read&& rd) {
+    if (!h || !buf) return 0;
+    World* w = as_world(h);
+    uintptr_t written = 0;  // values
+    for (size_t i = 0; i < w->subsystem_count(); ++i) {
+        auto* f = dynamic_cast<FluidSubsystem*>(w->get(i));
+        if (!f) continue;
+        size_t n = f->world().len();
+        if (!n) continue;
+        std::vector<T> tmp(n * comps);
+        rd(f->world(), tmp.data());
+        for (size_t p = 0; p < n; ++p) {
+            if (written + comps > len) return written / comps;
+            std::memcpy(buf + written, tmp.data() + p * comps, comps * sizeof(T));
+            written += comps;
+        }
+    }
+    return written / comps;
+}
+
+int field_comps(int32_t f) {
+    switch (f) {
+        case PHY_FIELD_POS:
+        case PHY_FIELD_VEL:
+        case PHY_FIELD_ACC:
+        case PHY_FIELD_XSPH:
+            return 3;
+        case PHY_FIELD_RHO:
+        case PHY_FIELD_P:
+        case PHY_FIELD_MASS:
+        case PHY_FIELD_MU_EFF:
+            return 1;
+        default:
+            return 0;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+uint32_t phy_ffi_abi_version(void) { return PHY_FFI_ABI_VERSION; }
+
+const char* phy_last_error(void) { return g_err.c_str(); }
+
+PhyWorldHandle* phy_world_create_fluid(void) {
+    return guard<PhyWorldHandle*>(nullptr, [] { return box_world(build_fluid_world()); });
+}
+
+#define PHY_UNSUPPORTED_FACTORY(name, what)                                              \
+    PhyWorldHandle* name(void) {                                                         \
+        set_err(what " worlds are not provided by this library (fluid path only)");      \
+        return nullptr;                                                                  \
+    }
+PHY_UNSUPPORTED_FACTORY(phy_world_create_rigid, "rigid")
+PHY_UNSUPPORTED_FACTORY(phy_world_create_rigid_empty, "rigid")
+PHY_UNSUPPORTED_FACTORY(phy_world_create_granular, "granular")
+PHY_UNSUPPORTED_FACTORY(phy_world_create_coupled, "coupled fluid+rigid")
+PHY_UNSUPPORTED_FACTORY(phy_world_create_soft_empty, "soft-body")
+PHY_UNSUPPORTED_FACTORY(phy_world_create_field_empty, "field")
+PHY_UNSUPPORTED_FACTORY(phy_world_create_optics_empty, "optics")
+PHY_UNSUPPORTED_FACTORY(phy_world_create_solid_empty, "solid")
+#undef PHY_UNSUPPORTED_FACTORY
+
+int32_t phy_world_step(PhyWorldHandle* w, double dt) {
+    if (!w) return -1;
+    return guard<int32_t>(-1, [&] {
+        as_world(w)->step(dt);
+        return 0;
+    });
+}
+
+int32_t phy_world_step_n(PhyWorldHandle* w, double dt, uint32_t n) {
+    if (!w) return -1;
+    return guard<int32_t>(-1, [&] {
+        for (uint32_t i = 0; i < n; ++i) as_world(w)->step(dt);
+        return 0;
+    });
+}
+
+int32_t phy_world_step_checked(PhyWorldHandle* w, double dt) {
+    if (!w) return -1;
+    return guard<int32_t>(-1, [&] {
+        auto e = as_world(w)->step_checked(dt);
+        if (!e) return 0;
+        if (e->kind == phy::WorldError::NonFinite) return 2;
+        if (e->kind == phy::WorldError::Stalled) return 3;
+        return 4;
+    });
+}
+
+int32_t phy_world_step_f32(PhyWorldHandle* w, float dt) { return phy_world_step(w, (double)dt); }
+
+double phy_world_time(PhyWorldHandle* w) {
+    if (!w) return NAN;
+    return as_world(w)->time();
+}
+
+uintptr_t phy_world_sub_count(PhyWorldHandle* w) {
+    if (!w) return 0;
+    return as_world(w)->subsystem_count();
+}
+
+uintptr_t phy_world_fluid_count(PhyWorldHandle* w) {
+    if (!w) return 0;
+    return guard<uintptr_t>(0, [&] {
+        World* W = as_world(w);
+        uintptr_t total = 0;
+        for (size_t i = 0; i < W->subsystem_count(); ++i)
+            if (auto* f = dynamic_cast<FluidSubsystem*>(W->get(i))) total += f->world().len();
+        return total;
+    });
+}
+
+uintptr_t phy_world_get_fluid_positions(PhyWorldHandle* w, double* buf, uintptr_t len) {
+    return guard<uintptr_t>(0, [&] {
+        return read_all<double>(w, buf, len, 3, [](FluidWorld& f, double* o) { f.read_vec(phy::F_POS, o); });
+    });
+}
+uintptr_t phy_world_get_fluid_velocities(PhyWorldHandle* w, double* buf, uintptr_t len) {
+    return guard<uintptr_t>(0, [&] {
+        return read_all<double>(w, buf, len, 3, [](FluidWorld& f, double* o) { f.read_vec(phy::F_VEL, o); });
+    });
+}
+uintptr_t phy_world_get_fluid_positions_f32(PhyWorldHandle* w, float* buf, uintptr_t len) {
+    return guard<uintptr_t>(0, [&] {
+        return read_all<float>(w, buf, len, 3, [](FluidWorld& f, float* o) { f.read_vec_f32(phy::F_POS, o); });
+    });
+}
+uintptr_t phy_world_get_fluid_velocities_f32(PhyWorldHandle* w, float* buf, uintptr_t len) {
+    return guard<uintptr_t>(0, [&] {
+        return read_all<float>(w, buf, len, 3, [](FluidWorld& f, float* o) { f.read_vec_f32(phy::F_VEL, o); });
+    });
+}
+
+uintptr_t phy_world_rigid_count(PhyWorldHandle* w) {
+    (void)w;
+    return 0;
+}
+uintptr_t phy_world_get_rigid_transforms(PhyWorldHandle* w, double* buf, uintptr_t len) {
+    (void)w; (void)buf; (void)len;
+    return 0;
+}
+uintptr_t phy_world_get_rigid_transforms_f32(PhyWorldHandle* w, float* buf, uintptr_t len) {
+    (void)w; (void)buf; (void)len;
+    return 0;
+}
+int64_t phy_world_rigid_add_body(PhyWorldHandle* w, int32_t shape_kind, double mass,
+                                 const double* pos7, const double* inertia3) {
+    (void)w; (void)shape_kind; (void)mass; (void)pos7; (void)inertia3;
+    set_err("no rigid subsystem in this library");
+    return -1;
+}
+int32_t phy_world_rigid_apply_force(PhyWorldHandle* w, int64_t id, const double* f3, double dt,
+                                    int32_t mode) {
+    (void)w; (void)id; (void)f3; (void)dt; (void)mode;
+    set_err("no rigid subsystem in this library");
+    return -1;
+}
+int32_t phy_world_rigid_apply_torque(PhyWorldHandle* w, int64_t id, const double* t3, double dt,
+                                     int32_t mode) {
+    (void)w; (void)id; (void)t3; (void)dt; (void)mode;
+    set_err("no rigid subsystem in this library");
+    return -1;
+}
+int32_t phy_world_rigid_get_velocity(PhyWorldHandle* w, int64_t id, double* out3) {
+    (void)w; (void)id; (void)out3;
+    set_err("no rigid subsystem in this library");
+    return -1;
+}
+int32_t phy_world_rigid_get_angular_velocity(PhyWorldHandle* w, int64_t id, double* out3) {
+    (void)w; (void)id; (void)out3;
+    set_err("no rigid subsystem in this library");
+    return -1;
+}
+
+int32_t phy_world_save(PhyWorldHandle* w, const char* path) {
+    (void)w; (void)path;
+    set_err("phy_world_save: the JSON archive is not provided by this library");
+    return -1;
+}
+PhyWorldHandle* phy_world_load(const char* path) {
+    (void)path;
+    set_err("phy_world_load: the JSON archive is not provided by this library");
+    return nullptr;
+}
+
+void phy_world_destroy(PhyWorldHandle* w) {
+    if (!w) return;
+    guard<int>(0, [&] {
+        delete as_world(w);
+        return 0;
+    });
+}
+
+// ---- additive ----------------------------------------------------------------
+
+void phy_sph_params_defaults(PhySphParams* out) {
+    if (out) to_c(phy::SphParams::defaults(), out);
+}
+
+PhyWorldHandle* phy_world_create_fluid_with(const PhySphParams* params) {
+    if (!params) {
+        set_err("null params");
+        return nullptr;
+    }
+    return guard<PhyWorldHandle*>(nullptr, [&] {
+        auto fw = std::make_unique<FluidWorld>(from_c(*params));
+        auto* w = new World();
+        w->add_subsystem(std::make_unique<FluidSubsystem>(std::move(fw)));
+        return box_world(w);
+    });
+}
+
+int32_t phy_fluid_set_params(PhyWorldHandle* w, const PhySphParams* params) {
+    return guard<int32_t>(-1, [&] {
+        if (!params) throw std::invalid_argument("null params");
+        need_fluid(w)->world().set_params(from_c(*params));
+        return 0;
+    });
+}
+
+int32_t phy_fluid_get_params(PhyWorldHandle* w, PhySphParams* out) {
+    return guard<int32_t>(-1, [&] {
+        if (!out) throw std::invalid_argument("null out");
+        to_c(need_fluid(w)->world().params(), out);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_fill_box(PhyWorldHandle* w, const double* mn, const double* mx, double spacing,
+                           double margin) {
+    return guard<int32_t>(-1, [&] {
+        if (!mn || !mx) throw std::invalid_argument("null box corner");
+        need_fluid(w)->world().fill_box(phy::Vec3{mn[0], mn[1], mn[2]}, phy::Vec3{mx[0], mx[1], mx[2]},
+                                        spacing, margin);
+        return 0;
+    });
+}
+
+double phy_fluid_lattice_mass(PhyWorldHandle* w, double spacing) {
+    return guard<double>(NAN, [&] { return need_fluid(w)->world().lattice_mass(spacing); });
+}
+
+int32_t phy_fluid_set_particles(PhyWorldHandle* w, uintptr_t n, const double* pos3, const double* vel3,
+                                const double* mass, const uint32_t* material) {
+    return guard<int32_t>(-1, [&] {
+        if (n && !pos3) throw std::invalid_argument("null positions");
+        need_fluid(w)->world().upload(n, pos3, vel3, mass, material);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_set_particles_f32(PhyWorldHandle* w, uintptr_t n, const float* pos3, const float* vel3,
+                                    const float* mass, const uint32_t* material) {
+    return guard<int32_t>(-1, [&] {
+        if (n && !pos3) throw std::invalid_argument("null positions");
+        need_fluid(w)->world().upload_f32(n, pos3, vel3, mass, material);
+        return 0;
+    });
+}
+
+uintptr_t phy_fluid_get_field(PhyWorldHandle* w, int32_t field, double* buf, uintptr_t len) {
+    return guard<uintptr_t>(0, [&] {
+        int c = field_comps(field);
+        if (!c) throw std::invalid_argument("unknown field");
+        FluidWorld& f = need_fluid(w)->world();
+        size_t n = f.len();
+        if (!buf || !n) return (uintptr_t)0;
+        std::vector<double> tmp(n * c);
+        if (c == 3) f.read_vec((phy::Field)field, tmp.data());
+        else f.read_scalar((phy::Field)field, tmp.data());
+        uintptr_t np = std::min<uintptr_t>(n, len / c);
+        std::memcpy(buf, tmp.data(), np * c * sizeof(double));
+        return np;
+    });
+}
+
+uintptr_t phy_fluid_get_field_f32(PhyWorldHandle* w, int32_t field, float* buf, uintptr_t len) {
+    return guard<uintptr_t>(0, [&] {
+        int c = field_comps(field);
+        if (!c) throw std::invalid_argument("unknown field");
+        FluidWorld& f = need_fluid(w)->world();
+        size_t n = f.len();
+        if (!buf || !n) return (uintptr_t)0;
+        std::vector<float> tmp(n * c);
+        if (c == 3) f.read_vec_f32((phy::Field)field, tmp.data());
+        else f.read_scalar_f32((phy::Field)field, tmp.data());
+        uintptr_t np = std::min<uintptr_t>(n, len / c);
+        std::memcpy(buf, tmp.data(), np * c * sizeof(float));
+        return np;
+    });
+}
+
+int32_t phy_fluid_set_keep_aux(PhyWorldHandle* w, int32_t on) {
+    return guard<int32_t>(-1, [&] {
+        need_fluid(w)->world().set_keep_aux(on != 0);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_set_body_acc(PhyWorldHandle* w, const double* b3, uintptr_t len) {
+    return guard<int32_t>(-1, [&] {
+        FluidWorld& f = need_fluid(w)->world();
+        if (!b3 || len < 3 * f.len()) throw std::invalid_argument("body_acc buffer too short");
+        f.set_body_acc(b3);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_diagnostics_f64(PhyWorldHandle* w, double* out5) {
+    return guard<int32_t>(-1, [&] {
+        if (!out5) throw std::invalid_argument("null out");
+        need_fluid(w)->world().diagnostics(out5);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_neighbor_stats(PhyWorldHandle* w, uint32_t* counts, uint64_t* sums, uint64_t* xors,
+                                 uintptr_t len) {
+    return guard<int32_t>(-1, [&] {
+        FluidWorld& f = need_fluid(w)->world();
+        if (!counts || !sums || !xors || len < f.len()) throw std::invalid_argument("buffers too short");
+        f.neighbor_stats(counts, sums, xors);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_sync(PhyWorldHandle* w) {
+    return guard<int32_t>(-1, [&] {
+        if (!w) throw std::invalid_argument("null world handle");
+        World* W = as_world(w);
+        for (size_t i = 0; i < W->subsystem_count(); ++i)
+            if (auto* f = dynamic_cast<FluidSubsystem*>(W->get(i))) f->world().device().sync();
+        return 0;
+    });
+}
+
+int32_t phy_fluid_profile_begin(PhyWorldHandle* w) {
+    return guard<int32_t>(-1, [&] {
+        need_fluid(w)->world().device().profile_begin();
+        return 0;
+    });
+}
+
+int32_t phy_fluid_profile_end(PhyWorldHandle* w, double* ms4, uint32_t* steps) {
+    return guard<int32_t>(-1, [&] {
+        if (!ms4) throw std::invalid_argument("null out");
+        uint32_t s = need_fluid(w)->world().device().profile_end(ms4);
+        if (steps) *steps = s;
+        return 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,73 @@
+// sph_kernels.h — host-side launch interface of sph_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sph_types.h"
+
+namespace phy {
+
+constexpr int kScanBlocks = 1024;  // fixed grid of the three-phase cell scan
+constexpr int kDiagBlocks = 512;   // fixed grid of the FP64 diagnostics reduction
+constexpr int kForceList = 32;     // per-lane LDS neighbour list length of k_force
+
+struct SortBuffers {
+    // inputs (current order)
+    const float4* posm_in;
+    const float4* vel_in;
+    const uint32_t* id_in;
+    const uint32_t* mat_in;   // may be null
+    const float4* body_in;    // may be null
+    // scratch
+    uint32_t* cell;
+    uint32_t* rank;
+    uint32_t* counts;         // kept all-zero between steps
+    uint32_t* partial;
+    uint32_t* start;
+    uint32_t* tmp;
+    uint32_t* idtmp;
+    // outputs (cell order)
+    float4* posm_out;
+    float4* velm_out;
+    uint32_t* id_out;
+    uint32_t* mat_out;
+    float4* body_out;
+    uint32_t* cell_out;
+};
+
+struct ForceArgs {
+    const float4* posm;
+    const float4* velm;
+    const float* rho;
+    const float* prs;
+    const float* mp;
+    const uint32_t* cell;
+    const uint32_t* start;
+    const uint32_t* id;
+    const uint32_t* mat;      // may be null when one material
+    const float4* body;       // may be null
+    float4* posm_out;
+    float4* vel_out;
+    float4* acc_out;          // optional (null: not stored)
+    float* mu_out;
+    float4* xsph_out;
+};
+
+void launch_domain(hipStream_t st, int n, const float4* posm, double inv, Domain* d,
+                   unsigned long long cap, const long long bk[6]);
+void launch_sort(hipStream_t st, int n, const SortBuffers& b, const Domain* d, double inv);
+void launch_density(hipStream_t st, int n, const float4* posm, const uint32_t* cellv,
+                    const uint32_t* start, const Domain* d, const DevParams& P, float* rho,
+                    float* prs, float4* velm, float* mp);
+void launch_force(hipStream_t st, int n, float dt, const ForceArgs& a, Domain* d,
+                  const DevParams& P);
+void launch_diag(hipStream_t st, int n, const float4* posm, const float4* vel, double* partial,
+                 double* out);
+void launch_unsort4(hipStream_t st, int n, const uint32_t* id, const float4* in, float4* out);
+void launch_unsort1(hipStream_t st, int n, const uint32_t* id, const float* in, float* out);
+void launch_neighbor_stats(hipStream_t st, int n, const float4* posm, const uint32_t* cellv,
+                           const uint32_t* start, const uint32_t* id, const Domain* d,
+                           const DevParams& P, uint32_t* cnt, unsigned long long* sum,
+                           unsigned long long* xr);
+
+}  // namespace phy

@@ -1,0 +1,824 @@
+// sph_kernels.hip — gfx950 kernels of one FluidWorld::step
+// (crates/phy-fluid/src/sph/world.rs:183-192):
+//
+//   k_aabb / k_domain_finalize   dense-grid domain from the finite cell keys
+//   k_bin, k_scan_*, k_scatter   counting sort of particles into cells of size h
+//   k_reorder                    deterministic within-cell rank + gather to cell order
+//   k_density                    rho / p gather (world.rs:204-246)
+//   k_force                      forces (world.rs:252-347) fused with integrate
+//                                (352-363), enforce_bounds (366-397) and the
+//                                validate watchdog (subsystem.rs:74-90)
+//   k_diag_*                     FP64 mass / momentum / kinetic-energy reductions
+//                                (subsystem.rs:149-163)
+//
+// Every float sum is a gather in a fixed order (the reference's cell order
+// di, dj, dk with ascending original index inside a cell), compensated with
+// Kahan's algorithm, so repeated runs are bit-identical. The only atomics are
+// integer ones (cell counts, min/max of keys, watchdog indices) whose results do
+// not depend on arrival order.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+#include "sph_types.h"
+#include "sph_kernels.h"
+
+namespace phy {
+
+// ---------------------------------------------------------------------------
+// helpers
+
+// num-traits ToPrimitive::to_i64 on the floored f64 key (grid.rs:29-31):
+// NaN and values outside [-2^63, 2^63) become 0 (unwrap_or(0)).
+__device__ __forceinline__ long long key1(float x, double inv) {
+    double f = floor((double)x * inv);
+    if (f >= -9223372036854775808.0 && f < 9223372036854775808.0) return (long long)f;
+    return 0;
+}
+
+__device__ __forceinline__ bool finite3(float x, float y, float z) {
+    return isfinite(x) && isfinite(y) && isfinite(z);
+}
+
+// Exact f64 distance predicate of the reference, r = sqrt((dx*dx + dy*dy) + dz*dz)
+// in f64 on the FP32 coordinates (world.rs:226, 295-296; nalgebra norm order).
+// Contraction is switched off so the f64 sum rounds exactly like the oracle's.
+__device__ __attribute__((noinline)) double exact_s64(float xi, float yi, float zi, float xj,
+                                                      float yj, float zj) {
+#pragma clang fp contract(off)
+    double dx = (double)xi - (double)xj;
+    double dy = (double)yi - (double)yj;
+    double dz = (double)zi - (double)zj;
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+// Classifies a candidate pair from its FP32 squared distance s32 (any rounding,
+// FMA allowed): returns 1 inside (r < h), 0 outside, 2 for NaN distance.
+// |s32 - s64| <= 6u * s, so the filter margins (2^-19 relative) are safe; the
+// band in between is decided by the exact f64 expression.
+__device__ __forceinline__ int classify(const DevParams& P, float s32, float xi, float yi, float zi,
+                                        float xj, float yj, float zj) {
+    if (s32 < P.s_lo) return 1;
+    if (s32 > P.s_hi) return 0;
+    double s = exact_s64(xi, yi, zi, xj, yj, zj);
+    if (s < P.s_thresh) return 1;
+    if (s != s) return 2;
+    return 0;
+}
+
+__device__ __forceinline__ void kahan_add(float& sum, float& c, float x) {
+    float y = x - c;
+    float t = sum + y;
+    c = (t - sum) - y;
+    sum = t;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; remap so consecutive
+// logical blocks (spatially adjacent cells) share one XCD's L2.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    int per = nb >> 3, rem = nb & 7;
+    int x = b & 7, k = b >> 3;
+    return x * per + min(x, rem) + k;
+}
+
+__device__ __forceinline__ long long wave_min_ll(long long v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        long long w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ long long wave_max_ll(long long v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        long long w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// domain
+
+__global__ void k_domain_reset(Domain* d) {
+    for (int a = 0; a < 3; ++a) {
+        d->amin[a] = 0x7fffffffffffffffll;
+        d->amax[a] = (long long)0x8000000000000000ull;
+    }
+    d->bad_pos = 0xffffffffu;
+    d->bad_vel = 0xffffffffu;
+}
+
+__global__ __launch_bounds__(256) void k_aabb(int n, const float4* __restrict__ posm, double inv,
+                                              Domain* d) {
+    long long mn[3] = {0x7fffffffffffffffll, 0x7fffffffffffffffll, 0x7fffffffffffffffll};
+    long long mx[3] = {(long long)0x8000000000000000ull, (long long)0x8000000000000000ull,
+                       (long long)0x8000000000000000ull};
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        float4 p = posm[s];
+        if (!finite3(p.x, p.y, p.z)) continue;
+        long long k[3] = {key1(p.x, inv), key1(p.y, inv), key1(p.z, inv)};
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = k[a] < mn[a] ? k[a] : mn[a];
+            mx[a] = k[a] > mx[a] ? k[a] : mx[a];
+        }
+    }
+    for (int a = 0; a < 3; ++a) {
+        mn[a] = wave_min_ll(mn[a]);
+        mx[a] = wave_max_ll(mx[a]);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        for (int a = 0; a < 3; ++a) {
+            atomicMin(&d->amin[a], mn[a]);
+            atomicMax(&d->amax[a], mx[a]);
+        }
+    }
+}
+
+// One thread: turn the key AABB into the dense table domain, clipped to the
+// table capacity. Particles outside the domain are binned to the overflow cell.
+__global__ void k_domain_finalize(Domain* d, unsigned long long cap, long long bk0, long long bk1,
+                                  long long bk2, long long bK0, long long bK1, long long bK2) {
+    long long lo[3], hi[3];
+    long long blo[3] = {bk0, bk1, bk2}, bhi[3] = {bK0, bK1, bK2};
+    bool any = d->amin[0] <= d->amax[0];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = any ? d->amin[a] : blo[a];
+        hi[a] = any ? d->amax[a] : blo[a];
+    }
+    // extents as unsigned differences: exact for any hi >= lo
+    auto ext = [&](int a) { return ((unsigned long long)hi[a] - (unsigned long long)lo[a]) + 1ull; };
+    auto vol = [&]() {
+        double v = 1.0;
+        for (int a = 0; a < 3; ++a) v *= (double)ext(a);
+        return v;
+    };
+    if (vol() > (double)cap) {
+        // keep the part inside the bounds box first
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = lo[a] > blo[a] ? lo[a] : blo[a];
+            hi[a] = hi[a] < bhi[a] ? hi[a] : bhi[a];
+            if (hi[a] < lo[a]) hi[a] = lo[a];
+        }
+    }
+    while (vol() > (double)cap) {
+        // shrink the longest axis symmetrically
+        int am = 0;
+        for (int a = 1; a < 3; ++a)
+            if (ext(a) > ext(am)) am = a;
+        unsigned long long cut = ext(am) / 4 + 1;
+        lo[am] = (long long)((unsigned long long)lo[am] + cut / 2);
+        hi[am] = (long long)((unsigned long long)hi[am] - (cut - cut / 2));
+        if (hi[am] < lo[am]) hi[am] = lo[am];
+    }
+    unsigned long long nc = 1;
+    for (int a = 0; a < 3; ++a) {
+        d->kmin[a] = lo[a];
+        d->dims[a] = hi[a] - lo[a] + 1;
+        nc *= (unsigned long long)d->dims[a];
+    }
+    d->ncells = (uint32_t)nc;
+}
+
+// ---------------------------------------------------------------------------
+// counting sort
+
+__device__ __forceinline__ uint32_t cell_of(float x, float y, float z, double inv, const Domain& d) {
+    // wrapping differences: an out-of-domain key maps to a huge unsigned value
+    unsigned long long k0 = (unsigned long long)key1(x, inv) - (unsigned long long)d.kmin[0];
+    unsigned long long k1 = (unsigned long long)key1(y, inv) - (unsigned long long)d.kmin[1];
+    unsigned long long k2 = (unsigned long long)key1(z, inv) - (unsigned long long)d.kmin[2];
+    if (k0 >= (unsigned long long)d.dims[0] || k1 >= (unsigned long long)d.dims[1] ||
+        k2 >= (unsigned long long)d.dims[2])
+        return d.ncells;
+    return (uint32_t)((k0 * d.dims[1] + k1) * d.dims[2] + k2);
+}
+
+__global__ __launch_bounds__(256) void k_bin(int n, const float4* __restrict__ posm, double inv,
+                                             const Domain* __restrict__ dom, uint32_t* cell,
+                                             uint32_t* rank, uint32_t* counts) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    Domain d = *dom;
+    float4 p = posm[s];
+    uint32_t c = cell_of(p.x, p.y, p.z, inv, d);
+    cell[s] = c;
+    rank[s] = atomicAdd(&counts[c], 1u);
+}
+
+// Three-phase exclusive scan over counts[0 .. ncells] (ncells + 1 entries, the
+// last one the overflow cell) with a fixed grid; ncells is read on the device.
+// Phase 3 also clears counts for the next step.
+__device__ __forceinline__ void chunk_of(uint32_t total, int b, int nb, uint32_t& lo, uint32_t& hi) {
+    uint32_t per = (total + nb - 1) / nb;
+    lo = min(total, per * (uint32_t)b);
+    hi = min(total, lo + per);
+}
+
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    uint32_t t = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+    return t;
+}
+
+__global__ __launch_bounds__(256) void k_scan_partial(const uint32_t* __restrict__ counts,
+                                                      const Domain* __restrict__ dom,
+                                                      uint32_t* partial) {
+    __shared__ uint32_t sh[4];
+    uint32_t total = dom->ncells + 1;
+    uint32_t lo, hi;
+    chunk_of(total, blockIdx.x, gridDim.x, lo, hi);
+    uint32_t v = 0;
+    for (uint32_t c = lo + threadIdx.x; c < hi; c += blockDim.x) v += counts[c];
+    v = block_sum_u32(v, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+// single block of 1024 threads; nb <= 1024
+__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* partial, int nb,
+                                                   const Domain* __restrict__ dom, uint32_t* start) {
+    __shared__ uint32_t sh[1024];
+    int t = threadIdx.x;
+    uint32_t v = t < nb ? partial[t] : 0;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        uint32_t a = t >= o ? sh[t - o] : 0;
+        __syncthreads();
+        sh[t] += a;
+        __syncthreads();
+    }
+    if (t < nb) partial[t] = sh[t] - v;  // exclusive
+    if (t == 1023) start[dom->ncells + 1] = sh[1023];
+}
+
+__global__ __launch_bounds__(256) void k_scan_final(uint32_t* counts, const Domain* __restrict__ dom,
+                                                    const uint32_t* __restrict__ partial,
+                                                    uint32_t* start) {
+    __shared__ uint32_t sh[256];
+    uint32_t total = dom->ncells + 1;
+    uint32_t lo, hi;
+    chunk_of(total, blockIdx.x, gridDim.x, lo, hi);
+    uint32_t carry = partial[blockIdx.x];
+    int t = threadIdx.x;
+    for (uint32_t base = lo; base < hi; base += 256) {
+        uint32_t c = base + t;
+        uint32_t v = c < hi ? counts[c] : 0;
+        // inclusive scan in the block
+        uint32_t x = v;
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(x, o, 64);
+            if ((t & 63) >= o) x += y;
+        }
+        int w = t >> 6;
+        __syncthreads();
+        if ((t & 63) == 63) sh[w] = x;
+        __syncthreads();
+        uint32_t wo = 0;
+        for (int i = 0; i < w; ++i) wo += sh[i];
+        uint32_t incl = x + wo;
+        if (c < hi) {
+            start[c] = carry + incl - v;
+            counts[c] = 0;
+        }
+        uint32_t tot = sh[0] + sh[1] + sh[2] + sh[3];
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scatter(int n, const uint32_t* __restrict__ cell,
+                                                 const uint32_t* __restrict__ rank,
+                                                 const uint32_t* __restrict__ start,
+                                                 const uint32_t* __restrict__ id, uint32_t* tmp,
+                                                 uint32_t* idtmp) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    uint32_t q = start[cell[s]] + rank[s];
+    tmp[q] = (uint32_t)s;
+    idtmp[q] = id[s];
+}
+
+// Slot q of the scatter holds some member of its cell in arrival order; its
+// final slot is the cell start plus the number of members with a smaller
+// original index (ascending index inside a cell, grid.rs:37-38).
+__global__ __launch_bounds__(256) void k_reorder(
+    int n, const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ idtmp,
+    const uint32_t* __restrict__ cell, const uint32_t* __restrict__ start,
+    const float4* __restrict__ posm_in, const float4* __restrict__ vel_in,
+    const uint32_t* __restrict__ mat_in, const float4* __restrict__ body_in, float4* posm_out,
+    float4* velm_out, uint32_t* id_out, uint32_t* mat_out, float4* body_out, uint32_t* cell_out) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    uint32_t s = tmp[q];
+    uint32_t my = idtmp[q];
+    uint32_t c = cell[s];
+    uint32_t lo = start[c], hi = start[c + 1];
+    uint32_t r = 0;
+    for (uint32_t t = lo; t < hi; ++t) r += idtmp[t] < my ? 1u : 0u;
+    uint32_t dst = lo + r;
+    posm_out[dst] = posm_in[s];
+    float4 v = vel_in[s];
+    v.w = 0.f;
+    velm_out[dst] = v;
+    id_out[dst] = my;
+    cell_out[dst] = c;
+    if (mat_in) mat_out[dst] = mat_in[s];
+    if (body_in) body_out[dst] = body_in[s];
+}
+
+// ---------------------------------------------------------------------------
+// neighbour visiting
+//
+// Rows (dx, dy) in the reference order with the three z-adjacent cells of each
+// row as one contiguous run [start[c0], start[c2 + 1]).
+
+struct CellCoord {
+    long long cx, cy, cz;
+};
+
+__device__ __forceinline__ CellCoord decode(uint32_t c, const Domain& d) {
+    CellCoord r;
+    long long dz = d.dims[2], dy = d.dims[1];
+    r.cz = c % dz;
+    long long t = c / dz;
+    r.cy = t % dy;
+    r.cx = t / dy;
+    return r;
+}
+
+// Run of row k (0..8) of an in-domain particle; empty when the row leaves the
+// domain (those cells hold no dense particles).
+__device__ __forceinline__ void row_run(int k, const CellCoord& cc, const Domain& d,
+                                        const uint32_t* __restrict__ start, uint32_t& a,
+                                        uint32_t& b) {
+    long long x = cc.cx + (k / 3) - 1;
+    long long y = cc.cy + (k % 3) - 1;
+    if (x < 0 || x >= d.dims[0] || y < 0 || y >= d.dims[1]) {
+        a = b = 0;
+        return;
+    }
+    long long z0 = cc.cz > 0 ? cc.cz - 1 : 0;
+    long long z1 = cc.cz + 1 < d.dims[2] ? cc.cz + 1 : d.dims[2] - 1;
+    long long base = (x * d.dims[1] + y) * d.dims[2];
+    a = start[base + z0];
+    b = start[base + z1 + 1];
+}
+
+// Reference cell adjacency on full i64 keys with wrapping offsets (grid.rs:44-48).
+__device__ __forceinline__ bool keys_adjacent(float xi, float yi, float zi, float xj, float yj,
+                                              float zj, double inv) {
+    unsigned long long a = (unsigned long long)key1(xj, inv) - (unsigned long long)key1(xi, inv) + 1ull;
+    unsigned long long b = (unsigned long long)key1(yj, inv) - (unsigned long long)key1(yi, inv) + 1ull;
+    unsigned long long c = (unsigned long long)key1(zj, inv) - (unsigned long long)key1(zi, inv) + 1ull;
+    return a <= 2ull && b <= 2ull && c <= 2ull;
+}
+
+// Visits every candidate j of particle (xi, yi, zi) in the reference order and
+// calls f(j). Dense cells come from the table; the overflow segment (particles
+// binned outside the dense domain, e.g. NaN keys) is checked by key adjacency.
+template <class F>
+__device__ __forceinline__ void for_each_candidate(uint32_t ci, float xi, float yi, float zi,
+                                                   const Domain& d, double inv,
+                                                   const uint32_t* __restrict__ start,
+                                                   const float4* __restrict__ posm, F&& f) {
+    if (ci < d.ncells) {
+        CellCoord cc = decode(ci, d);
+        for (int k = 0; k < 9; ++k) {
+            uint32_t a, b;
+            row_run(k, cc, d, start, a, b);
+            for (uint32_t j = a; j < b; ++j) f(j);
+        }
+    } else {
+        // out-of-domain particle: visit the in-domain cells among its 27 keys
+        long long k0 = key1(xi, inv), k1 = key1(yi, inv), k2 = key1(zi, inv);
+        for (int di = -1; di <= 1; ++di)
+            for (int dj = -1; dj <= 1; ++dj)
+                for (int dk = -1; dk <= 1; ++dk) {
+                    unsigned long long x = (unsigned long long)k0 + (unsigned long long)(long long)di -
+                                           (unsigned long long)d.kmin[0];
+                    unsigned long long y = (unsigned long long)k1 + (unsigned long long)(long long)dj -
+                                           (unsigned long long)d.kmin[1];
+                    unsigned long long z = (unsigned long long)k2 + (unsigned long long)(long long)dk -
+                                           (unsigned long long)d.kmin[2];
+                    if (x >= (unsigned long long)d.dims[0] || y >= (unsigned long long)d.dims[1] ||
+                        z >= (unsigned long long)d.dims[2])
+                        continue;
+                    uint32_t c = (uint32_t)((x * d.dims[1] + y) * d.dims[2] + z);
+                    for (uint32_t j = start[c]; j < start[c + 1]; ++j) f(j);
+                }
+    }
+    uint32_t o0 = start[d.ncells], o1 = start[d.ncells + 1];
+    for (uint32_t j = o0; j < o1; ++j) {
+        float4 q = posm[j];
+        if (keys_adjacent(xi, yi, zi, q.x, q.y, q.z, inv)) f(j);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// density / EOS (world.rs:204-246)
+
+__global__ __launch_bounds__(256) void k_density(int n, const float4* __restrict__ posm,
+                                                 const uint32_t* __restrict__ cellv,
+                                                 const uint32_t* __restrict__ start,
+                                                 const Domain* __restrict__ dom, DevParams P,
+                                                 float* rho_out, float* p_out, float4* velm,
+                                                 float* mp_out) {
+    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Domain d = *dom;
+    float4 pi = posm[i];
+    float sum = 0.f, comp = 0.f;
+    for_each_candidate(cellv[i], pi.x, pi.y, pi.z, d, P.inv_cell, start, posm, [&](uint32_t j) {
+        float4 pj = posm[j];
+        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        float s = dx * dx + dy * dy + dz * dz;
+        if (classify(P, s, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) == 1) {
+            float q = P.h2 - s;
+            kahan_add(sum, comp, pj.w * (q * q * q));
+        }
+    });
+    float rho = P.c6 * sum;
+    float over = rho - P.rest;
+    float p = over > 0.f ? P.stiff * over : 0.f;
+    rho_out[i] = rho;
+    p_out[i] = p;
+    velm[i].w = pi.w / rho;            // m_j / rho_j of the force pass
+    mp_out[i] = pi.w * (p / (rho * rho));  // m_j * p_j / rho_j^2
+}
+
+// ---------------------------------------------------------------------------
+// forces + integrate + bounds + watchdog (world.rs:252-397, subsystem.rs:74-90)
+
+struct ForceAcc {
+    float fp[3], cfp[3];   // pressure  (x m_i * cs)
+    float fv[3], cfv[3];   // viscosity (x m_i * cv)
+    float xs[3], cxs[3];   // XSPH      (x c6)
+    float sh, csh;         // shear-rate proxy
+};
+
+__device__ __forceinline__ void pair_force(ForceAcc& A, const DevParams& P, float4 pi, float4 vi,
+                                           float Pi, float4 pj, float4 vj, float mpj) {
+    float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+    // distinct FP32 points closer than ~1e-19 underflow s; keep r tiny but finite
+    float s = fmaxf(dx * dx + dy * dy + dz * dz, 1.17549435e-38f);
+    float inv_r = __frsqrt_rn(s);
+    float r = s * inv_r;
+    float hr = P.h - r;
+    // pressure: dir * (m_i m_j (P_i + P_j) * cs (h - r)^2), dir = d / r
+    float a = (pj.w * Pi + mpj) * (hr * hr) * inv_r;
+    kahan_add(A.fp[0], A.cfp[0], a * dx);
+    kahan_add(A.fp[1], A.cfp[1], a * dy);
+    kahan_add(A.fp[2], A.cfp[2], a * dz);
+    // viscosity: (v_j - v_i) * (m_i m_j / rho_j * cv (h - r))
+    float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+    float mr = vj.w;  // m_j / rho_j
+    float b = mr * hr;
+    kahan_add(A.fv[0], A.cfv[0], b * dvx);
+    kahan_add(A.fv[1], A.cfv[1], b * dvy);
+    kahan_add(A.fv[2], A.cfv[2], b * dvz);
+    // XSPH: (v_j - v_i) * (m_j / rho_j * poly6(r))
+    float q = P.h2 - s;
+    float w = mr * (q * q * q);
+    kahan_add(A.xs[0], A.cxs[0], w * dvx);
+    kahan_add(A.xs[1], A.cxs[1], w * dvy);
+    kahan_add(A.xs[2], A.cxs[2], w * dvz);
+    // shear-rate proxy: |v_j - v_i| / (r + 1e-4) * (m_j / rho_j)
+    float dv = sqrtf(dvx * dvx + dvy * dvy + dvz * dvz);
+    kahan_add(A.sh, A.csh, dv / (r + P.r_eps) * mr);
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_force(
+    int n, float dt, const float4* __restrict__ posm, const float4* __restrict__ velm,
+    const float* __restrict__ rho, const float* __restrict__ prs, const float* __restrict__ mp,
+    const uint32_t* __restrict__ cellv, const uint32_t* __restrict__ start,
+    const uint32_t* __restrict__ idv, const uint32_t* __restrict__ matv,
+    const float4* __restrict__ body, Domain* dom, DevParams P, float4* posm_out, float4* vel_out,
+    float4* acc_out, float* mu_out, float4* xsph_out) {
+    __shared__ uint32_t lst[L][256];
+    const int tid = threadIdx.x;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + tid;
+    const bool act = i < n;
+    const Domain d = *dom;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi;
+    float rho_i = 1.f, Pi = 0.f;
+    uint32_t ci = d.ncells;
+    if (act) {
+        pi = posm[i];
+        vi = velm[i];
+        rho_i = rho[i];
+        Pi = prs[i] / (rho_i * rho_i);
+        ci = cellv[i];
+    }
+    ForceAcc A = {};
+
+    // Candidate cursor: rows 0..8 of the dense table, then the overflow segment
+    // (row 9). Every lane tests one candidate per round; accepted neighbours go
+    // to the lane's LDS list, and the whole wave drains the lists when any is full.
+    const uint32_t o0 = start[d.ncells], o1 = start[d.ncells + 1];
+    const bool dense_i = ci < d.ncells;
+    CellCoord cc = {};
+    if (act && dense_i) cc = decode(ci, d);
+    long long k0 = 0, k1 = 0, k2 = 0;
+    if (act && !dense_i) {
+        k0 = key1(pi.x, P.inv_cell);
+        k1 = key1(pi.y, P.inv_cell);
+        k2 = key1(pi.z, P.inv_cell);
+    }
+    int row = 0;  // 0..8 dense rows (or 27 single cells for an overflow particle), then overflow
+    uint32_t j = 0, jend = 0;
+    const int nrows = dense_i ? 9 : 27;
+    auto next_run = [&]() {
+        while (j >= jend && row <= nrows) {
+            if (row < nrows) {
+                if (dense_i) {
+                    row_run(row, cc, d, start, j, jend);
+                } else {
+                    int di = row / 9 - 1, dj = (row / 3) % 3 - 1, dk = row % 3 - 1;
+                    unsigned long long x = (unsigned long long)k0 + (unsigned long long)(long long)di -
+                                           (unsigned long long)d.kmin[0];
+                    unsigned long long y = (unsigned long long)k1 + (unsigned long long)(long long)dj -
+                                           (unsigned long long)d.kmin[1];
+                    unsigned long long z = (unsigned long long)k2 + (unsigned long long)(long long)dk -
+                                           (unsigned long long)d.kmin[2];
+                    if (x >= (unsigned long long)d.dims[0] || y >= (unsigned long long)d.dims[1] ||
+                        z >= (unsigned long long)d.dims[2]) {
+                        j = jend = 0;
+                    } else {
+                        uint32_t c = (uint32_t)((x * d.dims[1] + y) * d.dims[2] + z);
+                        j = start[c];
+                        jend = start[c + 1];
+                    }
+                }
+            } else {
+                j = o0;
+                jend = o1;
+            }
+            ++row;
+        }
+    };
+    if (act) next_run();
+    int cnt = 0;
+
+    auto drain = [&]() {
+        for (int k = 0; __ballot(k < cnt) != 0ull; ++k) {
+            if (k < cnt) {
+                uint32_t jj = lst[k][tid];
+                pair_force(A, P, pi, vi, Pi, posm[jj], velm[jj], mp[jj]);
+            }
+        }
+        cnt = 0;
+    };
+
+    while (__ballot(act && j < jend) != 0ull) {
+        if (act && j < jend) {
+            float4 pj = posm[j];
+            bool in_ov = row > nrows;  // the overflow run is last
+            bool ok = (uint32_t)i != j;
+            if (ok && in_ov) ok = keys_adjacent(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, P.inv_cell);
+            if (ok) {
+                float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                float s = dx * dx + dy * dy + dz * dz;
+                int cl = classify(P, s, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z);
+                // forces skip r >= h and r <= 0 only; a NaN distance falls through
+                // (world.rs:297-299). r > 0 exactly when the points differ.
+                ok = (cl == 1 && (dx != 0.f || dy != 0.f || dz != 0.f)) || cl == 2;
+            }
+            if (ok) lst[cnt++][tid] = j;
+            ++j;
+            if (j >= jend) next_run();
+        }
+        if (__ballot(cnt == L) != 0ull) drain();
+    }
+    drain();
+    if (!act) return;
+
+    // power-law effective viscosity (world.rs:319-327)
+    int m = 0;
+    if (P.n_mat > 1) {
+        uint32_t mt = matv[i];
+        m = mt < (uint32_t)P.n_mat ? (int)mt : 0;
+    }
+    float kc = P.visc_k[m];
+    float mu;
+    if (P.newtonian) {
+        mu = kc;  // powf(x, 0) == 1 for every x
+    } else {
+        float sreg = A.sh > P.shear_min ? A.sh : P.shear_min;
+        mu = kc * powf(sreg, P.visc_n[m] - 1.f);
+    }
+    float acc[3] = {0.f, 0.f, 0.f};
+    if (rho_i > 0.f) {
+        float mcs = pi.w * P.cs, mcv = pi.w * P.cv * mu;
+        for (int a = 0; a < 3; ++a) acc[a] = (mcs * A.fp[a] + mcv * A.fv[a]) / rho_i;
+    }
+    float xs[3] = {P.c6 * A.xs[0], P.c6 * A.xs[1], P.c6 * A.xs[2]};
+
+    // integrate (world.rs:352-363) and enforce_bounds (366-397)
+    float bacc[3] = {0.f, 0.f, 0.f};
+    if (body) {
+        float4 b = body[i];
+        bacc[0] = b.x;
+        bacc[1] = b.y;
+        bacc[2] = b.z;
+    }
+    float x[3] = {pi.x, pi.y, pi.z}, v[3] = {vi.x, vi.y, vi.z};
+    for (int a = 0; a < 3; ++a) {
+        float aa = (acc[a] + P.g[a]) + bacc[a];
+        v[a] = v[a] + aa * dt;
+        v[a] = v[a] + xs[a] * P.xsph_eps;
+        x[a] = x[a] + v[a] * dt;
+        if (x[a] < P.lo[a]) {
+            x[a] = P.lo[a];
+            v[a] = -v[a] * P.damp;
+        } else if (x[a] > P.hi[a]) {
+            x[a] = P.hi[a];
+            v[a] = -v[a] * P.damp;
+        }
+    }
+    posm_out[i] = make_float4(x[0], x[1], x[2], pi.w);
+    vel_out[i] = make_float4(v[0], v[1], v[2], 0.f);
+    if (acc_out) {
+        acc_out[i] = make_float4(acc[0], acc[1], acc[2], 0.f);
+        mu_out[i] = mu;
+        xsph_out[i] = make_float4(xs[0], xs[1], xs[2], 0.f);
+    }
+
+    // watchdog: smallest original index with a non-finite position / velocity
+    uint32_t id = idv[i];
+    uint32_t bp = finite3(x[0], x[1], x[2]) ? 0xffffffffu : id;
+    uint32_t bv = finite3(v[0], v[1], v[2]) ? 0xffffffffu : id;
+    if (__ballot(bp != 0xffffffffu || bv != 0xffffffffu) != 0ull) {
+        if (bp != 0xffffffffu) atomicMin(&dom->bad_pos, bp);
+        if (bv != 0xffffffffu) atomicMin(&dom->bad_vel, bv);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// FP64 diagnostics (subsystem.rs:149-163): mass, momentum, kinetic energy.
+// Block partials in a fixed order, then one block sums the partials in order.
+
+__global__ __launch_bounds__(256) void k_diag_partial(int n, const float4* __restrict__ posm,
+                                                      const float4* __restrict__ vel,
+                                                      double* partial) {
+    __shared__ double sh[5][256];
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        double m = posm[s].w;
+        float4 v = vel[s];
+        double vx = v.x, vy = v.y, vz = v.z;
+        acc[0] += m;
+        acc[1] += vx * m;
+        acc[2] += vy * m;
+        acc[3] += vz * m;
+        acc[4] += (((vx * vx + vy * vy) + vz * vz) * m) / 2.0;
+    }
+    for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+            for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 5; ++k) partial[blockIdx.x * 5 + k] = sh[k][0];
+}
+
+__global__ void k_diag_final(int nb, const double* __restrict__ partial, double* out) {
+    if (threadIdx.x != 0) return;
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (int b = 0; b < nb; ++b)
+        for (int k = 0; k < 5; ++k) acc[k] += partial[b * 5 + k];
+    for (int k = 0; k < 5; ++k) out[k] = acc[k];
+}
+
+// ---------------------------------------------------------------------------
+// original-order gather for readback: out[id[s]] = in[s]
+
+__global__ __launch_bounds__(256) void k_unsort4(int n, const uint32_t* __restrict__ id,
+                                                 const float4* __restrict__ in, float4* out) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) out[id[s]] = in[s];
+}
+__global__ __launch_bounds__(256) void k_unsort1(int n, const uint32_t* __restrict__ id,
+                                                 const float* __restrict__ in, float* out) {
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n) out[id[s]] = in[s];
+}
+
+// neighbour statistics of the density predicate, for exact set comparison:
+// count, sum of original indices, xor of SplitMix64-hashed original indices.
+__global__ __launch_bounds__(256) void k_neighbor_stats(int n, const float4* __restrict__ posm,
+                                                        const uint32_t* __restrict__ cellv,
+                                                        const uint32_t* __restrict__ start,
+                                                        const uint32_t* __restrict__ idv,
+                                                        const Domain* __restrict__ dom,
+                                                        DevParams P, uint32_t* cnt_out,
+                                                        unsigned long long* sum_out,
+                                                        unsigned long long* xor_out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Domain d = *dom;
+    float4 pi = posm[i];
+    uint32_t c = 0;
+    unsigned long long s1 = 0, x1 = 0;
+    for_each_candidate(cellv[i], pi.x, pi.y, pi.z, d, P.inv_cell, start, posm, [&](uint32_t j) {
+        float4 pj = posm[j];
+        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+        float s = dx * dx + dy * dy + dz * dz;
+        if (classify(P, s, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) == 1) {
+            unsigned long long id = idv[j];
+            ++c;
+            s1 += id;
+            unsigned long long z = id + 0x9E3779B97F4A7C15ull;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            x1 ^= z ^ (z >> 31);
+        }
+    });
+    uint32_t o = idv[i];
+    cnt_out[o] = c;
+    sum_out[o] = s1;
+    xor_out[o] = x1;
+}
+
+// ---------------------------------------------------------------------------
+// launch wrappers
+
+static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+void launch_domain(hipStream_t st, int n, const float4* posm, double inv, Domain* d,
+                   unsigned long long cap, const long long bk[6]) {
+    hipLaunchKernelGGL(k_domain_reset, dim3(1), dim3(1), 0, st, d);
+    if (n > 0) {
+        int nb = nblk(n, 256);
+        if (nb > 2048) nb = 2048;
+        hipLaunchKernelGGL(k_aabb, dim3(nb), dim3(256), 0, st, n, posm, inv, d);
+    }
+    hipLaunchKernelGGL(k_domain_finalize, dim3(1), dim3(1), 0, st, d, cap, bk[0], bk[1], bk[2],
+                       bk[3], bk[4], bk[5]);
+}
+
+void launch_sort(hipStream_t st, int n, const SortBuffers& b, const Domain* d, double inv) {
+    hipLaunchKernelGGL(k_bin, dim3(nblk(n, 256)), dim3(256), 0, st, n, b.posm_in, inv, d, b.cell,
+                       b.rank, b.counts);
+    hipLaunchKernelGGL(k_scan_partial, dim3(kScanBlocks), dim3(256), 0, st, b.counts, d, b.partial);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, b.partial, kScanBlocks, d, b.start);
+    hipLaunchKernelGGL(k_scan_final, dim3(kScanBlocks), dim3(256), 0, st, b.counts, d, b.partial,
+                       b.start);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, n, b.cell, b.rank, b.start,
+                       b.id_in, b.tmp, b.idtmp);
+    hipLaunchKernelGGL(k_reorder, dim3(nblk(n, 256)), dim3(256), 0, st, n, b.tmp, b.idtmp, b.cell,
+                       b.start, b.posm_in, b.vel_in, b.mat_in, b.body_in, b.posm_out, b.velm_out,
+                       b.id_out, b.mat_out, b.body_out, b.cell_out);
+}
+
+void launch_density(hipStream_t st, int n, const float4* posm, const uint32_t* cellv,
+                    const uint32_t* start, const Domain* d, const DevParams& P, float* rho,
+                    float* prs, float4* velm, float* mp) {
+    hipLaunchKernelGGL(k_density, dim3(nblk(n, 256)), dim3(256), 0, st, n, posm, cellv, start, d, P,
+                       rho, prs, velm, mp);
+}
+
+void launch_force(hipStream_t st, int n, float dt, const ForceArgs& a, Domain* d,
+                  const DevParams& P) {
+    hipLaunchKernelGGL(k_force<kForceList>, dim3(nblk(n, 256)), dim3(256), 0, st, n, dt, a.posm,
+                       a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body, d, P,
+                       a.posm_out, a.vel_out, a.acc_out, a.mu_out, a.xsph_out);
+}
+
+void launch_diag(hipStream_t st, int n, const float4* posm, const float4* vel, double* partial,
+                 double* out) {
+    int nb = kDiagBlocks;
+    hipLaunchKernelGGL(k_diag_partial, dim3(nb), dim3(256), 0, st, n, posm, vel, partial);
+    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, st, nb, partial, out);
+}
+
+void launch_unsort4(hipStream_t st, int n, const uint32_t* id, const float4* in, float4* out) {
+    hipLaunchKernelGGL(k_unsort4, dim3(nblk(n, 256)), dim3(256), 0, st, n, id, in, out);
+}
+void launch_unsort1(hipStream_t st, int n, const uint32_t* id, const float* in, float* out) {
+    hipLaunchKernelGGL(k_unsort1, dim3(nblk(n, 256)), dim3(256), 0, st, n, id, in, out);
+}
+
+void launch_neighbor_stats(hipStream_t st, int n, const float4* posm, const uint32_t* cellv,
+                           const uint32_t* start, const uint32_t* id, const Domain* d,
+                           const DevParams& P, uint32_t* cnt, unsigned long long* sum,
+                           unsigned long long* xr) {
+    hipLaunchKernelGGL(k_neighbor_stats, dim3(nblk(n, 256)), dim3(256), 0, st, n, posm, cellv,
+                       start, id, d, P, cnt, sum, xr);
+}
+
+}  // namespace phy

@@ -1,0 +1,60 @@
+// sph_types.h — plain-old-data shared by the host runtime and the gfx950 kernels.
+//
+// Particle state lives on the device in structure-of-arrays form, in cell order
+// (particles sorted by their uniform-grid cell, ties by original index):
+//   posm[s] = float4(x, y, z, mass)        16 B
+//   velm[s] = float4(vx, vy, vz, m/rho)    16 B  (w filled by the density pass)
+//   mp[s]   = mass * p / rho^2             4 B   (filled by the density pass)
+//   rho[s], prs[s]                         4 B each
+//   id[s]   = original particle index      4 B
+//   mat[s]  = material tag                 4 B   (only read with >1 material)
+// The reference keeps an AoS Vec<Particle<f64>> of 160 B per particle
+// (crates/phy-fluid/src/particle.rs:17-46).
+#pragma once
+#include <stdint.h>
+
+namespace phy {
+
+constexpr int kMaxMaterials = 16;
+// Largest dense cell table (cells) one world may allocate; particles whose cell
+// falls outside the table go to the overflow segment (exact, but O(n) per visit).
+constexpr uint64_t kMaxDenseCells = 1ull << 26;
+
+// Step constants in device-ready form. The neighbour predicate and the cell
+// keys use the exact f64 support radius; the per-pair arithmetic runs in FP32
+// with coefficients computed in f64 and rounded once.
+struct DevParams {
+    double h64;          // SphParams::h
+    double inv_cell;     // 1.0 / h in f64 (grid.rs:27: T::one() / self.cell)
+    double s_thresh;     // smallest f64 s with sqrt(s) >= h: r < h  <=>  s < s_thresh
+    float s_lo, s_hi;    // FP32 filter: s32 < s_lo => inside; s32 > s_hi => outside
+    float h, h2;         // FP32 support radius and its square
+    float c6, cs, cv;    // poly6, spiky-gradient, viscosity-laplacian coefficients
+    float rest, stiff, shear_min, r_eps;
+    float g[3], lo[3], hi[3];
+    float damp, xsph_eps;
+    int32_t n_mat;       // number of material entries used (visc_k.len())
+    int32_t newtonian;   // 1 when every visc_n entry is exactly 1 (mu_eff = visc_k)
+    float visc_k[kMaxMaterials];
+    float visc_n[kMaxMaterials];
+};
+
+// Dense grid domain for one step. Keys are the reference's i64 cell keys
+// (grid.rs:26-33); the table covers kmin .. kmin + dims - 1 per axis with z the
+// fastest-varying axis, so the three z-adjacent cells of a neighbour row are one
+// contiguous run of sorted particles. Cell index ncells is the overflow segment.
+struct Domain {
+    long long kmin[3];
+    long long dims[3];
+    uint32_t ncells;
+    uint32_t pad;
+    // AABB of the finite keys of the current positions (filled by k_aabb).
+    long long amin[3];
+    long long amax[3];
+    // Watchdog (subsystem.rs:74-90): smallest original index with a non-finite
+    // position / velocity after the step; 0xffffffff when none.
+    uint32_t bad_pos;
+    uint32_t bad_vel;
+};
+
+}  // namespace phy

@@ -1,0 +1,334 @@
+"""ctypes binding of libphy_ffi.so (include/phy_ffi.h).
+
+Python mirror of the reference's fluid path, called only through the C ABI:
+``FluidWorld`` follows crates/phy-fluid/src/sph/world.rs (new, fill_box,
+lattice_mass, step, particle access) and ``World`` the World/step_checked
+contract of crates/phy-core/src/world.rs:224-293. There is no CPU fallback:
+without the built library or a GPU every constructor raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libphy_ffi.so")
+MAX_MAT = 16
+
+FIELDS = {"pos": 0, "vel": 1, "acc": 2, "xsph": 3, "rho": 4, "p": 5, "mass": 6, "mu_eff": 7}
+VEC_FIELDS = {"pos", "vel", "acc", "xsph"}
+
+# every symbol include/phy_ffi.h declares
+EXPORTS = [
+    "phy_ffi_abi_version", "phy_world_create_fluid", "phy_world_create_rigid",
+    "phy_world_create_rigid_empty", "phy_world_create_granular", "phy_world_create_coupled",
+    "phy_world_create_soft_empty", "phy_world_create_field_empty", "phy_world_create_optics_empty",
+    "phy_world_create_solid_empty", "phy_world_step", "phy_world_step_checked", "phy_world_time",
+    "phy_world_sub_count", "phy_world_fluid_count", "phy_world_get_fluid_positions",
+    "phy_world_get_fluid_velocities", "phy_world_rigid_count", "phy_world_get_rigid_transforms",
+    "phy_world_save", "phy_world_load", "phy_world_destroy", "phy_world_step_f32",
+    "phy_world_get_fluid_positions_f32", "phy_world_get_fluid_velocities_f32",
+    "phy_world_get_rigid_transforms_f32", "phy_world_rigid_add_body",
+    "phy_world_rigid_apply_force", "phy_world_rigid_apply_torque", "phy_world_rigid_get_velocity",
+    "phy_world_rigid_get_angular_velocity", "phy_last_error", "phy_sph_params_defaults",
+    "phy_world_create_fluid_with", "phy_fluid_set_params", "phy_fluid_get_params",
+    "phy_fluid_fill_box", "phy_fluid_lattice_mass", "phy_fluid_set_particles",
+    "phy_fluid_set_particles_f32", "phy_fluid_get_field", "phy_fluid_get_field_f32",
+    "phy_fluid_set_keep_aux", "phy_fluid_set_body_acc", "phy_fluid_diagnostics_f64",
+    "phy_fluid_neighbor_stats", "phy_world_step_n", "phy_fluid_sync", "phy_fluid_profile_begin",
+    "phy_fluid_profile_end",
+]
+
+
+class PhySphParams(C.Structure):
+    """``PhySphParams`` = SphParams<f64> (crates/phy-fluid/src/sph/params.rs:12-50)."""
+
+    _fields_ = [
+        ("rest_density", C.c_double),
+        ("stiffness", C.c_double),
+        ("viscosity", C.c_double),
+        ("visc_k", C.c_double * MAX_MAT),
+        ("visc_n", C.c_double * MAX_MAT),
+        ("n_visc_k", C.c_uint32),
+        ("n_visc_n", C.c_uint32),
+        ("shear_min", C.c_double),
+        ("mass", C.c_double),
+        ("h", C.c_double),
+        ("gravity", C.c_double * 3),
+        ("bounds_min", C.c_double * 3),
+        ("bounds_max", C.c_double * 3),
+        ("boundary_damp", C.c_double),
+        ("xsph_eps", C.c_double),
+    ]
+
+    @classmethod
+    def defaults(cls) -> "PhySphParams":
+        p = cls()
+        lib().phy_sph_params_defaults(C.byref(p))
+        return p
+
+    def set_visc(self, k, n) -> None:
+        k, n = list(k), list(n)
+        for i, v in enumerate(k):
+            self.visc_k[i] = v
+        for i, v in enumerate(n):
+            self.visc_n[i] = v
+        self.n_visc_k, self.n_visc_n = len(k), len(n)
+
+    def copy(self) -> "PhySphParams":
+        q = PhySphParams()
+        C.memmove(C.byref(q), C.byref(self), C.sizeof(PhySphParams))
+        return q
+
+
+_lib = None
+
+
+class PhyError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libphy_ffi.so; raises when it is missing (no CPU fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise PhyError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    L = C.CDLL(LIB_PATH)
+    H = C.c_void_p
+    dp, fp, up = C.POINTER(C.c_double), C.POINTER(C.c_float), C.c_size_t
+    u32p, u64p = C.POINTER(C.c_uint32), C.POINTER(C.c_uint64)
+    sig = {
+        "phy_ffi_abi_version": ([], C.c_uint32),
+        "phy_world_create_fluid": ([], H),
+        "phy_world_create_rigid": ([], H),
+        "phy_world_create_rigid_empty": ([], H),
+        "phy_world_create_granular": ([], H),
+        "phy_world_create_coupled": ([], H),
+        "phy_world_create_soft_empty": ([], H),
+        "phy_world_create_field_empty": ([], H),
+        "phy_world_create_optics_empty": ([], H),
+        "phy_world_create_solid_empty": ([], H),
+        "phy_world_step": ([H, C.c_double], C.c_int32),
+        "phy_world_step_n": ([H, C.c_double, C.c_uint32], C.c_int32),
+        "phy_world_step_checked": ([H, C.c_double], C.c_int32),
+        "phy_world_step_f32": ([H, C.c_float], C.c_int32),
+        "phy_world_time": ([H], C.c_double),
+        "phy_world_sub_count": ([H], up),
+        "phy_world_fluid_count": ([H], up),
+        "phy_world_get_fluid_positions": ([H, dp, up], up),
+        "phy_world_get_fluid_velocities": ([H, dp, up], up),
+        "phy_world_get_fluid_positions_f32": ([H, fp, up], up),
+        "phy_world_get_fluid_velocities_f32": ([H, fp, up], up),
+        "phy_world_rigid_count": ([H], up),
+        "phy_world_get_rigid_transforms": ([H, dp, up], up),
+        "phy_world_get_rigid_transforms_f32": ([H, fp, up], up),
+        "phy_world_rigid_add_body": ([H, C.c_int32, C.c_double, dp, dp], C.c_int64),
+        "phy_world_rigid_apply_force": ([H, C.c_int64, dp, C.c_double, C.c_int32], C.c_int32),
+        "phy_world_rigid_apply_torque": ([H, C.c_int64, dp, C.c_double, C.c_int32], C.c_int32),
+        "phy_world_rigid_get_velocity": ([H, C.c_int64, dp], C.c_int32),
+        "phy_world_rigid_get_angular_velocity": ([H, C.c_int64, dp], C.c_int32),
+        "phy_world_save": ([H, C.c_char_p], C.c_int32),
+        "phy_world_load": ([C.c_char_p], H),
+        "phy_world_destroy": ([H], None),
+        "phy_last_error": ([], C.c_char_p),
+        "phy_sph_params_defaults": ([C.POINTER(PhySphParams)], None),
+        "phy_world_create_fluid_with": ([C.POINTER(PhySphParams)], H),
+        "phy_fluid_set_params": ([H, C.POINTER(PhySphParams)], C.c_int32),
+        "phy_fluid_get_params": ([H, C.POINTER(PhySphParams)], C.c_int32),
+        "phy_fluid_fill_box": ([H, dp, dp, C.c_double, C.c_double], C.c_int32),
+        "phy_fluid_lattice_mass": ([H, C.c_double], C.c_double),
+        "phy_fluid_set_particles": ([H, up, dp, dp, dp, u32p], C.c_int32),
+        "phy_fluid_set_particles_f32": ([H, up, fp, fp, fp, u32p], C.c_int32),
+        "phy_fluid_get_field": ([H, C.c_int32, dp, up], up),
+        "phy_fluid_get_field_f32": ([H, C.c_int32, fp, up], up),
+        "phy_fluid_set_keep_aux": ([H, C.c_int32], C.c_int32),
+        "phy_fluid_set_body_acc": ([H, dp, up], C.c_int32),
+        "phy_fluid_diagnostics_f64": ([H, dp], C.c_int32),
+        "phy_fluid_neighbor_stats": ([H, u32p, u64p, u64p, up], C.c_int32),
+        "phy_fluid_sync": ([H], C.c_int32),
+        "phy_fluid_profile_begin": ([H], C.c_int32),
+        "phy_fluid_profile_end": ([H, dp, u32p], C.c_int32),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    return (lib().phy_last_error() or b"").decode(errors="replace")
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class World:
+    """A ``World<f64>`` handle (crates/phy-core/src/world.rs:78-318) from the C ABI."""
+
+    def __init__(self, handle):
+        if not handle:
+            raise PhyError(f"world creation failed: {last_error()}")
+        self._h = handle
+
+    @classmethod
+    def create_fluid(cls) -> "World":
+        """``phy_world_create_fluid`` (crates/phy-ffi/src/lib.rs:113-126)."""
+        return cls(lib().phy_world_create_fluid())
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().phy_world_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def step(self, dt: float, n: int = 1) -> None:
+        rc = lib().phy_world_step(self._h, dt) if n == 1 else lib().phy_world_step_n(self._h, dt, n)
+        if rc != 0:
+            raise PhyError(f"phy_world_step rc={rc}: {last_error()}")
+
+    def step_checked(self, dt: float) -> int:
+        return int(lib().phy_world_step_checked(self._h, dt))
+
+    def time(self) -> float:
+        return lib().phy_world_time(self._h)
+
+    def sync(self) -> None:
+        if lib().phy_fluid_sync(self._h) != 0:
+            raise PhyError(last_error())
+
+    def fluid_count(self) -> int:
+        return int(lib().phy_world_fluid_count(self._h))
+
+    def positions(self) -> np.ndarray:
+        n = self.fluid_count()
+        out = np.zeros(3 * n)
+        got = lib().phy_world_get_fluid_positions(self._h, _dp(out), out.size)
+        assert got == n
+        return out.reshape(n, 3)
+
+    def velocities(self) -> np.ndarray:
+        n = self.fluid_count()
+        out = np.zeros(3 * n)
+        got = lib().phy_world_get_fluid_velocities(self._h, _dp(out), out.size)
+        assert got == n
+        return out.reshape(n, 3)
+
+
+class FluidWorld(World):
+    """``FluidWorld<f64>`` in a one-subsystem World, computed on the GPU in FP32."""
+
+    def __init__(self, params: PhySphParams | None = None):
+        self.params = (params or PhySphParams.defaults()).copy()
+        super().__init__(lib().phy_world_create_fluid_with(C.byref(self.params)))
+
+    def set_params(self, params: PhySphParams) -> None:
+        self.params = params.copy()
+        if lib().phy_fluid_set_params(self._h, C.byref(self.params)) != 0:
+            raise PhyError(last_error())
+
+    def __len__(self) -> int:
+        return self.fluid_count()
+
+    def fill_box(self, mn, mx, spacing: float, margin: float) -> None:
+        a = np.ascontiguousarray(mn, dtype=np.float64)
+        b = np.ascontiguousarray(mx, dtype=np.float64)
+        if lib().phy_fluid_fill_box(self._h, _dp(a), _dp(b), spacing, margin) != 0:
+            raise PhyError(last_error())
+
+    def lattice_mass(self, s: float) -> float:
+        return lib().phy_fluid_lattice_mass(self._h, s)
+
+    def set_particles(self, pos, vel=None, mass=None, material=None) -> None:
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, 3)
+        n = pos.shape[0]
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64).reshape(n, 3)
+        mass = None if mass is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(mass, dtype=np.float64), (n,)))
+        mat = None if material is None else np.ascontiguousarray(material, dtype=np.uint32)
+        rc = lib().phy_fluid_set_particles(
+            self._h, n, _dp(pos), _dp(vel), _dp(mass),
+            None if mat is None else mat.ctypes.data_as(C.POINTER(C.c_uint32)))
+        if rc != 0:
+            raise PhyError(last_error())
+
+    def set_particles_f32(self, pos, vel=None, mass=None, material=None) -> None:
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        n = pos.shape[0]
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(n, 3)
+        mass = None if mass is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(mass, dtype=np.float32), (n,)))
+        mat = None if material is None else np.ascontiguousarray(material, dtype=np.uint32)
+        rc = lib().phy_fluid_set_particles_f32(
+            self._h, n, _fp(pos), _fp(vel), _fp(mass),
+            None if mat is None else mat.ctypes.data_as(C.POINTER(C.c_uint32)))
+        if rc != 0:
+            raise PhyError(last_error())
+
+    def get(self, field: str, f32: bool = False) -> np.ndarray:
+        n = len(self)
+        comps = 3 if field in VEC_FIELDS else 1
+        out = np.zeros(n * comps, dtype=np.float32 if f32 else np.float64)
+        fn = lib().phy_fluid_get_field_f32 if f32 else lib().phy_fluid_get_field
+        got = fn(self._h, FIELDS[field], _fp(out) if f32 else _dp(out), out.size)
+        if got != n:
+            raise PhyError(f"get({field}) wrote {got} of {n}: {last_error()}")
+        return out.reshape(n, 3) if comps == 3 else out
+
+    def set_keep_aux(self, on: bool = True) -> None:
+        if lib().phy_fluid_set_keep_aux(self._h, 1 if on else 0) != 0:
+            raise PhyError(last_error())
+
+    def set_body_acc(self, b) -> None:
+        b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
+        if lib().phy_fluid_set_body_acc(self._h, _dp(b), b.size) != 0:
+            raise PhyError(last_error())
+
+    def diagnostics(self) -> np.ndarray:
+        out = np.zeros(5)
+        if lib().phy_fluid_diagnostics_f64(self._h, _dp(out)) != 0:
+            raise PhyError(last_error())
+        return out
+
+    def profile_begin(self) -> None:
+        if lib().phy_fluid_profile_begin(self._h) != 0:
+            raise PhyError(last_error())
+
+    def profile_end(self):
+        """Summed ms of {sort, density, force, step} and the steps timed."""
+        ms = np.zeros(4)
+        steps = C.c_uint32(0)
+        if lib().phy_fluid_profile_end(self._h, _dp(ms), C.byref(steps)) != 0:
+            raise PhyError(last_error())
+        return ms, int(steps.value)
+
+    def neighbor_stats(self):
+        n = len(self)
+        c = np.zeros(n, dtype=np.uint32)
+        s = np.zeros(n, dtype=np.uint64)
+        x = np.zeros(n, dtype=np.uint64)
+        rc = lib().phy_fluid_neighbor_stats(
+            self._h, c.ctypes.data_as(C.POINTER(C.c_uint32)), s.ctypes.data_as(C.POINTER(C.c_uint64)),
+            x.ctypes.data_as(C.POINTER(C.c_uint64)), n)
+        if rc != 0:
+            raise PhyError(last_error())
+        return c, s, x

@@ -1,0 +1,108 @@
+"""Synthetic dam-break scenes of the benchmark configurations.
+
+Randomness comes from the reference's portable SplitMix64 ``Rng``
+(crates/phy-core/src/replay.rs:29-53), so every scene is a pure function of
+its seed. Lattice blocks follow ``FluidWorld::fill_box``'s index order
+(x outer, y, z inner; crates/phy-fluid/src/sph/world.rs:130-143) and carry the
+mass of ``FluidWorld::lattice_mass`` (world.rs:148-169).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+H32 = float(np.float32(0.2))     # FP32-representable support radius (0.2f)
+S32 = float(np.float32(0.1))     # lattice spacing h/2 (0.1f)
+DT = 0.01                        # the README / perf_sweep step (no default in the reference)
+
+
+def splitmix64(seed: int, count: int) -> np.ndarray:
+    """``Rng::next_u64`` for ``count`` draws from ``Rng::seed(seed)`` (replay.rs:40-46)."""
+    with np.errstate(over="ignore"):
+        k = np.arange(1, count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + k * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return z ^ (z >> np.uint64(31))
+
+
+def rng_f64(seed: int, count: int) -> np.ndarray:
+    """``Rng::next_f64`` = (u >> 11) / 2^53 (replay.rs:49-52)."""
+    u = splitmix64(seed, count)
+    return (u >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+
+
+def kernel_coeffs(h: float):
+    """(poly6, spiky-gradient, viscosity-laplacian) coefficients, kernels.rs:22-35."""
+    pi = math.pi
+    h2 = h * h
+    h3 = h2 * h
+    h6 = h3 * h3
+    h9 = h6 * h3
+    return 315.0 / (64.0 * pi * h9), -45.0 / (pi * h6), 45.0 / (pi * h6)
+
+
+def lattice_mass(h: float, s: float, rest_density: float = 1000.0, fallback: float = 0.02) -> float:
+    """``FluidWorld::lattice_mass`` (world.rs:148-169) in f64."""
+    c6, _, _ = kernel_coeffs(h)
+    h2 = h * h
+    n = int(math.ceil(h / s)) + 1
+    sum_w = 0.0
+    for i in range(-n, n + 1):
+        for j in range(-n, n + 1):
+            for l in range(-n, n + 1):
+                rr = math.sqrt(float(i * i + j * j + l * l))
+                r = s * rr
+                if r < h:
+                    d = h2 - r * r
+                    sum_w += c6 * d * d * d
+    return rest_density / sum_w if sum_w > 0.0 else fallback
+
+
+@dataclass
+class Scene:
+    pos: np.ndarray          # (n, 3) float32
+    vel: np.ndarray          # (n, 3) float32
+    mass: float
+    h: float
+    bounds_min: tuple
+    bounds_max: tuple
+    dt: float = DT
+
+
+def lattice_block(nx: int, ny: int, nz: int, spacing: float, origin, jitter_frac: float,
+                  seed: int) -> np.ndarray:
+    """Lattice block (x outer, y, z inner) with each coordinate jittered by
+    spacing * jitter_frac * (2u - 1), u drawn in particle order x, y, z."""
+    ix, iy, iz = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    idx = np.stack([ix.ravel(), iy.ravel(), iz.ravel()], 1).astype(np.float64)
+    pos = np.asarray(origin, dtype=np.float64) + (idx + 0.5) * spacing
+    if jitter_frac:
+        u = rng_f64(seed, pos.size).reshape(-1, 3)
+        pos = pos + spacing * jitter_frac * (2.0 * u - 1.0)
+    return pos.astype(np.float32)
+
+
+def dam_break(n_side=(100, 100, 100), seed: int = 1, h: float = H32, spacing: float = S32,
+              tank_factor: float = 4.0, jitter_frac: float = 0.01) -> Scene:
+    """The 1M-particle FP32 dam break at its default size: a column of
+    100 x 100 x 100 particles at spacing h/2 in the corner of a closed cubic
+    tank 4x the column width, +-1 % seeded jitter, zero velocity."""
+    nx, ny, nz = n_side
+    pos = lattice_block(nx, ny, nz, spacing, (0.0, 0.0, 0.0), jitter_frac, seed)
+    width = max(nx, nz) * spacing
+    side = tank_factor * width
+    m = lattice_mass(h, spacing)
+    return Scene(pos=pos, vel=np.zeros_like(pos), mass=m, h=h, bounds_min=(0.0, 0.0, 0.0),
+                 bounds_max=(side, side, side))
+
+
+def apply_scene(params, scene: Scene):
+    """Copy a scene's h and tank bounds into a PhySphParams-like structure."""
+    params.h = scene.h
+    for a in range(3):
+        params.bounds_min[a] = scene.bounds_min[a]
+        params.bounds_max[a] = scene.bounds_max[a]
+    return params

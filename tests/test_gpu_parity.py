@@ -1,0 +1,243 @@
+"""GPU parity of the HIP step against the f64 oracle, through the C ABI.
+
+Bars: neighbour sets bit-exact; every per-step field within 1e-5 norm-wise
+relative of the oracle on identical FP32 inputs (parity.TOL_STEP); repeat runs
+bit-identical; the watchdog and the reference's FFI contract as in
+crates/phy-ffi/src/lib.rs:906-1001.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import parity as P
+from physics_amd import ffi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _block(n, seed, jit=0.01, s=scenes.S32, origin=(0.0, 0.0, 0.0)):
+    return scenes.lattice_block(n, n, n, s, origin, jit, seed)
+
+
+def _print(tag, err):
+    print(tag, " ".join(f"{k}={v:.2e}" for k, v in err.items()), flush=True)
+
+
+# ---- neighbour sets ---------------------------------------------------------
+
+@pytest.mark.parametrize("seed,jit", [(1, 0.01), (2, 0.3), (3, 0.5)])
+def test_neighbor_sets_bit_exact(seed, jit):
+    pos = _block(12, seed, jit, origin=(-0.6, -0.6, -0.6))
+    g, o = P.make_pair(pos, bounds=((-5, -5, -5), (5, 5, 5)))
+    gc, gs, gx = g.neighbor_stats()
+    oc, os_, ox = o.neighbor_stats()
+    assert np.array_equal(gc, oc)
+    assert np.array_equal(gs, os_)
+    assert np.array_equal(gx, ox)
+
+
+def test_neighbor_sets_at_the_support_boundary():
+    # pairs placed within a few ulps of r = h exercise the exact f64 fallback
+    h = scenes.H32
+    rng = np.random.default_rng(7)
+    base = rng.uniform(-1, 1, (400, 3)).astype(np.float32)
+    d = rng.normal(size=(400, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    eps = rng.integers(-4, 5, 400) * 1e-8
+    other = (base.astype(np.float64) + d * (h + eps)[:, None]).astype(np.float32)
+    pos = np.concatenate([base, other])
+    g, o = P.make_pair(pos)
+    gc, gs, gx = g.neighbor_stats()
+    oc, os_, ox = o.neighbor_stats()
+    assert np.array_equal(gc, oc) and np.array_equal(gs, os_) and np.array_equal(gx, ox)
+    # the band really is populated: some pairs in, some out
+    assert (oc[:400] == 2).any() and (oc[:400] == 1).any()
+
+
+# ---- one-step parity --------------------------------------------------------
+
+@pytest.mark.parametrize("n,seed", [(8, 1), (16, 2), (24, 3)])
+def test_one_step_parity_column(n, seed):
+    sc = scenes.dam_break((n, n, n), seed=seed)
+    g, o = P.make_pair(sc.pos, mass=sc.mass, bounds=(sc.bounds_min, sc.bounds_max))
+    x0 = P.step_both(g, o)
+    err = P.compare_step(g, o, x0)
+    _print(f"column n={n}", err)
+    for k, v in err.items():
+        assert v <= P.TOL_STEP, (k, v)
+
+
+def test_one_step_parity_moving_compressed_multimaterial():
+    # velocities, heavier particles (p > 0 everywhere), two non-Newtonian materials
+    pos = _block(14, 9, jit=0.2, origin=(-0.7, -0.7, -0.7))
+    n = len(pos)
+    vel = (scenes.rng_f64(4, 3 * n).reshape(n, 3) - 0.5).astype(np.float32)
+    mat = (np.arange(n) % 3).astype(np.uint32)
+    g, o = P.make_pair(pos, vel, mass=1.6, material=mat, visc_k=(3.5, 10.0, 1.0),
+                       visc_n=(1.0, 0.6, 1.4))
+    x0 = P.step_both(g, o)
+    err = P.compare_step(g, o, x0)
+    _print("moving", err)
+    for k, v in err.items():
+        assert v <= P.TOL_STEP, (k, v)
+
+
+def test_per_step_parity_over_a_collapse():
+    # the oracle is re-fed the GPU state every step: the per-step bar holds along
+    # a trajectory that compresses the column against the floor
+    sc = scenes.dam_break((12, 12, 12), seed=4)
+    g, o = P.make_pair(sc.pos, mass=sc.mass, bounds=(sc.bounds_min, sc.bounds_max))
+    worst = {}
+    for k in range(60):
+        P.sync_oracle_to_gpu(g, o)
+        x0 = P.step_both(g, o)
+        err = P.compare_step(g, o, x0)
+        for f, v in err.items():
+            worst[f] = max(worst.get(f, 0.0), v)
+    _print("per-step worst", worst)
+    for f, v in worst.items():
+        assert v <= P.TOL_STEP, (f, v)
+
+
+def test_trajectory_parity_first_20_steps():
+    # independent runs (no re-feeding); reported per step, bounded loosely
+    sc = scenes.dam_break((20, 20, 20), seed=1)
+    g, o = P.make_pair(sc.pos, mass=sc.mass, bounds=(sc.bounds_min, sc.bounds_max))
+    for k in range(20):
+        g.step(P.DT32)
+        o.step(P.DT32)
+    e_pos = P.normwise(g.get("pos"), o.get("pos"))
+    e_vel = P.normwise(g.get("vel"), o.get("vel"))
+    e_rho = P.normwise(g.get("rho"), o.get("rho"))
+    print(f"20-step trajectory: pos={e_pos:.2e} vel={e_vel:.2e} rho={e_rho:.2e}", flush=True)
+    assert e_pos < 1e-5 and e_vel < 1e-3 and e_rho < 1e-4
+
+
+# ---- edge cases of the reference semantics -----------------------------------
+
+def test_overflow_and_nan_particles_match_oracle():
+    # one particle far outside the dense table (overflow segment), one NaN
+    # position (reference key 0 on that axis, poisons force sums of neighbours)
+    pos = _block(8, 5, jit=0.1, origin=(-0.4, -0.4, -0.4)).astype(np.float32)
+    far = np.array([[3.0e6, 0.01, 0.02]], dtype=np.float32)
+    nanp = np.array([[np.nan, 0.03, 0.05]], dtype=np.float32)
+    pos = np.concatenate([pos, far, nanp])
+    g, o = P.make_pair(pos, mass=1.2, bounds=((-5, -5, -5), (5, 5, 5)))
+    g.step(P.DT32)
+    o.step(P.DT32)
+    for f in ("pos", "vel", "rho"):
+        a, b = g.get(f), o.get(f)
+        assert np.array_equal(np.isnan(a), np.isnan(b)), f
+        fin = np.isfinite(b)
+        assert P.normwise(np.where(fin, a, 0), np.where(fin, b, 0)) <= P.TOL_STEP, f
+
+
+def test_coincident_single_and_empty():
+    g, o = P.make_pair(np.array([[0.1, 0.2, 0.3], [0.1, 0.2, 0.3], [0.15, 0.2, 0.3]], np.float32),
+                       mass=2.0)
+    x0 = P.step_both(g, o)
+    err = P.compare_step(g, o, x0)
+    for k, v in err.items():
+        assert v <= P.TOL_STEP, (k, v)
+    gp, _ = P.params_pair()
+    e = ffi.FluidWorld(gp)
+    e.step(0.01)                       # empty world: no-op (world.rs:184-186)
+    assert len(e) == 0 and e.time() == pytest.approx(0.01)
+
+
+# ---- determinism --------------------------------------------------------------
+
+def test_repeat_runs_are_bit_identical():
+    sc = scenes.dam_break((24, 24, 24), seed=2)
+    outs = []
+    for _ in range(2):
+        gp, _ = P.params_pair(bounds=(sc.bounds_min, sc.bounds_max))
+        g = ffi.FluidWorld(gp)
+        g.set_particles_f32(sc.pos, None, np.full(len(sc.pos), sc.mass, np.float32))
+        g.step(P.DT32, 150)
+        outs.append((g.get("pos", f32=True), g.get("vel", f32=True), g.get("rho", f32=True)))
+    for a, b in zip(*outs):
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+# ---- the reference FFI contract (crates/phy-ffi/src/lib.rs:906-1001) -----------
+
+def test_ffi_fluid_lifecycle_and_readback():
+    w = ffi.World.create_fluid()
+    assert w.fluid_count() == 10648
+    t0 = w.time()
+    for _ in range(200):
+        w.step(0.005)
+    assert abs(w.time() - t0 - 1.0) < 1e-6
+    assert np.isfinite(w.positions()).all()
+
+
+def test_ffi_checked_step_and_f32_channel():
+    w = ffi.World.create_fluid()
+    for _ in range(200):
+        assert w.step_checked(0.005) == 0
+    L = ffi.lib()
+    n = w.fluid_count()
+    b64 = np.zeros(3 * n)
+    b32 = np.zeros(3 * n, dtype=np.float32)
+    assert L.phy_world_get_fluid_positions(w.handle, ffi._dp(b64), b64.size) == n
+    assert L.phy_world_get_fluid_positions_f32(w.handle, ffi._fp(b32), b32.size) == n
+    assert np.array_equal(b32, b64.astype(np.float32))
+    # truncation at len / 3 particles
+    assert L.phy_world_get_fluid_positions(w.handle, ffi._dp(b64), 3 * 5 + 2) == 5
+
+
+def test_watchdog_catches_injected_nan():
+    gp = ffi.PhySphParams.defaults()
+    g = ffi.FluidWorld(gp)
+    g.fill_box((-4, -4, -1), (0, 4, 1), 0.18, 0.05)
+    for _ in range(5):
+        assert g.step_checked(0.005) == 0
+    pos, vel, m = g.get("pos"), g.get("vel"), g.get("mass")
+    pos[0, 0] = np.nan
+    g.set_particles(pos, vel, m)
+    assert g.step_checked(0.005) == 2
+
+
+def test_watchdog_reports_velocity_blowup():
+    # a huge dt drives the state non-finite; step_checked must stop with rc 2
+    gp = ffi.PhySphParams.defaults()
+    gp.stiffness = 1e30
+    g = ffi.FluidWorld(gp)
+    g.fill_box((-1, -1, -1), (1, 1, 1), 0.1, 0.05)
+    rcs = [g.step_checked(0.5) for _ in range(10)]
+    assert 2 in rcs
+
+
+# ---- FP64 diagnostics -------------------------------------------------------------
+
+def test_fp64_diagnostics_match_oracle():
+    pos = _block(16, 3, jit=0.1)
+    n = len(pos)
+    vel = (scenes.rng_f64(8, 3 * n).reshape(n, 3) - 0.5).astype(np.float32)
+    g, o = P.make_pair(pos, vel, mass=0.9)
+    d = g.diagnostics()
+    assert d[0] == pytest.approx(n * float(np.float32(0.9)), rel=1e-12)
+    np.testing.assert_allclose(d[1:4], o.total_momentum(), rtol=1e-10, atol=1e-10)
+    assert d[4] == pytest.approx(o.kinetic_energy(), rel=1e-10)
+
+
+# ---- full size (1M particles): the configuration the throughput is quoted on -------
+
+def test_full_size_column_one_step_parity_and_invariants():
+    sc = scenes.dam_break((100, 100, 100), seed=1)
+    g, o = P.make_pair(sc.pos, mass=sc.mass, bounds=(sc.bounds_min, sc.bounds_max))
+    x0 = P.step_both(g, o)
+    err = P.compare_step(g, o, x0)
+    _print("1M one step", err)
+    for k, v in err.items():
+        assert v <= P.TOL_STEP, (k, v)
+    gc, gs, gx = g.neighbor_stats()
+    oc, os_, ox = o.neighbor_stats()
+    assert np.array_equal(gc, oc) and np.array_equal(gs, os_) and np.array_equal(gx, ox)
+    m0 = g.diagnostics()[0]
+    g.step(P.DT32, 30)
+    x = g.get("pos")
+    assert len(x) == 1_000_000 and np.isfinite(x).all()
+    assert (x >= 0.0).all() and (x <= sc.bounds_max[0] * (1 + 1e-7)).all()
+    assert g.diagnostics()[0] == m0           # mass is carried, never recomputed
