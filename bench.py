@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--cpu-budget", type=float, default=20.0,
                     help="seconds of oracle CPU work for the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--verbose", action="store_true", help="per-chunk timings on stderr")
     return ap.parse_args()
 
 
@@ -84,6 +85,10 @@ def main():
         ms, s = g.profile_end()
         stage += ms
         timed_steps += s
+        if a.verbose:
+            print(f"[bench] chunk {c}: {k} steps, {1e3 * (time.perf_counter() - t0) / k:.3f} ms/step wall, "
+                  f"sort {ms[0] / s:.3f} density {ms[1] / s:.3f} force {ms[2] / s:.3f} ms/step",
+                  file=sys.stderr, flush=True)
         if not a.no_cpu:
             snaps.append((g.get("pos"), g.get("vel")))
     assert timed_steps == K

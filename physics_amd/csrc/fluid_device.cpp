@@ -48,6 +48,10 @@ DevParams make_dev_params(const HostParams& hp) {
     // kernels.rs:22-35 in f64
     const double pi = 3.141592653589793;
     double h2 = h * h, h3 = h2 * h, h6 = h3 * h3, h9 = h6 * h3;
+    P.h2_64 = h2;
+    P.c6_64 = 315.0 / (64.0 * pi * h9);
+    P.rest64 = hp.rest_density;
+    P.stiff64 = hp.stiffness;
     P.h = (float)h;
     P.h2 = (float)h2;
     P.c6 = (float)(315.0 / (64.0 * pi * h9));

@@ -10,6 +10,7 @@ namespace phy {
 constexpr int kScanBlocks = 1024;  // fixed grid of the three-phase cell scan
 constexpr int kDiagBlocks = 512;   // fixed grid of the FP64 diagnostics reduction
 constexpr int kForceList = 32;     // per-lane LDS neighbour list length of k_force
+constexpr int kDensityList = 32;   // per-lane LDS neighbour list length of k_density
 
 struct SortBuffers {
     // inputs (current order)

@@ -427,35 +427,155 @@ __device__ __forceinline__ void for_each_candidate(uint32_t ci, float xi, float 
 }
 
 // ---------------------------------------------------------------------------
-// density / EOS (world.rs:204-246)
+// per-lane neighbour compaction
+//
+// Every lane walks its own candidate sequence (rows 0..8 of the dense table, or
+// the 27 single cells of an out-of-domain particle, then the overflow segment)
+// one candidate per round; accepted neighbours are appended to the lane's LDS
+// list, and the whole wave drains the lists as soon as one is full. The heavy
+// per-neighbour work then runs on dense lists instead of on every candidate
+// under divergence (about 15 % of the 27-cell candidates are neighbours).
 
+struct Cursor {
+    int row, nrows;
+    uint32_t j, jend, o0, o1;
+    bool dense;
+    CellCoord cc;
+    long long k0, k1, k2;
+
+    __device__ __forceinline__ bool in_overflow() const { return row > nrows; }
+
+    __device__ __forceinline__ void next_run(const Domain& d, const uint32_t* __restrict__ start) {
+        while (j >= jend && row <= nrows) {
+            if (row < nrows) {
+                if (dense) {
+                    row_run(row, cc, d, start, j, jend);
+                } else {
+                    int di = row / 9 - 1, dj = (row / 3) % 3 - 1, dk = row % 3 - 1;
+                    unsigned long long x = (unsigned long long)k0 + (unsigned long long)(long long)di -
+                                           (unsigned long long)d.kmin[0];
+                    unsigned long long y = (unsigned long long)k1 + (unsigned long long)(long long)dj -
+                                           (unsigned long long)d.kmin[1];
+                    unsigned long long z = (unsigned long long)k2 + (unsigned long long)(long long)dk -
+                                           (unsigned long long)d.kmin[2];
+                    if (x >= (unsigned long long)d.dims[0] || y >= (unsigned long long)d.dims[1] ||
+                        z >= (unsigned long long)d.dims[2]) {
+                        j = jend = 0;
+                    } else {
+                        uint32_t c = (uint32_t)((x * d.dims[1] + y) * d.dims[2] + z);
+                        j = start[c];
+                        jend = start[c + 1];
+                    }
+                }
+            } else {
+                j = o0;
+                jend = o1;
+            }
+            ++row;
+        }
+    }
+
+    __device__ __forceinline__ void init(bool act, uint32_t ci, float xi, float yi, float zi,
+                                         const Domain& d, double inv,
+                                         const uint32_t* __restrict__ start) {
+        row = 0;
+        j = jend = 0;
+        o0 = start[d.ncells];
+        o1 = start[d.ncells + 1];
+        dense = ci < d.ncells;
+        nrows = dense ? 9 : 27;
+        k0 = k1 = k2 = 0;
+        cc = CellCoord{0, 0, 0};
+        if (!act) return;
+        if (dense) {
+            cc = decode(ci, d);
+        } else {
+            k0 = key1(xi, inv);
+            k1 = key1(yi, inv);
+            k2 = key1(zi, inv);
+        }
+        next_run(d, start);
+    }
+};
+
+template <int L, class Accept, class Proc>
+__device__ __forceinline__ void gather(bool act, Cursor& cur, const Domain& d,
+                                       const uint32_t* __restrict__ start,
+                                       const float4* __restrict__ posm, uint32_t (*lst)[256],
+                                       int tid, Accept&& accept, Proc&& proc) {
+    int cnt = 0;
+    auto drain = [&]() {
+        for (int k = 0; __ballot(k < cnt) != 0ull; ++k) {
+            if (k < cnt) proc(lst[k][tid]);
+        }
+        cnt = 0;
+    };
+    while (__ballot(act && cur.j < cur.jend) != 0ull) {
+        if (act && cur.j < cur.jend) {
+            uint32_t j = cur.j;
+            float4 pj = posm[j];
+            if (accept(j, pj, cur.in_overflow())) lst[cnt++][tid] = j;
+            ++cur.j;
+            if (cur.j >= cur.jend) cur.next_run(d, start);
+        }
+        if (__ballot(cnt == L) != 0ull) drain();
+    }
+    drain();
+}
+
+// ---------------------------------------------------------------------------
+// density / EOS (world.rs:204-246)
+//
+// The pair geometry and the sum run in f64: p = k (rho - rho0) cancels near
+// the rest density and would amplify an FP32 density error of ~2e-7 by
+// rho / (rho - rho0). MI355X executes f64 FMA at the unpacked FP32 rate, and the
+// compaction limits it to true neighbours.
+
+template <int L>
 __global__ __launch_bounds__(256) void k_density(int n, const float4* __restrict__ posm,
                                                  const uint32_t* __restrict__ cellv,
                                                  const uint32_t* __restrict__ start,
                                                  const Domain* __restrict__ dom, DevParams P,
                                                  float* rho_out, float* p_out, float4* velm,
                                                  float* mp_out) {
-    int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Domain d = *dom;
-    float4 pi = posm[i];
-    float sum = 0.f, comp = 0.f;
-    for_each_candidate(cellv[i], pi.x, pi.y, pi.z, d, P.inv_cell, start, posm, [&](uint32_t j) {
-        float4 pj = posm[j];
-        float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-        float s = dx * dx + dy * dy + dz * dz;
-        if (classify(P, s, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) == 1) {
-            float q = P.h2 - s;
-            kahan_add(sum, comp, pj.w * (q * q * q));
-        }
-    });
-    float rho = P.c6 * sum;
-    float over = rho - P.rest;
-    float p = over > 0.f ? P.stiff * over : 0.f;
-    rho_out[i] = rho;
-    p_out[i] = p;
-    velm[i].w = pi.w / rho;            // m_j / rho_j of the force pass
-    mp_out[i] = pi.w * (p / (rho * rho));  // m_j * p_j / rho_j^2
+    __shared__ uint32_t lst[L][256];
+    const int tid = threadIdx.x;
+    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + tid;
+    const bool act = i < n;
+    const Domain d = *dom;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t ci = d.ncells;
+    if (act) {
+        pi = posm[i];
+        ci = cellv[i];
+    }
+    const double xi = pi.x, yi = pi.y, zi = pi.z;
+    Cursor cur;
+    cur.init(act, ci, pi.x, pi.y, pi.z, d, P.inv_cell, start);
+    double sum = 0.0;
+    gather<L>(
+        act, cur, d, start, posm, lst, tid,
+        [&](uint32_t j, float4 pj, bool in_ov) {
+            if (in_ov && !keys_adjacent(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, P.inv_cell)) return false;
+            float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            float s = dx * dx + dy * dy + dz * dz;
+            return classify(P, s, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) == 1;
+        },
+        [&](uint32_t j) {
+            float4 pj = posm[j];
+            double dx = xi - (double)pj.x, dy = yi - (double)pj.y, dz = zi - (double)pj.z;
+            double s = (dx * dx + dy * dy) + dz * dz;
+            double q = P.h2_64 - s;
+            sum += (double)pj.w * (q * q * q);
+        });
+    if (!act) return;
+    double rho = P.c6_64 * sum;
+    double over = rho - P.rest64;
+    double p = over > 0.0 ? P.stiff64 * over : 0.0;
+    rho_out[i] = (float)rho;
+    p_out[i] = (float)p;
+    velm[i].w = (float)((double)pi.w / rho);             // m_j / rho_j of the force pass
+    mp_out[i] = (float)((double)pi.w * (p / (rho * rho)));  // m_j * p_j / rho_j^2
 }
 
 // ---------------------------------------------------------------------------
@@ -471,8 +591,10 @@ struct ForceAcc {
 __device__ __forceinline__ void pair_force(ForceAcc& A, const DevParams& P, float4 pi, float4 vi,
                                            float Pi, float4 pj, float4 vj, float mpj) {
     float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-    // distinct FP32 points closer than ~1e-19 underflow s; keep r tiny but finite
-    float s = fmaxf(dx * dx + dy * dy + dz * dz, 1.17549435e-38f);
+    // distinct FP32 points closer than ~1e-19 underflow s; keep r tiny but finite.
+    // (not fmaxf: a NaN distance must stay NaN, as in the reference)
+    float s = dx * dx + dy * dy + dz * dz;
+    s = s < 1.17549435e-38f ? 1.17549435e-38f : s;
     float inv_r = __frsqrt_rn(s);
     float r = s * inv_r;
     float hr = P.h - r;
@@ -523,86 +645,21 @@ __global__ __launch_bounds__(256) void k_force(
         ci = cellv[i];
     }
     ForceAcc A = {};
-
-    // Candidate cursor: rows 0..8 of the dense table, then the overflow segment
-    // (row 9). Every lane tests one candidate per round; accepted neighbours go
-    // to the lane's LDS list, and the whole wave drains the lists when any is full.
-    const uint32_t o0 = start[d.ncells], o1 = start[d.ncells + 1];
-    const bool dense_i = ci < d.ncells;
-    CellCoord cc = {};
-    if (act && dense_i) cc = decode(ci, d);
-    long long k0 = 0, k1 = 0, k2 = 0;
-    if (act && !dense_i) {
-        k0 = key1(pi.x, P.inv_cell);
-        k1 = key1(pi.y, P.inv_cell);
-        k2 = key1(pi.z, P.inv_cell);
-    }
-    int row = 0;  // 0..8 dense rows (or 27 single cells for an overflow particle), then overflow
-    uint32_t j = 0, jend = 0;
-    const int nrows = dense_i ? 9 : 27;
-    auto next_run = [&]() {
-        while (j >= jend && row <= nrows) {
-            if (row < nrows) {
-                if (dense_i) {
-                    row_run(row, cc, d, start, j, jend);
-                } else {
-                    int di = row / 9 - 1, dj = (row / 3) % 3 - 1, dk = row % 3 - 1;
-                    unsigned long long x = (unsigned long long)k0 + (unsigned long long)(long long)di -
-                                           (unsigned long long)d.kmin[0];
-                    unsigned long long y = (unsigned long long)k1 + (unsigned long long)(long long)dj -
-                                           (unsigned long long)d.kmin[1];
-                    unsigned long long z = (unsigned long long)k2 + (unsigned long long)(long long)dk -
-                                           (unsigned long long)d.kmin[2];
-                    if (x >= (unsigned long long)d.dims[0] || y >= (unsigned long long)d.dims[1] ||
-                        z >= (unsigned long long)d.dims[2]) {
-                        j = jend = 0;
-                    } else {
-                        uint32_t c = (uint32_t)((x * d.dims[1] + y) * d.dims[2] + z);
-                        j = start[c];
-                        jend = start[c + 1];
-                    }
-                }
-            } else {
-                j = o0;
-                jend = o1;
-            }
-            ++row;
-        }
-    };
-    if (act) next_run();
-    int cnt = 0;
-
-    auto drain = [&]() {
-        for (int k = 0; __ballot(k < cnt) != 0ull; ++k) {
-            if (k < cnt) {
-                uint32_t jj = lst[k][tid];
-                pair_force(A, P, pi, vi, Pi, posm[jj], velm[jj], mp[jj]);
-            }
-        }
-        cnt = 0;
-    };
-
-    while (__ballot(act && j < jend) != 0ull) {
-        if (act && j < jend) {
-            float4 pj = posm[j];
-            bool in_ov = row > nrows;  // the overflow run is last
-            bool ok = (uint32_t)i != j;
-            if (ok && in_ov) ok = keys_adjacent(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, P.inv_cell);
-            if (ok) {
-                float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                float s = dx * dx + dy * dy + dz * dz;
-                int cl = classify(P, s, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z);
-                // forces skip r >= h and r <= 0 only; a NaN distance falls through
-                // (world.rs:297-299). r > 0 exactly when the points differ.
-                ok = (cl == 1 && (dx != 0.f || dy != 0.f || dz != 0.f)) || cl == 2;
-            }
-            if (ok) lst[cnt++][tid] = j;
-            ++j;
-            if (j >= jend) next_run();
-        }
-        if (__ballot(cnt == L) != 0ull) drain();
-    }
-    drain();
+    Cursor cur;
+    cur.init(act, ci, pi.x, pi.y, pi.z, d, P.inv_cell, start);
+    gather<L>(
+        act, cur, d, start, posm, lst, tid,
+        [&](uint32_t j, float4 pj, bool in_ov) {
+            if ((uint32_t)i == j) return false;
+            if (in_ov && !keys_adjacent(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, P.inv_cell)) return false;
+            float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            float s = dx * dx + dy * dy + dz * dz;
+            int cl = classify(P, s, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z);
+            // forces skip r >= h and r <= 0 only; a NaN distance falls through
+            // (world.rs:297-299). r > 0 exactly when the points differ.
+            return (cl == 1 && (dx != 0.f || dy != 0.f || dz != 0.f)) || cl == 2;
+        },
+        [&](uint32_t j) { pair_force(A, P, pi, vi, Pi, posm[j], velm[j], mp[j]); });
     if (!act) return;
 
     // power-law effective viscosity (world.rs:319-327)
@@ -660,10 +717,8 @@ __global__ __launch_bounds__(256) void k_force(
     uint32_t id = idv[i];
     uint32_t bp = finite3(x[0], x[1], x[2]) ? 0xffffffffu : id;
     uint32_t bv = finite3(v[0], v[1], v[2]) ? 0xffffffffu : id;
-    if (__ballot(bp != 0xffffffffu || bv != 0xffffffffu) != 0ull) {
-        if (bp != 0xffffffffu) atomicMin(&dom->bad_pos, bp);
-        if (bv != 0xffffffffu) atomicMin(&dom->bad_vel, bv);
-    }
+    if (bp != 0xffffffffu) atomicMin(&dom->bad_pos, bp);
+    if (bv != 0xffffffffu) atomicMin(&dom->bad_vel, bv);
 }
 
 // ---------------------------------------------------------------------------
@@ -788,8 +843,8 @@ void launch_sort(hipStream_t st, int n, const SortBuffers& b, const Domain* d, d
 void launch_density(hipStream_t st, int n, const float4* posm, const uint32_t* cellv,
                     const uint32_t* start, const Domain* d, const DevParams& P, float* rho,
                     float* prs, float4* velm, float* mp) {
-    hipLaunchKernelGGL(k_density, dim3(nblk(n, 256)), dim3(256), 0, st, n, posm, cellv, start, d, P,
-                       rho, prs, velm, mp);
+    hipLaunchKernelGGL(k_density<kDensityList>, dim3(nblk(n, 256)), dim3(256), 0, st, n, posm, cellv,
+                       start, d, P, rho, prs, velm, mp);
 }
 
 void launch_force(hipStream_t st, int n, float dt, const ForceArgs& a, Domain* d,

@@ -27,6 +27,8 @@ struct DevParams {
     double h64;          // SphParams::h
     double inv_cell;     // 1.0 / h in f64 (grid.rs:27: T::one() / self.cell)
     double s_thresh;     // smallest f64 s with sqrt(s) >= h: r < h  <=>  s < s_thresh
+    double h2_64, c6_64;  // f64 h*h and poly6 coefficient (density pass)
+    double rest64, stiff64;
     float s_lo, s_hi;    // FP32 filter: s32 < s_lo => inside; s32 > s_hi => outside
     float h, h2;         // FP32 support radius and its square
     float c6, cs, cv;    // poly6, spiky-gradient, viscosity-laplacian coefficients

@@ -199,14 +199,48 @@ def test_watchdog_catches_injected_nan():
     assert g.step_checked(0.005) == 2
 
 
-def test_watchdog_reports_velocity_blowup():
-    # a huge dt drives the state non-finite; step_checked must stop with rc 2
-    gp = ffi.PhySphParams.defaults()
-    gp.stiffness = 1e30
+def _sph_world_with(gp, n):
+    # crates/phy-demo/tests/stability.rs:22-37: spacing from n in a 10 m box
+    half = 5.0
+    spacing = (8.0 * half ** 3 / n) ** (1.0 / 3.0)
+    gp.h = spacing * 2.0
     g = ffi.FluidWorld(gp)
-    g.fill_box((-1, -1, -1), (1, 1, 1), 0.1, 0.05)
-    rcs = [g.step_checked(0.5) for _ in range(10)]
-    assert 2 in rcs
+    g.fill_box((-half, 1.0, -half), (half, 1.0 + 2 * half, half), spacing, 0.1)
+    return g
+
+
+def test_extreme_stiffness_detected_or_bounded():       # stability.rs:140-173
+    gp = ffi.PhySphParams.defaults()
+    gp.stiffness = 1.0e6
+    g = _sph_world_with(gp, 400)
+    saw_err, e_max = False, 0.0
+    for _ in range(50):
+        if g.step_checked(0.01) != 0:
+            saw_err = True
+            break
+        e = g.diagnostics()[4]
+        if np.isfinite(e):
+            e_max = max(e_max, e)
+    assert saw_err or np.isfinite(e_max)
+
+
+def test_large_dt_does_not_crash():                      # stability.rs:196-220
+    g = _sph_world_with(ffi.PhySphParams.defaults(), 400)
+    for _ in range(50):
+        assert g.step_checked(0.5) in (0, 2)
+
+
+def test_zero_mass_fires_the_watchdog_like_the_oracle():
+    # mass 0: rho = 0, then m_j / rho_j = 0/0 in the force sums -> NaN velocity
+    pos = _block(6, 2, jit=0.1)
+    g, o = P.make_pair(pos, mass=0.0)
+    rc = g.step_checked(P.DT32)
+    o.step(P.DT32)
+    # x' = x + v' dt turns NaN too, and validate checks pos first: NonFinite{pos}
+    assert o.validate() == 1
+    assert rc == 2
+    for f in ("pos", "vel"):
+        assert np.array_equal(np.isnan(g.get(f)), np.isnan(o.get(f))), f
 
 
 # ---- FP64 diagnostics -------------------------------------------------------------
