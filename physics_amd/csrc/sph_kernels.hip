@@ -117,6 +117,7 @@ __global__ void k_domain_reset(Domain* d) {
 
 __global__ __launch_bounds__(256) void k_aabb(int n, const float4* __restrict__ posm, double inv,
                                               Domain* d) {
+    __shared__ long long sh[6][4];
     long long mn[3] = {0x7fffffffffffffffll, 0x7fffffffffffffffll, 0x7fffffffffffffffll};
     long long mx[3] = {(long long)0x8000000000000000ull, (long long)0x8000000000000000ull,
                        (long long)0x8000000000000000ull};
@@ -133,11 +134,22 @@ __global__ __launch_bounds__(256) void k_aabb(int n, const float4* __restrict__ 
         mn[a] = wave_min_ll(mn[a]);
         mx[a] = wave_max_ll(mx[a]);
     }
-    if ((threadIdx.x & 63) == 0) {
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
         for (int a = 0; a < 3; ++a) {
-            atomicMin(&d->amin[a], mn[a]);
-            atomicMax(&d->amax[a], mx[a]);
+            sh[a][w] = mn[a];
+            sh[3 + a][w] = mx[a];
         }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        const int a = threadIdx.x;
+        long long lo = sh[a][0], hi = sh[3 + a][0];
+        for (int k = 1; k < 4; ++k) {
+            lo = sh[a][k] < lo ? sh[a][k] : lo;
+            hi = sh[3 + a][k] > hi ? sh[3 + a][k] : hi;
+        }
+        atomicMin(&d->amin[a], lo);
+        atomicMax(&d->amax[a], hi);
     }
 }
 
@@ -200,16 +212,36 @@ __device__ __forceinline__ uint32_t cell_of(float x, float y, float z, double in
     return (uint32_t)((k0 * d.dims[1] + k1) * d.dims[2] + k2);
 }
 
+// Particles arrive in the previous step's cell order, so equal cells form runs
+// of lanes: one atomic per run (wave-aggregated), each lane's arrival rank is
+// the run's base plus its position in the run. The arrival order is not
+// deterministic; k_reorder replaces it by the original-index order.
 __global__ __launch_bounds__(256) void k_bin(int n, const float4* __restrict__ posm, double inv,
                                              const Domain* __restrict__ dom, uint32_t* cell,
                                              uint32_t* rank, uint32_t* counts) {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    Domain d = *dom;
-    float4 p = posm[s];
-    uint32_t c = cell_of(p.x, p.y, p.z, inv, d);
-    cell[s] = c;
-    rank[s] = atomicAdd(&counts[c], 1u);
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const bool valid = s < n;
+    const Domain d = *dom;
+    uint32_t c = 0xffffffffu;
+    if (valid) {
+        float4 p = posm[s];
+        c = cell_of(p.x, p.y, p.z, inv, d);
+    }
+    const uint32_t prev = __shfl_up(c, 1, 64);
+    const bool head = lane == 0 || prev != c;
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long le = heads & (~0ull >> (63 - lane));     // heads at lanes <= lane
+    const int first = 63 - __clzll((long long)le);                     // this run's first lane
+    const unsigned long long above = heads & ~(~0ull >> (63 - first)); // heads after it
+    const int next = above ? __ffsll((long long)above) - 1 : 64;
+    uint32_t base = 0;
+    if (head && valid) base = atomicAdd(&counts[c], (uint32_t)(next - first));
+    base = __shfl(base, first, 64);
+    if (valid) {
+        cell[s] = c;
+        rank[s] = base + (uint32_t)(lane - first);
+    }
 }
 
 // Three-phase exclusive scan over counts[0 .. ncells] (ncells + 1 entries, the
@@ -427,14 +459,20 @@ __device__ __forceinline__ void for_each_candidate(uint32_t ci, float xi, float 
 }
 
 // ---------------------------------------------------------------------------
-// per-lane neighbour compaction
+// neighbour compaction
 //
-// Every lane walks its own candidate sequence (rows 0..8 of the dense table, or
-// the 27 single cells of an out-of-domain particle, then the overflow segment)
-// one candidate per round; accepted neighbours are appended to the lane's LDS
-// list, and the whole wave drains the lists as soon as one is full. The heavy
-// per-neighbour work then runs on dense lists instead of on every candidate
-// under divergence (about 15 % of the 27-cell candidates are neighbours).
+// Candidates are tested with the cheap FP32 filter (exact f64 fallback in the
+// band) and accepted ones are appended to a per-lane list in LDS; the wave
+// drains the lists whenever one is full, so the heavy per-neighbour work runs
+// on dense lists instead of on every candidate under divergence (about 15 % of
+// the 27-cell candidates are neighbours at rest, 35 % in a collapsed sheet).
+//
+// Two scans produce the same lists in the same order:
+//  * uniform: every active lane of the wave sits in the same dense cell (the
+//    common case once the column has collapsed, ~400 particles per cell). The
+//    candidate sequence is then wave-uniform: loop control runs on the scalar
+//    unit and candidate loads are uniform.
+//  * per-lane: each lane walks its own rows with a cursor, one candidate per round.
 
 struct Cursor {
     int row, nrows;
@@ -498,27 +536,79 @@ struct Cursor {
     }
 };
 
-template <int L, class Accept, class Proc>
-__device__ __forceinline__ void gather(bool act, Cursor& cur, const Domain& d,
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// test(j, pj) -> bool (FP32 filter, exact f64 band); load(j) -> T;
+// proc(T, valid) accumulates (an invalid slot must contribute exactly nothing);
+// flush() after every drain. self_j is a valid index used to pad drain slots.
+// Overflow-segment candidates additionally need reference key adjacency.
+template <int L, class Test, class Load, class Proc, class Flush>
+__device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi, float zi,
+                                       uint32_t self_j, const Domain& d, double inv,
                                        const uint32_t* __restrict__ start,
                                        const float4* __restrict__ posm, uint32_t (*lst)[256],
-                                       int tid, Accept&& accept, Proc&& proc) {
+                                       int tid, Test&& test, Load&& load, Proc&& proc,
+                                       Flush&& flush) {
     int cnt = 0;
     auto drain = [&]() {
-        for (int k = 0; __ballot(k < cnt) != 0ull; ++k) {
-            if (k < cnt) proc(lst[k][tid]);
+        for (int k = 0; __ballot(k < cnt) != 0ull; k += 2) {
+            const bool v0 = k < cnt, v1 = k + 1 < cnt;
+            const uint32_t j0 = v0 ? lst[k][tid] : self_j;
+            const uint32_t j1 = v1 ? lst[k + 1][tid] : self_j;
+            auto t0 = load(j0);
+            auto t1 = load(j1);
+            proc(t0, v0);
+            proc(t1, v1);
         }
         cnt = 0;
+        flush();
     };
-    while (__ballot(act && cur.j < cur.jend) != 0ull) {
-        if (act && cur.j < cur.jend) {
-            uint32_t j = cur.j;
-            float4 pj = posm[j];
-            if (accept(j, pj, cur.in_overflow())) lst[cnt++][tid] = j;
-            ++cur.j;
-            if (cur.j >= cur.jend) cur.next_run(d, start);
+    auto ov_ok = [&](const float4& pj) {
+        return keys_adjacent(xi, yi, zi, pj.x, pj.y, pj.z, inv);
+    };
+    const uint32_t ci0 = uni(ci);
+    const bool uniform = ci0 < d.ncells && __ballot(act && ci != ci0) == 0ull;
+    if (uniform) {
+        const CellCoord cc = decode(ci0, d);
+        for (int row = 0; row < 9; ++row) {
+            uint32_t a, b;
+            row_run(row, cc, d, start, a, b);
+            a = uni(a);
+            b = uni(b);
+            for (uint32_t j = a; j < b; j += 4) {
+                const uint32_t last = b - 1;
+                const uint32_t j1 = min(j + 1, last), j2 = min(j + 2, last), j3 = min(j + 3, last);
+                const float4 p0 = posm[j], p1 = posm[j1], p2 = posm[j2], p3 = posm[j3];
+                const bool t0 = act && test(j, p0);
+                const bool t1 = act && j + 1 <= last && test(j1, p1);
+                const bool t2 = act && j + 2 <= last && test(j2, p2);
+                const bool t3 = act && j + 3 <= last && test(j3, p3);
+                if (t0) lst[cnt++][tid] = j;
+                if (t1) lst[cnt++][tid] = j1;
+                if (t2) lst[cnt++][tid] = j2;
+                if (t3) lst[cnt++][tid] = j3;
+                if (__ballot(cnt > L - 4) != 0ull) drain();
+            }
         }
-        if (__ballot(cnt == L) != 0ull) drain();
+        const uint32_t o0 = uni(start[d.ncells]), o1 = uni(start[d.ncells + 1]);
+        for (uint32_t j = o0; j < o1; ++j) {
+            const float4 pj = posm[j];
+            if (act && ov_ok(pj) && test(j, pj)) lst[cnt++][tid] = j;
+            if (__ballot(cnt == L) != 0ull) drain();
+        }
+    } else {
+        Cursor cur;
+        cur.init(act, ci, xi, yi, zi, d, inv, start);
+        while (__ballot(act && cur.j < cur.jend) != 0ull) {
+            if (act && cur.j < cur.jend) {
+                const uint32_t j = cur.j;
+                const float4 pj = posm[j];
+                if ((!cur.in_overflow() || ov_ok(pj)) && test(j, pj)) lst[cnt++][tid] = j;
+                ++cur.j;
+                if (cur.j >= cur.jend) cur.next_run(d, start);
+            }
+            if (__ballot(cnt == L) != 0ull) drain();
+        }
     }
     drain();
 }
@@ -550,24 +640,28 @@ __global__ __launch_bounds__(256) void k_density(int n, const float4* __restrict
         ci = cellv[i];
     }
     const double xi = pi.x, yi = pi.y, zi = pi.z;
-    Cursor cur;
-    cur.init(act, ci, pi.x, pi.y, pi.z, d, P.inv_cell, start);
     double sum = 0.0;
     gather<L>(
-        act, cur, d, start, posm, lst, tid,
-        [&](uint32_t j, float4 pj, bool in_ov) {
-            if (in_ov && !keys_adjacent(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, P.inv_cell)) return false;
+        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, start, posm, lst, tid,
+        [&](uint32_t j, const float4& pj) {
             float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
             float s = dx * dx + dy * dy + dz * dz;
-            return classify(P, s, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) == 1;
+            bool in = s < P.s_lo;
+            if (!in && !(s > P.s_hi)) {  // f64 band or NaN: decide exactly (rare)
+                double e = exact_s64(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z);
+                in = e < P.s_thresh;
+            }
+            return in;
         },
-        [&](uint32_t j) {
-            float4 pj = posm[j];
+        [&](uint32_t j) { return posm[j]; },
+        [&](const float4& pj, bool valid) {
             double dx = xi - (double)pj.x, dy = yi - (double)pj.y, dz = zi - (double)pj.z;
             double s = (dx * dx + dy * dy) + dz * dz;
             double q = P.h2_64 - s;
-            sum += (double)pj.w * (q * q * q);
-        });
+            double t = (double)pj.w * (q * q * q);
+            sum += valid ? t : 0.0;
+        },
+        [&]() {});
     if (!act) return;
     double rho = P.c6_64 * sum;
     double over = rho - P.rest64;
@@ -580,48 +674,75 @@ __global__ __launch_bounds__(256) void k_density(int n, const float4* __restrict
 
 // ---------------------------------------------------------------------------
 // forces + integrate + bounds + watchdog (world.rs:252-397, subsystem.rs:74-90)
+//
+// Compensated summation over blocks: the contributions of one drained batch
+// (<= 2 L neighbours of this lane) are summed with plain FMAs, then each batch
+// partial is added to the running total with Kahan's algorithm.
 
+template <bool SHEAR>
 struct ForceAcc {
-    float fp[3], cfp[3];   // pressure  (x m_i * cs)
-    float fv[3], cfv[3];   // viscosity (x m_i * cv)
-    float xs[3], cxs[3];   // XSPH      (x c6)
-    float sh, csh;         // shear-rate proxy
+    float b[10];            // batch partials: pressure xyz, viscosity xyz, XSPH xyz, shear
+    float s[10], c[10];     // compensated totals
+    __device__ __forceinline__ void zero() {
+        for (int k = 0; k < 10; ++k) b[k] = s[k] = c[k] = 0.f;
+    }
+    __device__ __forceinline__ void merge() {
+        for (int k = 0; k < (SHEAR ? 10 : 9); ++k) {
+            kahan_add(s[k], c[k], b[k]);
+            b[k] = 0.f;
+        }
+    }
 };
 
-__device__ __forceinline__ void pair_force(ForceAcc& A, const DevParams& P, float4 pi, float4 vi,
-                                           float Pi, float4 pj, float4 vj, float mpj) {
+struct NbrData {
+    float4 p, v;
+    float mp;
+};
+
+template <bool SHEAR>
+__device__ __forceinline__ void pair_force(ForceAcc<SHEAR>& A, const DevParams& P, float4 pi,
+                                           float4 vi, float Pi, const NbrData& nb, bool valid) {
+    const float4 pj = nb.p, vj = nb.v;
     float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
     // distinct FP32 points closer than ~1e-19 underflow s; keep r tiny but finite.
     // (not fmaxf: a NaN distance must stay NaN, as in the reference)
     float s = dx * dx + dy * dy + dz * dz;
     s = s < 1.17549435e-38f ? 1.17549435e-38f : s;
-    float inv_r = __frsqrt_rn(s);
+    float inv_r = __builtin_amdgcn_rsqf(s);
     float r = s * inv_r;
     float hr = P.h - r;
-    // pressure: dir * (m_i m_j (P_i + P_j) * cs (h - r)^2), dir = d / r
-    float a = (pj.w * Pi + mpj) * (hr * hr) * inv_r;
-    kahan_add(A.fp[0], A.cfp[0], a * dx);
-    kahan_add(A.fp[1], A.cfp[1], a * dy);
-    kahan_add(A.fp[2], A.cfp[2], a * dz);
-    // viscosity: (v_j - v_i) * (m_i m_j / rho_j * cv (h - r))
     float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
     float mr = vj.w;  // m_j / rho_j
-    float b = mr * hr;
-    kahan_add(A.fv[0], A.cfv[0], b * dvx);
-    kahan_add(A.fv[1], A.cfv[1], b * dvy);
-    kahan_add(A.fv[2], A.cfv[2], b * dvz);
-    // XSPH: (v_j - v_i) * (m_j / rho_j * poly6(r))
     float q = P.h2 - s;
+    // pressure: dir * (m_i m_j (P_i + P_j) * cs (h - r)^2), dir = d / r
+    float a = fmaf(pj.w, Pi, nb.mp) * (hr * hr) * inv_r;
+    // viscosity: (v_j - v_i) * (m_i m_j / rho_j * cv (h - r))
+    float bb = mr * hr;
+    // XSPH: (v_j - v_i) * (m_j / rho_j * poly6(r))
     float w = mr * (q * q * q);
-    kahan_add(A.xs[0], A.cxs[0], w * dvx);
-    kahan_add(A.xs[1], A.cxs[1], w * dvy);
-    kahan_add(A.xs[2], A.cxs[2], w * dvz);
-    // shear-rate proxy: |v_j - v_i| / (r + 1e-4) * (m_j / rho_j)
-    float dv = sqrtf(dvx * dvx + dvy * dvy + dvz * dvz);
-    kahan_add(A.sh, A.csh, dv / (r + P.r_eps) * mr);
+    // an invalid (padding) slot holds the particle itself: finite, zero weight
+    a = valid ? a : 0.f;
+    bb = valid ? bb : 0.f;
+    w = valid ? w : 0.f;
+    A.b[0] = fmaf(a, dx, A.b[0]);
+    A.b[1] = fmaf(a, dy, A.b[1]);
+    A.b[2] = fmaf(a, dz, A.b[2]);
+    A.b[3] = fmaf(bb, dvx, A.b[3]);
+    A.b[4] = fmaf(bb, dvy, A.b[4]);
+    A.b[5] = fmaf(bb, dvz, A.b[5]);
+    A.b[6] = fmaf(w, dvx, A.b[6]);
+    A.b[7] = fmaf(w, dvy, A.b[7]);
+    A.b[8] = fmaf(w, dvz, A.b[8]);
+    if (SHEAR) {
+        // shear-rate proxy: |v_j - v_i| / (r + 1e-4) * (m_j / rho_j); only a
+        // non-Newtonian material (visc_n != 1) reads it
+        float dv = sqrtf(dvx * dvx + dvy * dvy + dvz * dvz);
+        float t = dv * __builtin_amdgcn_rcpf(r + P.r_eps) * mr;
+        A.b[9] += valid ? t : 0.f;
+    }
 }
 
-template <int L>
+template <int L, bool SHEAR>
 __global__ __launch_bounds__(256) void k_force(
     int n, float dt, const float4* __restrict__ posm, const float4* __restrict__ velm,
     const float* __restrict__ rho, const float* __restrict__ prs, const float* __restrict__ mp,
@@ -644,22 +765,28 @@ __global__ __launch_bounds__(256) void k_force(
         Pi = prs[i] / (rho_i * rho_i);
         ci = cellv[i];
     }
-    ForceAcc A = {};
-    Cursor cur;
-    cur.init(act, ci, pi.x, pi.y, pi.z, d, P.inv_cell, start);
+    ForceAcc<SHEAR> A;
+    A.zero();
     gather<L>(
-        act, cur, d, start, posm, lst, tid,
-        [&](uint32_t j, float4 pj, bool in_ov) {
-            if ((uint32_t)i == j) return false;
-            if (in_ov && !keys_adjacent(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z, P.inv_cell)) return false;
+        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, start, posm, lst, tid,
+        [&](uint32_t j, const float4& pj) {
             float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
             float s = dx * dx + dy * dy + dz * dz;
-            int cl = classify(P, s, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z);
-            // forces skip r >= h and r <= 0 only; a NaN distance falls through
-            // (world.rs:297-299). r > 0 exactly when the points differ.
-            return (cl == 1 && (dx != 0.f || dy != 0.f || dz != 0.f)) || cl == 2;
+            // s > 0 rejects the particle itself and coincident points (r <= 0)
+            bool ok = s < P.s_lo && s > 0.f;
+            bool rare = (!(s < P.s_lo) && !(s > P.s_hi)) || (s == 0.f && j != (uint32_t)i);
+            if (rare) {
+                // f64 band, NaN distance (not skipped by the reference,
+                // world.rs:297-299) or an underflowed distinct pair
+                double e = exact_s64(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z);
+                bool nz = dx != 0.f || dy != 0.f || dz != 0.f;
+                ok = j != (uint32_t)i && ((e < P.s_thresh && nz) || e != e);
+            }
+            return ok;
         },
-        [&](uint32_t j) { pair_force(A, P, pi, vi, Pi, posm[j], velm[j], mp[j]); });
+        [&](uint32_t j) { return NbrData{posm[j], velm[j], mp[j]}; },
+        [&](const NbrData& nb, bool valid) { pair_force<SHEAR>(A, P, pi, vi, Pi, nb, valid); },
+        [&]() { A.merge(); });
     if (!act) return;
 
     // power-law effective viscosity (world.rs:319-327)
@@ -670,18 +797,19 @@ __global__ __launch_bounds__(256) void k_force(
     }
     float kc = P.visc_k[m];
     float mu;
-    if (P.newtonian) {
+    if (!SHEAR) {
         mu = kc;  // powf(x, 0) == 1 for every x
     } else {
-        float sreg = A.sh > P.shear_min ? A.sh : P.shear_min;
+        float sh = A.s[9];
+        float sreg = sh > P.shear_min ? sh : P.shear_min;
         mu = kc * powf(sreg, P.visc_n[m] - 1.f);
     }
     float acc[3] = {0.f, 0.f, 0.f};
     if (rho_i > 0.f) {
         float mcs = pi.w * P.cs, mcv = pi.w * P.cv * mu;
-        for (int a = 0; a < 3; ++a) acc[a] = (mcs * A.fp[a] + mcv * A.fv[a]) / rho_i;
+        for (int a = 0; a < 3; ++a) acc[a] = (mcs * A.s[a] + mcv * A.s[3 + a]) / rho_i;
     }
-    float xs[3] = {P.c6 * A.xs[0], P.c6 * A.xs[1], P.c6 * A.xs[2]};
+    float xs[3] = {P.c6 * A.s[6], P.c6 * A.s[7], P.c6 * A.s[8]};
 
     // integrate (world.rs:352-363) and enforce_bounds (366-397)
     float bacc[3] = {0.f, 0.f, 0.f};
@@ -819,7 +947,7 @@ void launch_domain(hipStream_t st, int n, const float4* posm, double inv, Domain
     hipLaunchKernelGGL(k_domain_reset, dim3(1), dim3(1), 0, st, d);
     if (n > 0) {
         int nb = nblk(n, 256);
-        if (nb > 2048) nb = 2048;
+        if (nb > 512) nb = 512;
         hipLaunchKernelGGL(k_aabb, dim3(nb), dim3(256), 0, st, n, posm, inv, d);
     }
     hipLaunchKernelGGL(k_domain_finalize, dim3(1), dim3(1), 0, st, d, cap, bk[0], bk[1], bk[2],
@@ -849,9 +977,14 @@ void launch_density(hipStream_t st, int n, const float4* posm, const uint32_t* c
 
 void launch_force(hipStream_t st, int n, float dt, const ForceArgs& a, Domain* d,
                   const DevParams& P) {
-    hipLaunchKernelGGL(k_force<kForceList>, dim3(nblk(n, 256)), dim3(256), 0, st, n, dt, a.posm,
-                       a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body, d, P,
-                       a.posm_out, a.vel_out, a.acc_out, a.mu_out, a.xsph_out);
+    if (P.newtonian)
+        hipLaunchKernelGGL((k_force<kForceList, false>), dim3(nblk(n, 256)), dim3(256), 0, st, n, dt,
+                           a.posm, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body, d,
+                           P, a.posm_out, a.vel_out, a.acc_out, a.mu_out, a.xsph_out);
+    else
+        hipLaunchKernelGGL((k_force<kForceList, true>), dim3(nblk(n, 256)), dim3(256), 0, st, n, dt,
+                           a.posm, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body, d,
+                           P, a.posm_out, a.vel_out, a.acc_out, a.mu_out, a.xsph_out);
 }
 
 void launch_diag(hipStream_t st, int n, const float4* posm, const float4* vel, double* partial,

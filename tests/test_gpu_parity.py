@@ -261,14 +261,15 @@ def test_fp64_diagnostics_match_oracle():
 def test_full_size_column_one_step_parity_and_invariants():
     sc = scenes.dam_break((100, 100, 100), seed=1)
     g, o = P.make_pair(sc.pos, mass=sc.mass, bounds=(sc.bounds_min, sc.bounds_max))
+    # neighbour sets on the identical input state
+    gc, gs, gx = g.neighbor_stats()
+    oc, os_, ox = o.neighbor_stats()
+    assert np.array_equal(gc, oc) and np.array_equal(gs, os_) and np.array_equal(gx, ox)
     x0 = P.step_both(g, o)
     err = P.compare_step(g, o, x0)
     _print("1M one step", err)
     for k, v in err.items():
         assert v <= P.TOL_STEP, (k, v)
-    gc, gs, gx = g.neighbor_stats()
-    oc, os_, ox = o.neighbor_stats()
-    assert np.array_equal(gc, oc) and np.array_equal(gs, os_) and np.array_equal(gx, ox)
     m0 = g.diagnostics()[0]
     g.step(P.DT32, 30)
     x = g.get("pos")
