@@ -206,6 +206,26 @@ int32_t phy_fluid_neighbor_stats(PhyWorldHandle *w, uint32_t *counts, uint64_t *
 /* n steps of dt in one call (same result as n phy_world_step calls). */
 int32_t phy_world_step_n(PhyWorldHandle *w, double dt, uint32_t n);
 
+/* Reverse-mode gradients of a rollout (adjoint of FluidWorld::step).
+ * phy_fluid_tape_begin(w, K) marks the current state as the rollout start;
+ * every following phy_world_step* is recorded and the state is checkpointed
+ * every K steps (memory: one 40 B/particle checkpoint per K steps plus K step
+ * slots of ~120 B/particle). phy_fluid_tape_backward takes the cotangents of
+ * the final positions and velocities (3 per particle) and of the density of
+ * the last step (1 per particle), in index order, any of them NULL for zero,
+ * and writes the cotangents of the initial positions / velocities (3 per
+ * particle, may be NULL) and dparams[0] = dL/d stiffness, dparams[1] =
+ * dL/d visc_k[0] (FP64 device reductions). Segments are recomputed from their
+ * checkpoints; the world state is unchanged. One Newtonian material only
+ * (visc_n = [1]); no body_acc. Returns 0 ok, -1 error. Replacing the particles
+ * while recording invalidates the tape. */
+int32_t phy_fluid_tape_begin(PhyWorldHandle *w, uint32_t checkpoint_every);
+uint32_t phy_fluid_tape_steps(PhyWorldHandle *w);
+int32_t phy_fluid_tape_backward(PhyWorldHandle *w, const double *xbar_final, const double *vbar_final,
+                                const double *rhobar_last, double *xbar0, double *vbar0,
+                                double *dparams);
+int32_t phy_fluid_tape_end(PhyWorldHandle *w);
+
 /* Steps are asynchronous on the world's HIP stream; block until the GPU has
  * finished every step issued so far. 0 ok, -1 error. */
 int32_t phy_fluid_sync(PhyWorldHandle *w);

@@ -134,6 +134,7 @@ void DeviceFluid::set_params(const HostParams& hp) {
                   "memset counts");
     }
     dens_valid_ = aux_valid_ = false;
+    tape_invalidate();
 }
 
 void DeviceFluid::sync() { hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize"); }
@@ -229,6 +230,7 @@ void DeviceFluid::upload(size_t n, const float* pos3, const float* vel3, const f
     }
     sync();  // host vectors go out of scope
     dens_valid_ = aux_valid_ = false;
+    tape_invalidate();
 }
 
 void DeviceFluid::set_keep_aux(bool on) {
@@ -241,9 +243,29 @@ void DeviceFluid::set_keep_aux(bool on) {
     if (!on) aux_valid_ = false;
 }
 
-void DeviceFluid::run_domain_and_sort() {
+DeviceFluid::StepBufs DeviceFluid::own_bufs() {
+    StepBufs s;
+    s.posm_s = posm_s_.get();
+    s.velm_s = velm_s_.get();
+    s.body_s = body_s_.get();
+    s.id_s = id_s_.get();
+    s.mat_s = mat_s_.get();
+    s.cell_s = cell_s_.get();
+    s.rho = rho_.get();
+    s.prs = prs_.get();
+    s.mp = mp_.get();
+    s.start = start_.get();
+    s.dom = dom_.get();
+    s.acc = keep_aux_ ? acc_.get() : nullptr;
+    s.mu = keep_aux_ ? mu_.get() : nullptr;
+    s.xsph = keep_aux_ ? xsph_.get() : nullptr;
+    s.fvisc = nullptr;
+    return s;
+}
+
+void DeviceFluid::run_domain_and_sort(const StepBufs& s) {
     const int n = (int)n_;
-    launch_domain(st_, n, posm_.get(), P_.inv_cell, dom_.get(), cell_cap_, bkey_);
+    launch_domain(st_, n, posm_.get(), P_.inv_cell, s.dom, cell_cap_, bkey_);
     SortBuffers b;
     b.posm_in = posm_.get();
     b.vel_in = vel_.get();
@@ -254,17 +276,19 @@ void DeviceFluid::run_domain_and_sort() {
     b.rank = rank_.get();
     b.counts = counts_.get();
     b.partial = partial_.get();
-    b.start = start_.get();
+    b.start = s.start;
     b.tmp = tmp_.get();
     b.idtmp = idtmp_.get();
-    b.posm_out = posm_s_.get();
-    b.velm_out = velm_s_.get();
-    b.id_out = id_s_.get();
-    b.mat_out = mat_s_.get();
-    b.body_out = body_s_.get();
-    b.cell_out = cell_s_.get();
-    launch_sort(st_, n, b, dom_.get(), P_.inv_cell);
+    b.posm_out = s.posm_s;
+    b.velm_out = s.velm_s;
+    b.id_out = s.id_s;
+    b.mat_out = s.mat_s;
+    b.body_out = s.body_s;
+    b.cell_out = s.cell_s;
+    launch_sort(st_, n, b, s.dom, P_.inv_cell);
 }
+
+void DeviceFluid::run_domain_and_sort() { run_domain_and_sort(own_bufs()); }
 
 hipEvent_t DeviceFluid::ev() {
     hipEvent_t e;
@@ -302,40 +326,215 @@ uint32_t DeviceFluid::profile_end(double ms4[4]) {
     return steps;
 }
 
-void DeviceFluid::step(float dt) {
-    if (n_ == 0) return;  // world.rs:184-186
+// One step into the given buffer set; the new state lands in posm_ / vel_ in
+// this step's cell order.
+void DeviceFluid::step_core(float dt, const StepBufs& s) {
     const int n = (int)n_;
     if (prof_) ev();
-    run_domain_and_sort();
+    run_domain_and_sort(s);
     if (prof_) ev();
-    launch_density(st_, n, posm_s_.get(), cell_s_.get(), start_.get(), dom_.get(), P_, rho_.get(),
-                   prs_.get(), velm_s_.get(), mp_.get());
+    launch_density(st_, n, s.posm_s, s.cell_s, s.start, s.dom, P_, s.rho, s.prs, s.velm_s, s.mp);
     ForceArgs a;
-    a.posm = posm_s_.get();
-    a.velm = velm_s_.get();
-    a.rho = rho_.get();
-    a.prs = prs_.get();
-    a.mp = mp_.get();
-    a.cell = cell_s_.get();
-    a.start = start_.get();
-    a.id = id_s_.get();
-    a.mat = P_.n_mat > 1 ? mat_s_.get() : nullptr;
-    a.body = has_body_ ? body_s_.get() : nullptr;
+    a.posm = s.posm_s;
+    a.velm = s.velm_s;
+    a.rho = s.rho;
+    a.prs = s.prs;
+    a.mp = s.mp;
+    a.cell = s.cell_s;
+    a.start = s.start;
+    a.id = s.id_s;
+    a.mat = P_.n_mat > 1 ? s.mat_s : nullptr;
+    a.body = has_body_ ? s.body_s : nullptr;
     a.posm_out = posm_.get();
     a.vel_out = vel_.get();
-    a.acc_out = keep_aux_ ? acc_.get() : nullptr;
-    a.mu_out = keep_aux_ ? mu_.get() : nullptr;
-    a.xsph_out = keep_aux_ ? xsph_.get() : nullptr;
+    a.acc_out = s.acc;
+    a.mu_out = s.mu;
+    a.xsph_out = s.xsph;
+    a.fvisc_out = s.fvisc;
     if (prof_) ev();
-    launch_force(st_, n, dt, a, dom_.get(), P_);
+    launch_force(st_, n, dt, a, s.dom, P_);
     if (prof_) ev();
     hip_check(hipGetLastError(), "step launch");
+    has_body_ = false;  // integrate zeroes body_acc (world.rs:361)
+}
+
+void DeviceFluid::step(float dt) {
+    if (n_ == 0) return;  // world.rs:184-186
+    step_core(dt, own_bufs());
     // the new state is in cell order: ids and materials follow the sort
     id_.swap(id_s_);
     mat_.swap(mat_s_);
-    has_body_ = false;  // integrate zeroes body_acc (world.rs:361)
     dens_valid_ = true;
     aux_valid_ = keep_aux_;
+    if (taping_) tape_record(dt);
+}
+
+// ---------------------------------------------------------------------------
+// adjoint tape
+
+void DeviceFluid::save_state(Snapshot& c) {
+    const size_t n = n_ ? n_ : 1;
+    c.posm.ensure(n);
+    c.vel.ensure(n);
+    c.id.ensure(n);
+    c.mat.ensure(n);
+    if (!n_) return;
+    hip_check(hipMemcpyAsync(c.posm.get(), posm_.get(), n_ * sizeof(float4), hipMemcpyDeviceToDevice, st_), "ckpt");
+    hip_check(hipMemcpyAsync(c.vel.get(), vel_.get(), n_ * sizeof(float4), hipMemcpyDeviceToDevice, st_), "ckpt");
+    hip_check(hipMemcpyAsync(c.id.get(), id_.get(), n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, st_), "ckpt");
+    hip_check(hipMemcpyAsync(c.mat.get(), mat_.get(), n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, st_), "ckpt");
+}
+
+void DeviceFluid::load_state(const Snapshot& c) {
+    if (!n_) return;
+    hip_check(hipMemcpyAsync(posm_.get(), c.posm.get(), n_ * sizeof(float4), hipMemcpyDeviceToDevice, st_), "restore");
+    hip_check(hipMemcpyAsync(vel_.get(), c.vel.get(), n_ * sizeof(float4), hipMemcpyDeviceToDevice, st_), "restore");
+    hip_check(hipMemcpyAsync(id_.get(), c.id.get(), n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, st_), "restore");
+    hip_check(hipMemcpyAsync(mat_.get(), c.mat.get(), n_ * sizeof(uint32_t), hipMemcpyDeviceToDevice, st_), "restore");
+}
+
+void DeviceFluid::tape_begin(uint32_t K) {
+    if (K == 0) throw std::invalid_argument("tape: checkpoint interval must be >= 1");
+    if (!P_.newtonian || P_.n_mat != 1)
+        throw std::invalid_argument("tape: the adjoint supports one Newtonian material (visc_n = [1])");
+    if (has_body_) throw std::invalid_argument("tape: body_acc is not differentiated");
+    tape_end();
+    taping_ = true;
+    tape_K_ = K;
+    tape_n_ = n_;
+    tape_dt_.clear();
+    ckpt_.emplace_back(new Snapshot());
+    save_state(*ckpt_.back());
+}
+
+void DeviceFluid::tape_record(float dt) {
+    if (n_ != tape_n_) throw std::logic_error("tape: the particle set changed while recording");
+    tape_dt_.push_back(dt);
+    if (tape_dt_.size() % tape_K_ == 0) {
+        ckpt_.emplace_back(new Snapshot());
+        save_state(*ckpt_.back());
+    }
+}
+
+void DeviceFluid::tape_invalidate() {
+    if (taping_) tape_broken_ = true;
+}
+
+void DeviceFluid::tape_end() {
+    ckpt_.clear();
+    slots_.clear();
+    tape_dt_.clear();
+    taping_ = false;
+    tape_broken_ = false;
+}
+
+DeviceFluid::StepBufs DeviceFluid::slot_bufs(TapeSlot& t) {
+    const size_t n = n_ ? n_ : 1;
+    t.posm_s.ensure(n); t.velm_s.ensure(n); t.body_s.ensure(1);
+    t.id_s.ensure(n); t.mat_s.ensure(n); t.cell_s.ensure(n);
+    t.rho.ensure(n); t.prs.ensure(n); t.mp.ensure(n);
+    t.acc.ensure(n); t.xsph.ensure(n); t.fvisc.ensure(n); t.mu.ensure(n);
+    t.start.ensure(cell_cap_ + 2);
+    t.dom.ensure(1);
+    StepBufs s;
+    s.posm_s = t.posm_s.get(); s.velm_s = t.velm_s.get(); s.body_s = t.body_s.get();
+    s.id_s = t.id_s.get(); s.mat_s = t.mat_s.get(); s.cell_s = t.cell_s.get();
+    s.rho = t.rho.get(); s.prs = t.prs.get(); s.mp = t.mp.get();
+    s.start = t.start.get(); s.dom = t.dom.get();
+    s.acc = t.acc.get(); s.mu = t.mu.get(); s.xsph = t.xsph.get(); s.fvisc = t.fvisc.get();
+    return s;
+}
+
+void DeviceFluid::tape_backward(const float* xbar_T, const float* vbar_T, const float* rhobar_T,
+                                float* xbar0, float* vbar0, double* dk, double* dmu) {
+    if (!taping_) throw std::logic_error("tape: no recording (call tape_begin first)");
+    if (tape_broken_) throw std::logic_error("tape: particles were replaced while recording");
+    const size_t T = tape_dt_.size();
+    const int n = (int)n_;
+    *dk = *dmu = 0.0;
+    if (T == 0 || n == 0) {
+        for (size_t k = 0; k < 3 * n_; ++k) {
+            if (xbar0) xbar0[k] = xbar_T ? xbar_T[k] : 0.f;
+            if (vbar0) vbar0[k] = vbar_T ? vbar_T[k] : 0.f;
+        }
+        return;
+    }
+    const int nb = (n + 255) / 256;
+    for (int b = 0; b < 2; ++b) { xbar_[b].ensure(n); vbar_[b].ensure(n); }
+    Fb_.ensure(n); Sb_.ensure(n); xb_loc_.ensure(n); vb_loc_.ensure(n); xb_acc_.ensure(n);
+    rhob_loc_.ensure(n); rhob_.ensure(n); seed_.ensure(n);
+    adj_mu_.ensure(nb); adj_k_.ensure(nb); adj_tot_.ensure(2);
+    {
+        std::vector<float4> xs(n), vs(n);
+        for (int i = 0; i < n; ++i) {
+            xs[i] = xbar_T ? make_float4(xbar_T[3 * i], xbar_T[3 * i + 1], xbar_T[3 * i + 2], 0.f)
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+            vs[i] = vbar_T ? make_float4(vbar_T[3 * i], vbar_T[3 * i + 1], vbar_T[3 * i + 2], 0.f)
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        hip_check(hipMemcpyAsync(xbar_[0].get(), xs.data(), n * sizeof(float4), hipMemcpyHostToDevice, st_), "adj");
+        hip_check(hipMemcpyAsync(vbar_[0].get(), vs.data(), n * sizeof(float4), hipMemcpyHostToDevice, st_), "adj");
+        if (rhobar_T)
+            hip_check(hipMemcpyAsync(seed_.get(), rhobar_T, n * sizeof(float), hipMemcpyHostToDevice, st_), "adj");
+        double z[2] = {0.0, 0.0};
+        hip_check(hipMemcpyAsync(adj_tot_.get(), z, sizeof(z), hipMemcpyHostToDevice, st_), "adj");
+        sync();
+    }
+    Snapshot final_state;
+    save_state(final_state);
+    const bool keep_aux = keep_aux_;
+    const bool prof = prof_;
+    prof_ = false;
+    int cur = 0;
+    const size_t K = tape_K_;
+    const size_t nseg = (T + K - 1) / K;
+    if (slots_.size() < std::min(K, T)) {
+        slots_.clear();
+        for (size_t k = 0; k < std::min(K, T); ++k) slots_.emplace_back(new TapeSlot());
+    }
+    for (size_t seg = nseg; seg-- > 0;) {
+        load_state(*ckpt_[seg]);
+        const size_t s0 = seg * K, s1 = std::min(T, s0 + K);
+        for (size_t t = s0; t < s1; ++t) {
+            StepBufs sb = slot_bufs(*slots_[t - s0]);
+            step_core(tape_dt_[t], sb);
+            hip_check(hipMemcpyAsync(id_.get(), sb.id_s, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st_), "ids");
+            hip_check(hipMemcpyAsync(mat_.get(), sb.mat_s, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st_), "mats");
+        }
+        for (size_t t = s1; t-- > s0;) {
+            TapeSlot& sl = *slots_[t - s0];
+            AdjSlot S;
+            S.posm = sl.posm_s.get(); S.velm = sl.velm_s.get(); S.rho = sl.rho.get(); S.prs = sl.prs.get();
+            S.cell = sl.cell_s.get(); S.start = sl.start.get(); S.id = sl.id_s.get(); S.dom = sl.dom.get();
+            S.acc = sl.acc.get(); S.xsph = sl.xsph.get(); S.fvisc = sl.fvisc.get(); S.dt = tape_dt_[t];
+            AdjBuffers B;
+            B.xbar_in = xbar_[cur].get(); B.vbar_in = vbar_[cur].get();
+            B.Fb = Fb_.get(); B.Sb = Sb_.get(); B.xb_loc = xb_loc_.get(); B.vb_loc = vb_loc_.get();
+            B.rhob_loc = rhob_loc_.get(); B.rhob = rhob_.get(); B.xb_acc = xb_acc_.get();
+            B.xbar_out = xbar_[1 - cur].get(); B.vbar_out = vbar_[1 - cur].get();
+            B.mu_part = adj_mu_.get(); B.k_part = adj_k_.get(); B.totals = adj_tot_.get();
+            launch_adjoint_step(st_, n, S, P_, B, (t == T - 1 && rhobar_T) ? seed_.get() : nullptr);
+            hip_check(hipGetLastError(), "adjoint launch");
+            cur = 1 - cur;
+        }
+    }
+    std::vector<float4> xs(n), vs(n);
+    double tot[2];
+    hip_check(hipMemcpyAsync(xs.data(), xbar_[cur].get(), n * sizeof(float4), hipMemcpyDeviceToHost, st_), "adj out");
+    hip_check(hipMemcpyAsync(vs.data(), vbar_[cur].get(), n * sizeof(float4), hipMemcpyDeviceToHost, st_), "adj out");
+    hip_check(hipMemcpyAsync(tot, adj_tot_.get(), sizeof(tot), hipMemcpyDeviceToHost, st_), "adj out");
+    load_state(final_state);
+    sync();
+    for (int i = 0; i < n; ++i) {
+        if (xbar0) { xbar0[3 * i] = xs[i].x; xbar0[3 * i + 1] = xs[i].y; xbar0[3 * i + 2] = xs[i].z; }
+        if (vbar0) { vbar0[3 * i] = vs[i].x; vbar0[3 * i + 1] = vs[i].y; vbar0[3 * i + 2] = vs[i].z; }
+    }
+    *dk = tot[0];
+    *dmu = tot[1];
+    keep_aux_ = keep_aux;
+    prof_ = prof;
+    // the recomputation overwrote the last step's rho / p in the own buffers
+    // only if they were shared; the own buffers are untouched by slot steps.
 }
 
 void DeviceFluid::download(Field f, float* out) {

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <memory>
 #include <stdexcept>
 #include <utility>
 #include <string>
@@ -124,6 +125,21 @@ class DeviceFluid {
     hipStream_t stream() const { return st_; }
     void sync();
 
+    // Adjoint tape. tape_begin(K) marks the current state as the start of a
+    // recorded rollout; every later step is recorded and the state is
+    // checkpointed every K steps. tape_backward() takes the cotangents of the
+    // final x, v (3 per particle) and of the last step's rho (1 per particle),
+    // any of them null for zero, all in original index order, and returns the
+    // cotangents of the initial x, v and dL/d stiffness, dL/d visc_k[0]. Each
+    // segment is recomputed from its checkpoint into per-step slots, then the
+    // adjoint runs backwards through it. The world state is left unchanged.
+    void tape_begin(uint32_t K);
+    void tape_backward(const float* xbar_T, const float* vbar_T, const float* rhobar_T,
+                       float* xbar0, float* vbar0, double* dk, double* dmu);
+    void tape_end();
+    size_t tape_steps() const { return tape_dt_.size(); }
+    bool taping() const { return taping_; }
+
     // Per-stage HIP-event timing on this world's stream: while enabled, every
     // step records events around sort (domain + counting sort), density and
     // force; profile_end() synchronises and returns the summed milliseconds
@@ -133,9 +149,36 @@ class DeviceFluid {
     int device() const { return dev_; }
 
    private:
+    struct StepBufs {
+        float4 *posm_s, *velm_s, *body_s;
+        uint32_t *id_s, *mat_s, *cell_s;
+        float *rho, *prs, *mp;
+        uint32_t* start;
+        Domain* dom;
+        float4 *acc, *xsph, *fvisc;
+        float* mu;
+    };
+    struct Snapshot {
+        DevBuf<float4> posm, vel;
+        DevBuf<uint32_t> id, mat;
+    };
+    struct TapeSlot {
+        DevBuf<float4> posm_s, velm_s, body_s, acc, xsph, fvisc;
+        DevBuf<uint32_t> id_s, mat_s, cell_s, start;
+        DevBuf<float> rho, prs, mp, mu;
+        DevBuf<Domain> dom;
+    };
     void ensure_capacity(size_t n);
     void compute_cell_cap(const float* pos3, size_t n);
+    StepBufs own_bufs();
+    StepBufs slot_bufs(TapeSlot& t);
+    void run_domain_and_sort(const StepBufs& s);
     void run_domain_and_sort();
+    void step_core(float dt, const StepBufs& s);
+    void save_state(Snapshot& c);
+    void load_state(const Snapshot& c);
+    void tape_record(float dt);
+    void tape_invalidate();
 
     HostParams hp_;
     DevParams P_;
@@ -170,6 +213,16 @@ class DeviceFluid {
     bool prof_ = false;
     std::vector<hipEvent_t> prof_ev_;  // 4 per profiled step
     hipEvent_t ev();
+
+    bool taping_ = false, tape_broken_ = false;
+    uint32_t tape_K_ = 0;
+    size_t tape_n_ = 0;
+    std::vector<float> tape_dt_;
+    std::vector<std::unique_ptr<Snapshot>> ckpt_;  // state before steps 0, K, 2K, ...
+    std::vector<std::unique_ptr<TapeSlot>> slots_;
+    DevBuf<float4> xbar_[2], vbar_[2], Fb_, Sb_, xb_loc_, vb_loc_, xb_acc_;
+    DevBuf<float> rhob_loc_, rhob_, seed_;
+    DevBuf<double> adj_mu_, adj_k_, adj_tot_;
 };
 
 }  // namespace phy

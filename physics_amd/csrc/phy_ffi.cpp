@@ -434,6 +434,38 @@ int32_t phy_fluid_neighbor_stats(PhyWorldHandle* w, uint32_t* counts, uint64_t* 
     });
 }
 
+int32_t phy_fluid_tape_begin(PhyWorldHandle* w, uint32_t checkpoint_every) {
+    return guard<int32_t>(-1, [&] {
+        need_fluid(w)->world().tape_begin(checkpoint_every);
+        return 0;
+    });
+}
+
+uint32_t phy_fluid_tape_steps(PhyWorldHandle* w) {
+    return guard<uint32_t>(0, [&] { return (uint32_t)need_fluid(w)->world().tape_steps(); });
+}
+
+int32_t phy_fluid_tape_backward(PhyWorldHandle* w, const double* xbar_T, const double* vbar_T,
+                                const double* rhobar_T, double* xbar0, double* vbar0,
+                                double* dparams) {
+    return guard<int32_t>(-1, [&] {
+        double dk = 0.0, dmu = 0.0;
+        need_fluid(w)->world().tape_backward(xbar_T, vbar_T, rhobar_T, xbar0, vbar0, &dk, &dmu);
+        if (dparams) {
+            dparams[0] = dk;
+            dparams[1] = dmu;
+        }
+        return 0;
+    });
+}
+
+int32_t phy_fluid_tape_end(PhyWorldHandle* w) {
+    return guard<int32_t>(-1, [&] {
+        need_fluid(w)->world().tape_end();
+        return 0;
+    });
+}
+
 int32_t phy_fluid_sync(PhyWorldHandle* w) {
     return guard<int32_t>(-1, [&] {
         if (!w) throw std::invalid_argument("null world handle");

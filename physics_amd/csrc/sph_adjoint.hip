@@ -1,0 +1,303 @@
+// sph_adjoint.hip — reverse-mode (vector-Jacobian product) of one SPH step.
+//
+// The forward step (crates/phy-fluid/src/sph/world.rs:183-397) maps
+// (x, v; k = stiffness, mu = visc_k[0]) to (x', v') and the density rho.
+// Given the cotangents of (x', v') (and of rho on the last step), three
+// passes produce the cotangents of (x, v) and accumulate dL/dk and dL/dmu:
+//
+//   k_adj_local    per particle: wall clamp and semi-implicit Euler backwards;
+//                  F_bar = a_bar / rho (acceleration = force / rho), the XSPH
+//                  cotangent s_bar, and the local rho_bar, mu_bar terms.
+//   k_adj_force    gather over the force neighbours: every pair term of the
+//                  pressure, viscosity and XSPH sums, differentiated with
+//                  respect to the particle's own position, velocity, P = p /
+//                  rho^2 and rho, collected from both its own sums and its
+//                  neighbours' sums. The neighbour relation is symmetric, so
+//                  the transpose is a gather, not a scatter; then the EOS
+//                  p = k max(0, rho - rho0) backwards (dL/dk).
+//   k_adj_density  gather over the density neighbours: rho = sum m W(r) backwards.
+//
+// Neighbour sets and visiting order are the forward's (shared accept tests in
+// sph_device.h) and all sums are gathers, so the adjoint is deterministic.
+// Sums are plain FP32 within a particle; dL/dk and dL/dmu are reduced in FP64
+// over fixed block partials.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+#include "sph_types.h"
+#include "sph_device.h"
+#include "sph_kernels.h"
+
+namespace phy {
+
+__device__ __forceinline__ double block_sum_f64(double v, double* sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += sh[k];
+    return t;
+}
+
+// Integrate + bounds backwards (world.rs:352-397) and the acc = F / rho split.
+__global__ __launch_bounds__(256) void k_adj_local(int n, AdjSlot S, DevParams P,
+                                                   const float4* __restrict__ xbar_in,
+                                                   const float4* __restrict__ vbar_in, float4* Fb,
+                                                   float4* Sb, float4* xb_loc, float4* vb_loc,
+                                                   float* rhob_loc, double* mu_part) {
+    __shared__ double sh[4];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double mu_b = 0.0;
+    if (i < n) {
+        const uint32_t id = S.id[i];
+        const float4 xo = xbar_in[id], vo = vbar_in[id];
+        const float4 xs = S.xsph[i];
+        const uint32_t clamp = __float_as_uint(xs.w);
+        float xb[3] = {xo.x, xo.y, xo.z}, vb[3] = {vo.x, vo.y, vo.z};
+        for (int a = 0; a < 3; ++a) {
+            if (clamp & (1u << a)) {  // x' = wall, v' = -damp v'_pre
+                xb[a] = 0.f;
+                vb[a] = -P.damp * vb[a];
+            }
+        }
+        float vt[3], ab[3], sb[3];
+        for (int a = 0; a < 3; ++a) {
+            vt[a] = vb[a] + xb[a] * S.dt;  // x' = x + v' dt
+            ab[a] = vt[a] * S.dt;          // v' = v + (acc + g) dt + eps xsph
+            sb[a] = vt[a] * P.xsph_eps;
+        }
+        const float rho = S.rho[i];
+        const float4 acc = S.acc[i];
+        float F[3] = {0.f, 0.f, 0.f}, rb = 0.f;
+        if (rho > 0.f) {  // acc = (f_press + f_visc) / rho (world.rs:332-336)
+            for (int a = 0; a < 3; ++a) F[a] = ab[a] / rho;
+            rb = -(ab[0] * acc.x + ab[1] * acc.y + ab[2] * acc.z) / rho;
+            const float4 fv = S.fvisc[i];
+            mu_b = ((double)F[0] * fv.x + (double)F[1] * fv.y + (double)F[2] * fv.z) / (double)P.visc_k[0];
+        }
+        const float prs = S.prs[i];
+        Fb[i] = make_float4(F[0], F[1], F[2], prs / (rho * rho));
+        Sb[i] = make_float4(sb[0], sb[1], sb[2], 1.f / rho);
+        xb_loc[i] = make_float4(xb[0], xb[1], xb[2], 0.f);
+        vb_loc[i] = make_float4(vt[0], vt[1], vt[2], 0.f);
+        rhob_loc[i] = rb;
+    }
+    mu_b = block_sum_f64(mu_b, sh);
+    if (threadIdx.x == 0) mu_part[blockIdx.x] = mu_b;
+}
+
+struct AdjNbr {
+    float4 p, v, F, s;
+};
+
+struct ForceAdjAcc {
+    float A1;                     // sum m_b G (F_a - F_b) . D
+    float A2[3];                  // sum m_b (P_a + P_b) [G dF + (G'/r)(dF . D) D]
+    float A3[3], A4;              // sum m_b V F_b ; sum (m_b / rho_b) V
+    float A5;                     // sum m_b V (F_b . dv)
+    float A6[3];                  // sum m_b [(F_a . dv) / rho_b - (F_b . dv) / rho_a] D / r
+    float A7, A8[3], A9;          // sum (m_b / rho_b) W ; sum W s_b ; sum W (s_b . dv)
+    float A10[3];                 // sum (W'/r) [(m_b/rho_b)(s_a . dv) - (m_a/rho_a)(s_b . dv)] D
+};
+
+template <int L>
+__global__ __launch_bounds__(256) void k_adj_force(int n, AdjSlot S, DevParams P,
+                                                   const float4* __restrict__ Fb,
+                                                   const float4* __restrict__ Sb,
+                                                   const float4* __restrict__ xb_loc,
+                                                   const float4* __restrict__ vb_loc,
+                                                   const float* __restrict__ rhob_loc,
+                                                   const float* __restrict__ rhob_seed,
+                                                   float* rhob_out, float4* xb_acc, float4* vbar_out,
+                                                   double* k_part) {
+    __shared__ uint32_t lst[L][256];
+    __shared__ double sh[4];
+    const int tid = threadIdx.x;
+    const int a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + tid;
+    const bool act = a < n;
+    const Domain d = *S.dom;
+    float4 pa = make_float4(0.f, 0.f, 0.f, 0.f), va = pa, Fa = pa, sa = pa;
+    uint32_t ca = d.ncells;
+    if (act) {
+        pa = S.posm[a];
+        va = S.velm[a];
+        Fa = Fb[a];
+        sa = Sb[a];
+        ca = S.cell[a];
+    }
+    const float ma = pa.w, ira = sa.w, mra = va.w;  // m_a, 1/rho_a, m_a/rho_a
+    const float h = P.h, h2 = P.h2;
+    ForceAdjAcc A = {};
+    gather<L>(
+        act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, S.start, S.posm, lst, tid,
+        [&](uint32_t j, const float4& pj) { return force_accept(P, (uint32_t)a, pa.x, pa.y, pa.z, j, pj); },
+        [&](uint32_t j) { return AdjNbr{S.posm[j], S.velm[j], Fb[j], Sb[j]}; },
+        [&](const AdjNbr& nb, bool valid) {
+            const float Dx = nb.p.x - pa.x, Dy = nb.p.y - pa.y, Dz = nb.p.z - pa.z;
+            float s = Dx * Dx + Dy * Dy + Dz * Dz;
+            s = s < 1.17549435e-38f ? 1.17549435e-38f : s;
+            // padding slot (the particle itself, D = 0): evaluate the kernels at
+            // r = h/2 so every term stays finite before its zero weight applies
+            s = valid ? s : 0.25f * h2;
+            const float ir = __builtin_amdgcn_rsqf(s);
+            const float r = s * ir;
+            const float hr = h - r;
+            const float mb = nb.p.w, mrb = nb.v.w, irb = nb.s.w;
+            const float dvx = nb.v.x - va.x, dvy = nb.v.y - va.y, dvz = nb.v.z - va.z;
+            const float dFx = nb.F.x - Fa.x, dFy = nb.F.y - Fa.y, dFz = nb.F.z - Fa.z;
+            const float dFD = dFx * Dx + dFy * Dy + dFz * Dz;
+            // kernels: G = cs (h-r)^2 / r, G'/r = -cs (h-r)(h+r) / r^3,
+            // V = cv (h - r), W = c6 (h^2-r^2)^3, W'/r = -6 c6 (h^2-r^2)^2
+            const float G = P.cs * hr * hr * ir;
+            const float Gpr = -P.cs * hr * (h + r) * ir * ir * ir;
+            const float V = P.cv * hr;
+            const float q = h2 - s;
+            const float W = P.c6 * q * q * q;
+            const float Wpr = -6.f * P.c6 * q * q;
+            const float Fb_dv = nb.F.x * dvx + nb.F.y * dvy + nb.F.z * dvz;
+            const float Fa_dv = Fa.x * dvx + Fa.y * dvy + Fa.z * dvz;
+            const float sb_dv = nb.s.x * dvx + nb.s.y * dvy + nb.s.z * dvz;
+            const float sa_dv = sa.x * dvx + sa.y * dvy + sa.z * dvz;
+            const float w = valid ? 1.f : 0.f;  // padding slot: the particle itself, finite
+            const float mG = w * mb * G;
+            const float PP = Fa.w + nb.F.w;
+            A.A1 += -mG * dFD;
+            const float c2a = w * mb * PP * G, c2b = w * mb * PP * Gpr * dFD;
+            A.A2[0] += c2a * dFx + c2b * Dx;
+            A.A2[1] += c2a * dFy + c2b * Dy;
+            A.A2[2] += c2a * dFz + c2b * Dz;
+            const float mV = w * mb * V;
+            A.A3[0] += mV * nb.F.x;
+            A.A3[1] += mV * nb.F.y;
+            A.A3[2] += mV * nb.F.z;
+            A.A4 += w * mrb * V;
+            A.A5 += mV * Fb_dv;
+            const float c6v = w * mb * (Fa_dv * irb - Fb_dv * ira) * ir;
+            A.A6[0] += c6v * Dx;
+            A.A6[1] += c6v * Dy;
+            A.A6[2] += c6v * Dz;
+            const float wW = w * W;
+            A.A7 += wW * mrb;
+            A.A8[0] += wW * nb.s.x;
+            A.A8[1] += wW * nb.s.y;
+            A.A8[2] += wW * nb.s.z;
+            A.A9 += wW * sb_dv;
+            const float c10 = w * Wpr * (mrb * sa_dv - mra * sb_dv);
+            A.A10[0] += c10 * Dx;
+            A.A10[1] += c10 * Dy;
+            A.A10[2] += c10 * Dz;
+        },
+        [&]() {});
+    double kb = 0.0;
+    if (act) {
+        const float mu = P.visc_k[0];
+        const float4 xl = xb_loc[a], vl = vb_loc[a];
+        const float rho = S.rho[a];
+        float xb[3], vb[3];
+        const float c6m = mu * ma * P.cv;
+        xb[0] = xl.x + ma * A.A2[0] + c6m * A.A6[0] - A.A10[0];
+        xb[1] = xl.y + ma * A.A2[1] + c6m * A.A6[1] - A.A10[1];
+        xb[2] = xl.z + ma * A.A2[2] + c6m * A.A6[2] - A.A10[2];
+        vb[0] = vl.x + mu * ma * (A.A3[0] * ira - Fa.x * A.A4) - sa.x * A.A7 + mra * A.A8[0];
+        vb[1] = vl.y + mu * ma * (A.A3[1] * ira - Fa.y * A.A4) - sa.y * A.A7 + mra * A.A8[1];
+        vb[2] = vl.z + mu * ma * (A.A3[2] * ira - Fa.z * A.A4) - sa.z * A.A7 + mra * A.A8[2];
+        float rb = rhob_loc[a] + (mu * ma * ira * ira) * A.A5 + (ma * ira * ira) * A.A9;
+        // EOS backwards: P = p / rho^2, p = k max(0, rho - rho0)
+        const float Pbar = ma * A.A1;
+        const float pbar = Pbar * ira * ira;
+        const float prs = S.prs[a];
+        rb += -2.f * Pbar * prs * ira * ira * ira;
+        const float over = rho - P.rest;
+        if (over > 0.f) {
+            rb += P.stiff * pbar;
+            kb = (double)pbar * (double)over;
+        }
+        const uint32_t id = S.id[a];
+        if (rhob_seed) rb += rhob_seed[id];
+        rhob_out[a] = rb;
+        xb_acc[a] = make_float4(xb[0], xb[1], xb[2], 0.f);
+        vbar_out[id] = make_float4(vb[0], vb[1], vb[2], 0.f);
+    }
+    kb = block_sum_f64(kb, sh);
+    if (tid == 0) k_part[blockIdx.x] = kb;
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_adj_density(int n, AdjSlot S, DevParams P,
+                                                     const float* __restrict__ rhob,
+                                                     const float4* __restrict__ xb_acc,
+                                                     float4* xbar_out) {
+    __shared__ uint32_t lst[L][256];
+    const int tid = threadIdx.x;
+    const int a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + tid;
+    const bool act = a < n;
+    const Domain d = *S.dom;
+    float4 pa = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t ca = d.ncells;
+    if (act) {
+        pa = S.posm[a];
+        ca = S.cell[a];
+    }
+    float B1[3] = {0.f, 0.f, 0.f}, B2[3] = {0.f, 0.f, 0.f};
+    struct DN {
+        float4 p;
+        float rb;
+    };
+    gather<L>(
+        act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, S.start, S.posm, lst, tid,
+        [&](uint32_t j, const float4& pj) { return density_accept(P, pa.x, pa.y, pa.z, pj); },
+        [&](uint32_t j) { return DN{S.posm[j], rhob[j]}; },
+        [&](const DN& nb, bool valid) {
+            const float Dx = nb.p.x - pa.x, Dy = nb.p.y - pa.y, Dz = nb.p.z - pa.z;
+            const float s = Dx * Dx + Dy * Dy + Dz * Dz;
+            const float q = P.h2 - s;
+            const float Wpr = valid ? -6.f * P.c6 * q * q : 0.f;  // W'(r) / r
+            const float c1 = Wpr * nb.p.w, c2 = Wpr * nb.rb;
+            B1[0] += c1 * Dx;
+            B1[1] += c1 * Dy;
+            B1[2] += c1 * Dz;
+            B2[0] += c2 * Dx;
+            B2[1] += c2 * Dy;
+            B2[2] += c2 * Dz;
+        },
+        [&]() {});
+    if (!act) return;
+    // rho_a = sum_b m_b W(r_ab): x_bar_a -= W'/r (rho_bar_a m_b + rho_bar_b m_a) D
+    const float ra = rhob[a], ma = pa.w;
+    const float4 xa = xb_acc[a];
+    xbar_out[S.id[a]] = make_float4(xa.x - (ra * B1[0] + ma * B2[0]), xa.y - (ra * B1[1] + ma * B2[1]),
+                                    xa.z - (ra * B1[2] + ma * B2[2]), 0.f);
+}
+
+// Adds the block partials of one step to the running FP64 totals in a fixed order.
+__global__ void k_adj_reduce(int nb, const double* __restrict__ mu_part,
+                             const double* __restrict__ k_part, double* totals) {
+    if (threadIdx.x != 0) return;
+    double km = 0.0, kk = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        km += mu_part[b];
+        kk += k_part[b];
+    }
+    totals[0] += kk;
+    totals[1] += km;
+}
+
+static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
+
+void launch_adjoint_step(hipStream_t st, int n, const AdjSlot& S, const DevParams& P,
+                         const AdjBuffers& B, const float* rhob_seed) {
+    const int nb = nblk(n, 256);
+    hipLaunchKernelGGL(k_adj_local, dim3(nb), dim3(256), 0, st, n, S, P, B.xbar_in, B.vbar_in, B.Fb,
+                       B.Sb, B.xb_loc, B.vb_loc, B.rhob_loc, B.mu_part);
+    hipLaunchKernelGGL(k_adj_force<kForceList>, dim3(nb), dim3(256), 0, st, n, S, P, B.Fb, B.Sb,
+                       B.xb_loc, B.vb_loc, B.rhob_loc, rhob_seed, B.rhob, B.xb_acc, B.vbar_out,
+                       B.k_part);
+    hipLaunchKernelGGL(k_adj_density<kDensityList>, dim3(nb), dim3(256), 0, st, n, S, P, B.rhob,
+                       B.xb_acc, B.xbar_out);
+    hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(64), 0, st, nb, B.mu_part, B.k_part, B.totals);
+}
+
+}  // namespace phy

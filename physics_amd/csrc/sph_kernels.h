@@ -51,8 +51,45 @@ struct ForceArgs {
     float4* vel_out;
     float4* acc_out;          // optional (null: not stored)
     float* mu_out;
-    float4* xsph_out;
+    float4* xsph_out;         // w: wall-clamp flags (bit per axis)
+    float4* fvisc_out;        // optional viscous force (adjoint tape)
 };
+
+// One recorded step for the adjoint, in that step's cell order.
+struct AdjSlot {
+    const float4* posm;       // x, m
+    const float4* velm;       // v, m / rho
+    const float* rho;
+    const float* prs;
+    const uint32_t* cell;
+    const uint32_t* start;
+    const uint32_t* id;
+    const Domain* dom;
+    const float4* acc;
+    const float4* xsph;       // w: wall-clamp flags
+    const float4* fvisc;
+    float dt;
+};
+
+struct AdjBuffers {
+    const float4* xbar_in;    // cotangents of the step's output x', v' (original order)
+    const float4* vbar_in;
+    float4* Fb;               // cell order: (F_bar, P), (s_bar, 1/rho), local terms
+    float4* Sb;
+    float4* xb_loc;
+    float4* vb_loc;
+    float* rhob_loc;
+    float* rhob;
+    float4* xb_acc;
+    float4* xbar_out;         // cotangents of the step's input x, v (original order)
+    float4* vbar_out;
+    double* mu_part;          // per-block FP64 partials
+    double* k_part;
+    double* totals;           // running {dL/dk, dL/dmu}
+};
+
+void launch_adjoint_step(hipStream_t st, int n, const AdjSlot& S, const DevParams& P,
+                         const AdjBuffers& B, const float* rhob_seed);
 
 void launch_domain(hipStream_t st, int n, const float4* posm, double inv, Domain* d,
                    unsigned long long cap, const long long bk[6]);

@@ -21,87 +21,10 @@
 #include <math.h>
 
 #include "sph_types.h"
+#include "sph_device.h"
 #include "sph_kernels.h"
 
 namespace phy {
-
-// ---------------------------------------------------------------------------
-// helpers
-
-// num-traits ToPrimitive::to_i64 on the floored f64 key (grid.rs:29-31):
-// NaN and values outside [-2^63, 2^63) become 0 (unwrap_or(0)).
-__device__ __forceinline__ long long key1(float x, double inv) {
-    double f = floor((double)x * inv);
-    if (f >= -9223372036854775808.0 && f < 9223372036854775808.0) return (long long)f;
-    return 0;
-}
-
-__device__ __forceinline__ bool finite3(float x, float y, float z) {
-    return isfinite(x) && isfinite(y) && isfinite(z);
-}
-
-// Exact f64 distance predicate of the reference, r = sqrt((dx*dx + dy*dy) + dz*dz)
-// in f64 on the FP32 coordinates (world.rs:226, 295-296; nalgebra norm order).
-// Contraction is switched off so the f64 sum rounds exactly like the oracle's.
-__device__ __attribute__((noinline)) double exact_s64(float xi, float yi, float zi, float xj,
-                                                      float yj, float zj) {
-#pragma clang fp contract(off)
-    double dx = (double)xi - (double)xj;
-    double dy = (double)yi - (double)yj;
-    double dz = (double)zi - (double)zj;
-    return (dx * dx + dy * dy) + dz * dz;
-}
-
-// Classifies a candidate pair from its FP32 squared distance s32 (any rounding,
-// FMA allowed): returns 1 inside (r < h), 0 outside, 2 for NaN distance.
-// |s32 - s64| <= 6u * s, so the filter margins (2^-19 relative) are safe; the
-// band in between is decided by the exact f64 expression.
-__device__ __forceinline__ int classify(const DevParams& P, float s32, float xi, float yi, float zi,
-                                        float xj, float yj, float zj) {
-    if (s32 < P.s_lo) return 1;
-    if (s32 > P.s_hi) return 0;
-    double s = exact_s64(xi, yi, zi, xj, yj, zj);
-    if (s < P.s_thresh) return 1;
-    if (s != s) return 2;
-    return 0;
-}
-
-__device__ __forceinline__ void kahan_add(float& sum, float& c, float x) {
-    float y = x - c;
-    float t = sum + y;
-    c = (t - sum) - y;
-    sum = t;
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs; remap so consecutive
-// logical blocks (spatially adjacent cells) share one XCD's L2.
-__device__ __forceinline__ int xcd_block(int b, int nb) {
-    int per = nb >> 3, rem = nb & 7;
-    int x = b & 7, k = b >> 3;
-    return x * per + min(x, rem) + k;
-}
-
-__device__ __forceinline__ long long wave_min_ll(long long v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        long long w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
-}
-__device__ __forceinline__ long long wave_max_ll(long long v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        long long w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
-}
-__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-    for (int o = 32; o > 0; o >>= 1) {
-        uint32_t w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
-}
 
 // ---------------------------------------------------------------------------
 // domain
@@ -200,17 +123,6 @@ __global__ void k_domain_finalize(Domain* d, unsigned long long cap, long long b
 
 // ---------------------------------------------------------------------------
 // counting sort
-
-__device__ __forceinline__ uint32_t cell_of(float x, float y, float z, double inv, const Domain& d) {
-    // wrapping differences: an out-of-domain key maps to a huge unsigned value
-    unsigned long long k0 = (unsigned long long)key1(x, inv) - (unsigned long long)d.kmin[0];
-    unsigned long long k1 = (unsigned long long)key1(y, inv) - (unsigned long long)d.kmin[1];
-    unsigned long long k2 = (unsigned long long)key1(z, inv) - (unsigned long long)d.kmin[2];
-    if (k0 >= (unsigned long long)d.dims[0] || k1 >= (unsigned long long)d.dims[1] ||
-        k2 >= (unsigned long long)d.dims[2])
-        return d.ncells;
-    return (uint32_t)((k0 * d.dims[1] + k1) * d.dims[2] + k2);
-}
 
 // Particles arrive in the previous step's cell order, so equal cells form runs
 // of lanes: one atomic per run (wave-aggregated), each lane's arrival rank is
@@ -371,249 +283,6 @@ __global__ __launch_bounds__(256) void k_reorder(
 }
 
 // ---------------------------------------------------------------------------
-// neighbour visiting
-//
-// Rows (dx, dy) in the reference order with the three z-adjacent cells of each
-// row as one contiguous run [start[c0], start[c2 + 1]).
-
-struct CellCoord {
-    long long cx, cy, cz;
-};
-
-__device__ __forceinline__ CellCoord decode(uint32_t c, const Domain& d) {
-    CellCoord r;
-    long long dz = d.dims[2], dy = d.dims[1];
-    r.cz = c % dz;
-    long long t = c / dz;
-    r.cy = t % dy;
-    r.cx = t / dy;
-    return r;
-}
-
-// Run of row k (0..8) of an in-domain particle; empty when the row leaves the
-// domain (those cells hold no dense particles).
-__device__ __forceinline__ void row_run(int k, const CellCoord& cc, const Domain& d,
-                                        const uint32_t* __restrict__ start, uint32_t& a,
-                                        uint32_t& b) {
-    long long x = cc.cx + (k / 3) - 1;
-    long long y = cc.cy + (k % 3) - 1;
-    if (x < 0 || x >= d.dims[0] || y < 0 || y >= d.dims[1]) {
-        a = b = 0;
-        return;
-    }
-    long long z0 = cc.cz > 0 ? cc.cz - 1 : 0;
-    long long z1 = cc.cz + 1 < d.dims[2] ? cc.cz + 1 : d.dims[2] - 1;
-    long long base = (x * d.dims[1] + y) * d.dims[2];
-    a = start[base + z0];
-    b = start[base + z1 + 1];
-}
-
-// Reference cell adjacency on full i64 keys with wrapping offsets (grid.rs:44-48).
-__device__ __forceinline__ bool keys_adjacent(float xi, float yi, float zi, float xj, float yj,
-                                              float zj, double inv) {
-    unsigned long long a = (unsigned long long)key1(xj, inv) - (unsigned long long)key1(xi, inv) + 1ull;
-    unsigned long long b = (unsigned long long)key1(yj, inv) - (unsigned long long)key1(yi, inv) + 1ull;
-    unsigned long long c = (unsigned long long)key1(zj, inv) - (unsigned long long)key1(zi, inv) + 1ull;
-    return a <= 2ull && b <= 2ull && c <= 2ull;
-}
-
-// Visits every candidate j of particle (xi, yi, zi) in the reference order and
-// calls f(j). Dense cells come from the table; the overflow segment (particles
-// binned outside the dense domain, e.g. NaN keys) is checked by key adjacency.
-template <class F>
-__device__ __forceinline__ void for_each_candidate(uint32_t ci, float xi, float yi, float zi,
-                                                   const Domain& d, double inv,
-                                                   const uint32_t* __restrict__ start,
-                                                   const float4* __restrict__ posm, F&& f) {
-    if (ci < d.ncells) {
-        CellCoord cc = decode(ci, d);
-        for (int k = 0; k < 9; ++k) {
-            uint32_t a, b;
-            row_run(k, cc, d, start, a, b);
-            for (uint32_t j = a; j < b; ++j) f(j);
-        }
-    } else {
-        // out-of-domain particle: visit the in-domain cells among its 27 keys
-        long long k0 = key1(xi, inv), k1 = key1(yi, inv), k2 = key1(zi, inv);
-        for (int di = -1; di <= 1; ++di)
-            for (int dj = -1; dj <= 1; ++dj)
-                for (int dk = -1; dk <= 1; ++dk) {
-                    unsigned long long x = (unsigned long long)k0 + (unsigned long long)(long long)di -
-                                           (unsigned long long)d.kmin[0];
-                    unsigned long long y = (unsigned long long)k1 + (unsigned long long)(long long)dj -
-                                           (unsigned long long)d.kmin[1];
-                    unsigned long long z = (unsigned long long)k2 + (unsigned long long)(long long)dk -
-                                           (unsigned long long)d.kmin[2];
-                    if (x >= (unsigned long long)d.dims[0] || y >= (unsigned long long)d.dims[1] ||
-                        z >= (unsigned long long)d.dims[2])
-                        continue;
-                    uint32_t c = (uint32_t)((x * d.dims[1] + y) * d.dims[2] + z);
-                    for (uint32_t j = start[c]; j < start[c + 1]; ++j) f(j);
-                }
-    }
-    uint32_t o0 = start[d.ncells], o1 = start[d.ncells + 1];
-    for (uint32_t j = o0; j < o1; ++j) {
-        float4 q = posm[j];
-        if (keys_adjacent(xi, yi, zi, q.x, q.y, q.z, inv)) f(j);
-    }
-}
-
-// ---------------------------------------------------------------------------
-// neighbour compaction
-//
-// Candidates are tested with the cheap FP32 filter (exact f64 fallback in the
-// band) and accepted ones are appended to a per-lane list in LDS; the wave
-// drains the lists whenever one is full, so the heavy per-neighbour work runs
-// on dense lists instead of on every candidate under divergence (about 15 % of
-// the 27-cell candidates are neighbours at rest, 35 % in a collapsed sheet).
-//
-// Two scans produce the same lists in the same order:
-//  * uniform: every active lane of the wave sits in the same dense cell (the
-//    common case once the column has collapsed, ~400 particles per cell). The
-//    candidate sequence is then wave-uniform: loop control runs on the scalar
-//    unit and candidate loads are uniform.
-//  * per-lane: each lane walks its own rows with a cursor, one candidate per round.
-
-struct Cursor {
-    int row, nrows;
-    uint32_t j, jend, o0, o1;
-    bool dense;
-    CellCoord cc;
-    long long k0, k1, k2;
-
-    __device__ __forceinline__ bool in_overflow() const { return row > nrows; }
-
-    __device__ __forceinline__ void next_run(const Domain& d, const uint32_t* __restrict__ start) {
-        while (j >= jend && row <= nrows) {
-            if (row < nrows) {
-                if (dense) {
-                    row_run(row, cc, d, start, j, jend);
-                } else {
-                    int di = row / 9 - 1, dj = (row / 3) % 3 - 1, dk = row % 3 - 1;
-                    unsigned long long x = (unsigned long long)k0 + (unsigned long long)(long long)di -
-                                           (unsigned long long)d.kmin[0];
-                    unsigned long long y = (unsigned long long)k1 + (unsigned long long)(long long)dj -
-                                           (unsigned long long)d.kmin[1];
-                    unsigned long long z = (unsigned long long)k2 + (unsigned long long)(long long)dk -
-                                           (unsigned long long)d.kmin[2];
-                    if (x >= (unsigned long long)d.dims[0] || y >= (unsigned long long)d.dims[1] ||
-                        z >= (unsigned long long)d.dims[2]) {
-                        j = jend = 0;
-                    } else {
-                        uint32_t c = (uint32_t)((x * d.dims[1] + y) * d.dims[2] + z);
-                        j = start[c];
-                        jend = start[c + 1];
-                    }
-                }
-            } else {
-                j = o0;
-                jend = o1;
-            }
-            ++row;
-        }
-    }
-
-    __device__ __forceinline__ void init(bool act, uint32_t ci, float xi, float yi, float zi,
-                                         const Domain& d, double inv,
-                                         const uint32_t* __restrict__ start) {
-        row = 0;
-        j = jend = 0;
-        o0 = start[d.ncells];
-        o1 = start[d.ncells + 1];
-        dense = ci < d.ncells;
-        nrows = dense ? 9 : 27;
-        k0 = k1 = k2 = 0;
-        cc = CellCoord{0, 0, 0};
-        if (!act) return;
-        if (dense) {
-            cc = decode(ci, d);
-        } else {
-            k0 = key1(xi, inv);
-            k1 = key1(yi, inv);
-            k2 = key1(zi, inv);
-        }
-        next_run(d, start);
-    }
-};
-
-__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
-
-// test(j, pj) -> bool (FP32 filter, exact f64 band); load(j) -> T;
-// proc(T, valid) accumulates (an invalid slot must contribute exactly nothing);
-// flush() after every drain. self_j is a valid index used to pad drain slots.
-// Overflow-segment candidates additionally need reference key adjacency.
-template <int L, class Test, class Load, class Proc, class Flush>
-__device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi, float zi,
-                                       uint32_t self_j, const Domain& d, double inv,
-                                       const uint32_t* __restrict__ start,
-                                       const float4* __restrict__ posm, uint32_t (*lst)[256],
-                                       int tid, Test&& test, Load&& load, Proc&& proc,
-                                       Flush&& flush) {
-    int cnt = 0;
-    auto drain = [&]() {
-        for (int k = 0; __ballot(k < cnt) != 0ull; k += 2) {
-            const bool v0 = k < cnt, v1 = k + 1 < cnt;
-            const uint32_t j0 = v0 ? lst[k][tid] : self_j;
-            const uint32_t j1 = v1 ? lst[k + 1][tid] : self_j;
-            auto t0 = load(j0);
-            auto t1 = load(j1);
-            proc(t0, v0);
-            proc(t1, v1);
-        }
-        cnt = 0;
-        flush();
-    };
-    auto ov_ok = [&](const float4& pj) {
-        return keys_adjacent(xi, yi, zi, pj.x, pj.y, pj.z, inv);
-    };
-    const uint32_t ci0 = uni(ci);
-    const bool uniform = ci0 < d.ncells && __ballot(act && ci != ci0) == 0ull;
-    if (uniform) {
-        const CellCoord cc = decode(ci0, d);
-        for (int row = 0; row < 9; ++row) {
-            uint32_t a, b;
-            row_run(row, cc, d, start, a, b);
-            a = uni(a);
-            b = uni(b);
-            for (uint32_t j = a; j < b; j += 4) {
-                const uint32_t last = b - 1;
-                const uint32_t j1 = min(j + 1, last), j2 = min(j + 2, last), j3 = min(j + 3, last);
-                const float4 p0 = posm[j], p1 = posm[j1], p2 = posm[j2], p3 = posm[j3];
-                const bool t0 = act && test(j, p0);
-                const bool t1 = act && j + 1 <= last && test(j1, p1);
-                const bool t2 = act && j + 2 <= last && test(j2, p2);
-                const bool t3 = act && j + 3 <= last && test(j3, p3);
-                if (t0) lst[cnt++][tid] = j;
-                if (t1) lst[cnt++][tid] = j1;
-                if (t2) lst[cnt++][tid] = j2;
-                if (t3) lst[cnt++][tid] = j3;
-                if (__ballot(cnt > L - 4) != 0ull) drain();
-            }
-        }
-        const uint32_t o0 = uni(start[d.ncells]), o1 = uni(start[d.ncells + 1]);
-        for (uint32_t j = o0; j < o1; ++j) {
-            const float4 pj = posm[j];
-            if (act && ov_ok(pj) && test(j, pj)) lst[cnt++][tid] = j;
-            if (__ballot(cnt == L) != 0ull) drain();
-        }
-    } else {
-        Cursor cur;
-        cur.init(act, ci, xi, yi, zi, d, inv, start);
-        while (__ballot(act && cur.j < cur.jend) != 0ull) {
-            if (act && cur.j < cur.jend) {
-                const uint32_t j = cur.j;
-                const float4 pj = posm[j];
-                if ((!cur.in_overflow() || ov_ok(pj)) && test(j, pj)) lst[cnt++][tid] = j;
-                ++cur.j;
-                if (cur.j >= cur.jend) cur.next_run(d, start);
-            }
-            if (__ballot(cnt == L) != 0ull) drain();
-        }
-    }
-    drain();
-}
-
-// ---------------------------------------------------------------------------
 // density / EOS (world.rs:204-246)
 //
 // The pair geometry and the sum run in f64: p = k (rho - rho0) cancels near
@@ -643,16 +312,7 @@ __global__ __launch_bounds__(256) void k_density(int n, const float4* __restrict
     double sum = 0.0;
     gather<L>(
         act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, start, posm, lst, tid,
-        [&](uint32_t j, const float4& pj) {
-            float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-            float s = dx * dx + dy * dy + dz * dz;
-            bool in = s < P.s_lo;
-            if (!in && !(s > P.s_hi)) {  // f64 band or NaN: decide exactly (rare)
-                double e = exact_s64(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z);
-                in = e < P.s_thresh;
-            }
-            return in;
-        },
+        [&](uint32_t j, const float4& pj) { return density_accept(P, pi.x, pi.y, pi.z, pj); },
         [&](uint32_t j) { return posm[j]; },
         [&](const float4& pj, bool valid) {
             double dx = xi - (double)pj.x, dy = yi - (double)pj.y, dz = zi - (double)pj.z;
@@ -749,7 +409,7 @@ __global__ __launch_bounds__(256) void k_force(
     const uint32_t* __restrict__ cellv, const uint32_t* __restrict__ start,
     const uint32_t* __restrict__ idv, const uint32_t* __restrict__ matv,
     const float4* __restrict__ body, Domain* dom, DevParams P, float4* posm_out, float4* vel_out,
-    float4* acc_out, float* mu_out, float4* xsph_out) {
+    float4* acc_out, float* mu_out, float4* xsph_out, float4* fvisc_out) {
     __shared__ uint32_t lst[L][256];
     const int tid = threadIdx.x;
     const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + tid;
@@ -770,19 +430,7 @@ __global__ __launch_bounds__(256) void k_force(
     gather<L>(
         act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, start, posm, lst, tid,
         [&](uint32_t j, const float4& pj) {
-            float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-            float s = dx * dx + dy * dy + dz * dz;
-            // s > 0 rejects the particle itself and coincident points (r <= 0)
-            bool ok = s < P.s_lo && s > 0.f;
-            bool rare = (!(s < P.s_lo) && !(s > P.s_hi)) || (s == 0.f && j != (uint32_t)i);
-            if (rare) {
-                // f64 band, NaN distance (not skipped by the reference,
-                // world.rs:297-299) or an underflowed distinct pair
-                double e = exact_s64(pi.x, pi.y, pi.z, pj.x, pj.y, pj.z);
-                bool nz = dx != 0.f || dy != 0.f || dz != 0.f;
-                ok = j != (uint32_t)i && ((e < P.s_thresh && nz) || e != e);
-            }
-            return ok;
+            return force_accept(P, (uint32_t)i, pi.x, pi.y, pi.z, j, pj);
         },
         [&](uint32_t j) { return NbrData{posm[j], velm[j], mp[j]}; },
         [&](const NbrData& nb, bool valid) { pair_force<SHEAR>(A, P, pi, vi, Pi, nb, valid); },
@@ -804,10 +452,13 @@ __global__ __launch_bounds__(256) void k_force(
         float sreg = sh > P.shear_min ? sh : P.shear_min;
         mu = kc * powf(sreg, P.visc_n[m] - 1.f);
     }
-    float acc[3] = {0.f, 0.f, 0.f};
+    float acc[3] = {0.f, 0.f, 0.f}, fvis[3] = {0.f, 0.f, 0.f};
     if (rho_i > 0.f) {
         float mcs = pi.w * P.cs, mcv = pi.w * P.cv * mu;
-        for (int a = 0; a < 3; ++a) acc[a] = (mcs * A.s[a] + mcv * A.s[3 + a]) / rho_i;
+        for (int a = 0; a < 3; ++a) {
+            fvis[a] = mcv * A.s[3 + a];
+            acc[a] = (mcs * A.s[a] + fvis[a]) / rho_i;
+        }
     }
     float xs[3] = {P.c6 * A.s[6], P.c6 * A.s[7], P.c6 * A.s[8]};
 
@@ -820,6 +471,7 @@ __global__ __launch_bounds__(256) void k_force(
         bacc[2] = b.z;
     }
     float x[3] = {pi.x, pi.y, pi.z}, v[3] = {vi.x, vi.y, vi.z};
+    uint32_t clamp = 0;  // bit a: axis a clamped to a wall (the adjoint's bounds mask)
     for (int a = 0; a < 3; ++a) {
         float aa = (acc[a] + P.g[a]) + bacc[a];
         v[a] = v[a] + aa * dt;
@@ -828,9 +480,11 @@ __global__ __launch_bounds__(256) void k_force(
         if (x[a] < P.lo[a]) {
             x[a] = P.lo[a];
             v[a] = -v[a] * P.damp;
+            clamp |= 1u << a;
         } else if (x[a] > P.hi[a]) {
             x[a] = P.hi[a];
             v[a] = -v[a] * P.damp;
+            clamp |= 1u << a;
         }
     }
     posm_out[i] = make_float4(x[0], x[1], x[2], pi.w);
@@ -838,8 +492,9 @@ __global__ __launch_bounds__(256) void k_force(
     if (acc_out) {
         acc_out[i] = make_float4(acc[0], acc[1], acc[2], 0.f);
         mu_out[i] = mu;
-        xsph_out[i] = make_float4(xs[0], xs[1], xs[2], 0.f);
+        xsph_out[i] = make_float4(xs[0], xs[1], xs[2], __uint_as_float(clamp));
     }
+    if (fvisc_out) fvisc_out[i] = make_float4(fvis[0], fvis[1], fvis[2], 0.f);
 
     // watchdog: smallest original index with a non-finite position / velocity
     uint32_t id = idv[i];
@@ -980,11 +635,11 @@ void launch_force(hipStream_t st, int n, float dt, const ForceArgs& a, Domain* d
     if (P.newtonian)
         hipLaunchKernelGGL((k_force<kForceList, false>), dim3(nblk(n, 256)), dim3(256), 0, st, n, dt,
                            a.posm, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body, d,
-                           P, a.posm_out, a.vel_out, a.acc_out, a.mu_out, a.xsph_out);
+                           P, a.posm_out, a.vel_out, a.acc_out, a.mu_out, a.xsph_out, a.fvisc_out);
     else
         hipLaunchKernelGGL((k_force<kForceList, true>), dim3(nblk(n, 256)), dim3(256), 0, st, n, dt,
                            a.posm, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body, d,
-                           P, a.posm_out, a.vel_out, a.acc_out, a.mu_out, a.xsph_out);
+                           P, a.posm_out, a.vel_out, a.acc_out, a.mu_out, a.xsph_out, a.fvisc_out);
 }
 
 void launch_diag(hipStream_t st, int n, const float4* posm, const float4* vel, double* partial,

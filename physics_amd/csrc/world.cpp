@@ -284,6 +284,29 @@ void FluidWorld::set_body_acc(const double* b3) {
 }
 
 void FluidWorld::set_keep_aux(bool on) { dev_.set_keep_aux(on); }
+
+void FluidWorld::tape_begin(uint32_t K) {
+    flush_host();
+    if (!body_pending_.empty()) throw std::invalid_argument("tape: body_acc is not differentiated");
+    dev_.tape_begin(K);
+}
+
+void FluidWorld::tape_backward(const double* xbar_T, const double* vbar_T, const double* rhobar_T,
+                               double* xbar0, double* vbar0, double* dk, double* dmu) {
+    const size_t n = dev_.size();
+    std::vector<float> xb, vb, rb, xo(3 * n), vo(3 * n);
+    if (xbar_T) xb.assign(xbar_T, xbar_T + 3 * n);
+    if (vbar_T) vb.assign(vbar_T, vbar_T + 3 * n);
+    if (rhobar_T) rb.assign(rhobar_T, rhobar_T + n);
+    dev_.tape_backward(xbar_T ? xb.data() : nullptr, vbar_T ? vb.data() : nullptr,
+                       rhobar_T ? rb.data() : nullptr, xo.data(), vo.data(), dk, dmu);
+    for (size_t k = 0; k < 3 * n; ++k) {
+        if (xbar0) xbar0[k] = xo[k];
+        if (vbar0) vbar0[k] = vo[k];
+    }
+}
+
+void FluidWorld::tape_end() { dev_.tape_end(); }
 int FluidWorld::last_validate() { return dev_.last_validate(); }
 void FluidWorld::diagnostics(double out5[5]) {
     flush_host();

@@ -38,7 +38,8 @@ EXPORTS = [
     "phy_fluid_set_particles_f32", "phy_fluid_get_field", "phy_fluid_get_field_f32",
     "phy_fluid_set_keep_aux", "phy_fluid_set_body_acc", "phy_fluid_diagnostics_f64",
     "phy_fluid_neighbor_stats", "phy_world_step_n", "phy_fluid_sync", "phy_fluid_profile_begin",
-    "phy_fluid_profile_end",
+    "phy_fluid_profile_end", "phy_fluid_tape_begin", "phy_fluid_tape_steps", "phy_fluid_tape_backward",
+    "phy_fluid_tape_end",
 ]
 
 
@@ -152,6 +153,10 @@ def lib():
         "phy_fluid_sync": ([H], C.c_int32),
         "phy_fluid_profile_begin": ([H], C.c_int32),
         "phy_fluid_profile_end": ([H, dp, u32p], C.c_int32),
+        "phy_fluid_tape_begin": ([H, C.c_uint32], C.c_int32),
+        "phy_fluid_tape_steps": ([H], C.c_uint32),
+        "phy_fluid_tape_backward": ([H, dp, dp, dp, dp, dp, dp], C.c_int32),
+        "phy_fluid_tape_end": ([H], C.c_int32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -320,6 +325,33 @@ class FluidWorld(World):
         if lib().phy_fluid_profile_end(self._h, _dp(ms), C.byref(steps)) != 0:
             raise PhyError(last_error())
         return ms, int(steps.value)
+
+    def tape_begin(self, checkpoint_every: int) -> None:
+        """Start recording a rollout for reverse-mode gradients."""
+        if lib().phy_fluid_tape_begin(self._h, int(checkpoint_every)) != 0:
+            raise PhyError(last_error())
+
+    def tape_steps(self) -> int:
+        return int(lib().phy_fluid_tape_steps(self._h))
+
+    def tape_backward(self, xbar=None, vbar=None, rhobar=None):
+        """Cotangents of (final x, final v, last-step rho) -> (x0_bar, v0_bar,
+        dL/d stiffness, dL/d visc_k[0])."""
+        n = len(self)
+        xb = None if xbar is None else np.ascontiguousarray(xbar, dtype=np.float64).reshape(n, 3)
+        vb = None if vbar is None else np.ascontiguousarray(vbar, dtype=np.float64).reshape(n, 3)
+        rb = None if rhobar is None else np.ascontiguousarray(rhobar, dtype=np.float64).reshape(n)
+        x0 = np.zeros((n, 3))
+        v0 = np.zeros((n, 3))
+        dp = np.zeros(2)
+        rc = lib().phy_fluid_tape_backward(self._h, _dp(xb), _dp(vb), _dp(rb), _dp(x0), _dp(v0), _dp(dp))
+        if rc != 0:
+            raise PhyError(last_error())
+        return x0, v0, float(dp[0]), float(dp[1])
+
+    def tape_end(self) -> None:
+        if lib().phy_fluid_tape_end(self._h) != 0:
+            raise PhyError(last_error())
 
     def neighbor_stats(self):
         n = len(self)

@@ -1,0 +1,139 @@
+"""Reverse-mode gradients of GPU rollouts against central finite differences
+of the f64 oracle (the reference step restated; no adjoint exists in the
+reference).
+
+Loss (the differentiable-rollout configuration): L = |centroid(x_N)|^2 +
+mean_i (rho_i - rho0)^2, with rho the density of the last step. Directions in
+(v0, stiffness, visc_k[0]) use eps = 1e-4; directions in x0 cross the C0
+kinks of the viscosity kernel at r = h and of max(0, rho - rho0), which make
+a 1e-4 central difference only O(eps)-accurate, so they use a short rollout
+and eps = 1e-6. The walls are out of reach: a wall reflection is
+discontinuous in the initial state, so no finite difference can check it.
+Tolerance: 1e-3 relative (FP32 adjoint along the FP32 trajectory vs f64 FD).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from physics_amd import ffi, scenes
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-3
+DT32 = float(np.float32(scenes.DT))
+
+
+def _setup(nside=6, seed=3):
+    pos = scenes.lattice_block(nside, nside, nside, scenes.S32, (0.2, 0.5, -0.3), 0.01, seed)
+    n = len(pos)
+    vel = (0.1 * np.sqrt(-2 * np.log(1 - scenes.rng_f64(seed + 10, 3 * n))) *
+           np.cos(2 * np.pi * scenes.rng_f64(seed + 20, 3 * n))).reshape(n, 3).astype(np.float32)
+    mass = np.float32(scenes.lattice_mass(scenes.H32, scenes.S32) * 1.3)
+    return pos, vel, mass
+
+
+def _params(cls, k=250.0, mu=3.5):
+    p = cls.defaults()
+    p.h = scenes.H32
+    p.stiffness = k
+    p.set_visc([mu], [1.0])
+    for a in range(3):
+        p.bounds_min[a] = -100.0
+        p.bounds_max[a] = 100.0
+    return p
+
+
+def _loss(x, rho):
+    c = x.mean(0)
+    return float(c @ c + ((rho - 1000.0) ** 2).mean())
+
+
+def _gpu_grad(pos, vel, mass, steps, K):
+    g = ffi.FluidWorld(_params(ffi.PhySphParams))
+    n = len(pos)
+    g.set_particles_f32(pos, vel, np.full(n, mass, np.float32))
+    g.tape_begin(K)
+    g.step(DT32, steps)
+    assert g.tape_steps() == steps
+    x, rho = g.get("pos"), g.get("rho")
+    c = x.mean(0)
+    xbar = np.tile(2.0 * c / n, (n, 1))
+    rhobar = 2.0 * (rho - 1000.0) / n
+    x0b, v0b, dk, dmu = g.tape_backward(xbar, None, rhobar)
+    return _loss(x, rho), x0b, v0b, dk, dmu, g
+
+
+def _oracle_loss(pos, vel, mass, steps, dk=0.0, dmu=0.0):
+    w = O.World(_params(O.Params, 250.0 + dk, 3.5 + dmu))
+    w.set_particles(pos, vel, np.full(len(pos), float(mass)))
+    w.step(DT32, steps)
+    return _loss(w.get("pos"), w.get("rho"))
+
+
+def _fd(pos, vel, mass, steps, dx, dv, dkk, dmm, eps):
+    p64, v64 = pos.astype(np.float64), vel.astype(np.float64)
+    lp = _oracle_loss(p64 + eps * dx, v64 + eps * dv, mass, steps, eps * dkk, eps * dmm)
+    lm = _oracle_loss(p64 - eps * dx, v64 - eps * dv, mass, steps, -eps * dkk, -eps * dmm)
+    return (lp - lm) / (2 * eps)
+
+
+def test_gradients_match_fd_velocity_and_parameters():
+    pos, vel, mass = _setup()
+    steps = 30
+    L, x0b, v0b, dk, dmu, _ = _gpu_grad(pos, vel, mass, steps, K=7)
+    L64 = _oracle_loss(pos.astype(np.float64), vel.astype(np.float64), mass, steps)
+    assert abs(L - L64) / abs(L64) < 1e-5
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for t in range(8):
+        dv = rng.normal(size=vel.shape)
+        dkk, dmm = rng.normal() * 10.0, rng.normal() * 0.1
+        if t == 0:
+            dv[:] = 0.0; dmm = 0.0          # stiffness alone
+        if t == 1:
+            dv[:] = 0.0; dkk = 0.0          # viscosity alone
+        if t == 2:
+            dkk = dmm = 0.0                 # initial velocities alone
+        fd = _fd(pos, vel, mass, steps, np.zeros_like(dv), dv, dkk, dmm, 1e-4)
+        ad = float((v0b * dv).sum() + dk * dkk + dmu * dmm)
+        rel = abs(ad - fd) / max(abs(fd), 1e-30)
+        print(f"direction {t}: adjoint {ad:.8e} fd {fd:.8e} rel {rel:.2e}", flush=True)
+        worst = max(worst, rel)
+    assert worst < TOL
+
+
+def test_gradients_match_fd_positions_short_rollout():
+    pos, vel, mass = _setup(nside=5, seed=4)
+    steps = 10
+    _, x0b, v0b, dk, dmu, _ = _gpu_grad(pos, vel, mass, steps, K=4)
+    rng = np.random.default_rng(12)
+    worst = 0.0
+    for t in range(4):
+        dx = rng.normal(size=pos.shape)
+        fd = _fd(pos, vel, mass, steps, dx, np.zeros_like(dx), 0.0, 0.0, 1e-6)
+        ad = float((x0b * dx).sum())
+        rel = abs(ad - fd) / abs(fd)
+        print(f"x0 direction {t}: adjoint {ad:.8e} fd {fd:.8e} rel {rel:.2e}", flush=True)
+        worst = max(worst, rel)
+    assert worst < TOL
+
+
+def test_gradients_do_not_depend_on_checkpoint_interval():
+    pos, vel, mass = _setup(nside=6, seed=5)
+    outs = [_gpu_grad(pos, vel, mass, 20, K)[1:5] for K in (1, 6, 20)]
+    for o in outs[1:]:
+        assert np.array_equal(o[0], outs[0][0]) and np.array_equal(o[1], outs[0][1])
+        assert o[2] == outs[0][2] and o[3] == outs[0][3]
+
+
+def test_backward_leaves_the_world_state_and_reruns_identically():
+    pos, vel, mass = _setup(nside=6, seed=6)
+    L, x0b, v0b, dk, dmu, g = _gpu_grad(pos, vel, mass, 15, K=5)
+    x_before = g.get("pos")
+    again = g.tape_backward(np.ones((len(pos), 3)), None, None)
+    again2 = g.tape_backward(np.ones((len(pos), 3)), None, None)
+    assert np.array_equal(g.get("pos"), x_before)
+    assert np.array_equal(again[0], again2[0]) and again[2] == again2[2]
+    g.tape_end()
+    with pytest.raises(ffi.PhyError):
+        g.tape_backward(np.ones((len(pos), 3)), None, None)
