@@ -43,6 +43,10 @@ def parse():
                     help="seconds of oracle CPU work for the baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--verbose", action="store_true", help="per-chunk timings on stderr")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adj-side", type=int, default=32)
+    ap.add_argument("--adj-steps", type=int, default=100)
+    ap.add_argument("--adj-k", type=int, default=10)
     return ap.parse_args()
 
 
@@ -138,6 +142,10 @@ def main():
         },
     }
 
+    if not a.no_adjoint:
+        out["adjoint"] = adjoint_leg(a)
+        out["adjoint_forward_ratio"] = out["adjoint"]["ratio"]
+
     if not a.no_cpu and snaps:
         import oracle as O
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
@@ -169,6 +177,57 @@ def main():
                       f"{used} of {len(sizes)}; {t_cpu:.1f} s of CPU work",
         }
     print(json.dumps(out), flush=True)
+
+
+def adjoint_leg(a):
+    """Forward rollout vs its reverse-mode gradient on the differentiable-rollout
+    block: ratio = t(tape_backward: checkpoint recomputation + adjoint steps) /
+    t(forward rollout of the same steps)."""
+    from physics_amd import ffi, scenes
+    sc = scenes.rollout_block(a.adj_side, seed=1)
+    n = len(sc.pos)
+    gp = scenes.apply_scene(ffi.PhySphParams.defaults(), sc)
+    m = np.full(n, sc.mass, dtype=np.float32)
+    dt = float(np.float32(sc.dt))
+    g = ffi.FluidWorld(gp)
+    # warm-up (kernel loading, allocations)
+    g.set_particles_f32(sc.pos, sc.vel, m)
+    g.step(dt, 5)
+    g.tape_begin(a.adj_k)
+    g.step(dt, 5)
+    g.tape_backward(np.zeros((n, 3)), None, None)
+    g.tape_end()
+    # plain forward rollout
+    g.set_particles_f32(sc.pos, sc.vel, m)
+    g.sync()
+    t0 = time.perf_counter()
+    g.step(dt, a.adj_steps)
+    g.sync()
+    t_fwd = time.perf_counter() - t0
+    # recorded forward + backward
+    g.set_particles_f32(sc.pos, sc.vel, m)
+    g.sync()
+    t0 = time.perf_counter()
+    g.tape_begin(a.adj_k)
+    g.step(dt, a.adj_steps)
+    g.sync()
+    t_rec = time.perf_counter() - t0
+    L, xbar, rhobar = scenes.rollout_loss_cotangents(g.get("pos"), g.get("rho"))
+    t0 = time.perf_counter()
+    _, v0b, dk, dmu = g.tape_backward(xbar, None, rhobar)[0:4]
+    t_bwd = time.perf_counter() - t0
+    g.tape_end()
+    return {
+        "config": f"rollout block {a.adj_side}^3 = {n} particles, {a.adj_steps} FP32 steps, "
+                  f"checkpoint every {a.adj_k}; loss |centroid|^2 + mean (rho - rho0)^2",
+        "forward_ms": t_fwd * 1e3,
+        "recorded_forward_ms": t_rec * 1e3,
+        "backward_ms": t_bwd * 1e3,
+        "ratio": t_bwd / t_fwd,
+        "loss": L,
+        "dL_dstiffness": dk,
+        "dL_dviscosity": dmu,
+    }
 
 
 if __name__ == "__main__":

@@ -106,3 +106,30 @@ def apply_scene(params, scene: Scene):
         params.bounds_min[a] = scene.bounds_min[a]
         params.bounds_max[a] = scene.bounds_max[a]
     return params
+
+
+def rollout_block(nside: int = 32, seed: int = 1, vel_sigma: float = 0.1,
+                  mass_scale: float = 1.0) -> Scene:
+    """The differentiable-rollout block: nside^3 lattice at spacing h/2 with
+    +-1 % seeded jitter and seeded N(0, vel_sigma) velocities (Box-Muller over
+    SplitMix64), placed in the default +-5 m tank high enough that a 100-step
+    (1 s) free fall does not reach the floor."""
+    width = nside * S32
+    origin = (-0.5 * width, 4.7 - width, -0.5 * width)
+    pos = lattice_block(nside, nside, nside, S32, origin, 0.01, seed)
+    n = len(pos)
+    u1 = rng_f64(seed + 1000, 3 * n)
+    u2 = rng_f64(seed + 2000, 3 * n)
+    z = np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+    vel = (vel_sigma * z).reshape(n, 3).astype(np.float32)
+    m = lattice_mass(H32, S32) * mass_scale
+    return Scene(pos=pos, vel=vel, mass=m, h=H32, bounds_min=(-5.0, -5.0, -5.0),
+                 bounds_max=(5.0, 5.0, 5.0))
+
+
+def rollout_loss_cotangents(x: np.ndarray, rho: np.ndarray, rest_density: float = 1000.0):
+    """L = |centroid(x)|^2 + mean (rho - rho0)^2 and its cotangents (dL/dx, dL/drho)."""
+    n = len(x)
+    c = x.mean(0)
+    L = float(c @ c + ((rho - rest_density) ** 2).mean())
+    return L, np.tile(2.0 * c / n, (n, 1)), 2.0 * (rho - rest_density) / n

@@ -9,7 +9,9 @@ kinks of the viscosity kernel at r = h and of max(0, rho - rho0), which make
 a 1e-4 central difference only O(eps)-accurate, so they use a short rollout
 and eps = 1e-6. The walls are out of reach: a wall reflection is
 discontinuous in the initial state, so no finite difference can check it.
-Tolerance: 1e-3 relative (FP32 adjoint along the FP32 trajectory vs f64 FD).
+Tolerance: 1e-3 relative (FP32 adjoint along the FP32 trajectory vs f64 FD),
+per direction for the small blocks, and against the RMS directional
+derivative for the 32^3 rollout block (whose FD is itself noisy at 1e-4).
 """
 import numpy as np
 import pytest
@@ -137,3 +139,63 @@ def test_backward_leaves_the_world_state_and_reruns_identically():
     g.tape_end()
     with pytest.raises(ffi.PhyError):
         g.tape_backward(np.ones((len(pos), 3)), None, None)
+
+
+@pytest.mark.timeout(600)
+def test_rollout_block_gradients_match_fd():
+    """The differentiable-rollout configuration: 32^3 block, 100 FP32 steps,
+    checkpoint every 10, central FD (eps = 1e-4) of the f64 oracle along 64
+    seeded random directions in (v0, stiffness, visc_k[0]).
+
+    The error of each direction is taken relative to the RMS of the 64
+    directional derivatives, not to the direction's own value: at this size
+    the f64 central difference itself moves by 1e-4..4e-4 of that scale
+    between eps = 1e-3 and 1e-6 (thousands of pairs cross the C0 kinks at
+    r = h per rollout), so a direction whose derivative cancels to ~10 % of
+    the scale can show a 2e-3 per-direction relative difference that is FD
+    noise, not adjoint error. PHY_FD_DIRECTIONS overrides the count."""
+    import os
+    ndir = int(os.environ.get("PHY_FD_DIRECTIONS", "64"))
+    sc = scenes.rollout_block(32, seed=1)
+    n = len(sc.pos)
+    gp = scenes.apply_scene(ffi.PhySphParams.defaults(), sc)
+    g = ffi.FluidWorld(gp)
+    g.set_particles_f32(sc.pos, sc.vel, np.full(n, sc.mass, np.float32))
+    g.tape_begin(10)
+    g.step(DT32, 100)
+    L, xbar, rhobar = scenes.rollout_loss_cotangents(g.get("pos"), g.get("rho"))
+    _, v0b, dk, dmu = g.tape_backward(xbar, None, rhobar)
+
+    op = scenes.apply_scene(O.Params.defaults(), sc)
+    p64, v64 = sc.pos.astype(np.float64), sc.vel.astype(np.float64)
+    m64 = np.full(n, float(np.float32(sc.mass)))
+
+    def loss(dv, dkk, dmm):
+        q = op.copy()
+        q.stiffness = op.stiffness + dkk
+        q.set_visc([3.5 + dmm], [1.0])
+        w = O.World(q)
+        w.set_particles(p64, v64 + dv, m64)
+        w.step(DT32, 100)
+        return scenes.rollout_loss_cotangents(w.get("pos"), w.get("rho"))[0]
+
+    L64 = loss(0.0, 0.0, 0.0)
+    print(f"loss GPU {L:.10e} f64 {L64:.10e}", flush=True)
+    assert abs(L - L64) / abs(L64) < 1e-5
+    eps = 1e-4
+    ads, fds = [], []
+    for t in range(ndir):
+        r = np.random.default_rng(1000 + t)
+        dv = r.normal(size=(n, 3))
+        dkk, dmm = r.normal() * 10.0, r.normal() * 0.1
+        fd = (loss(eps * dv, eps * dkk, eps * dmm) - loss(-eps * dv, -eps * dkk, -eps * dmm)) / (2 * eps)
+        ad = float((v0b * dv).sum() + dk * dkk + dmu * dmm)
+        ads.append(ad)
+        fds.append(fd)
+        print(f"direction {t}: adjoint {ad:.8e} fd {fd:.8e} rel {abs(ad - fd) / abs(fd):.2e}", flush=True)
+    ads, fds = np.array(ads), np.array(fds)
+    scale = np.sqrt(np.mean(fds ** 2))
+    err = np.abs(ads - fds) / scale
+    print(f"{ndir} directions: max |adjoint - fd| / rms(fd) = {err.max():.2e}, "
+          f"median per-direction relative {np.median(np.abs(ads - fds) / np.abs(fds)):.2e}", flush=True)
+    assert err.max() < TOL
