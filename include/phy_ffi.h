@@ -237,6 +237,69 @@ int32_t phy_fluid_sync(PhyWorldHandle *w);
 int32_t phy_fluid_profile_begin(PhyWorldHandle *w);
 int32_t phy_fluid_profile_end(PhyWorldHandle *w, double *ms4, uint32_t *steps);
 
+/* Capture the fluid step of this world in a hipGraph and replay it on every
+ * following step (recaptured when dt, the particles or the parameters change).
+ * Results are bit-identical to eager steps. 0 ok, -1 error. */
+int32_t phy_fluid_set_graph(PhyWorldHandle *w, int32_t on);
+
+/* ---- Batches of independent scenes (no counterpart in the reference) ----
+ * A batch holds n_scenes independent FluidWorlds of uneven particle counts in
+ * one set of device arrays: scene s owns particles [offsets[s], offsets[s+1])
+ * in concatenated index order and follows FluidWorld::step with its own
+ * SphParams (bounds, viscosity, ...). All scenes share dt; every stage of a
+ * step is one launch over all scenes, and the step is replayed from a captured
+ * hipGraph (phy_fluid_batch_set_graph(b, 0) steps eagerly). Each scene's
+ * result is bit-identical to the same scene stepped alone. */
+typedef struct PhyFluidBatch PhyFluidBatch;
+
+/* params: n_scenes entries. NULL on error (phy_last_error). */
+PhyFluidBatch *phy_fluid_batch_create(const PhySphParams *params, uint32_t n_scenes);
+void phy_fluid_batch_destroy(PhyFluidBatch *b);
+uint32_t phy_fluid_batch_scenes(PhyFluidBatch *b);
+int32_t phy_fluid_batch_set_params(PhyFluidBatch *b, uint32_t scene, const PhySphParams *params);
+int32_t phy_fluid_batch_get_params(PhyFluidBatch *b, uint32_t scene, PhySphParams *out);
+
+/* Replace every scene's particles: counts[n_scenes], then FP32 arrays
+ * concatenated in scene order (pos3 / vel3: 3 per particle; mass, material: 1;
+ * vel3, mass, material may be NULL: zero, SphParams::mass, 0). 0 ok, -1. */
+int32_t phy_fluid_batch_set_particles_f32(PhyFluidBatch *b, const uint32_t *counts,
+                                          const float *pos3, const float *vel3,
+                                          const float *mass, const uint32_t *material);
+/* offsets: n_scenes + 1 entries (first particle of each scene, then the total). */
+int32_t phy_fluid_batch_offsets(PhyFluidBatch *b, uint32_t *offsets);
+
+/* n steps of dt for every scene. 0 ok, -1 error. */
+int32_t phy_fluid_batch_step(PhyFluidBatch *b, double dt, uint32_t n);
+/* One step, then the watchdog of every scene (FluidSubsystem::validate,
+ * subsystem.rs:74-90): status[s] = 0 ok, 1 non-finite position, 2 non-finite
+ * velocity. Returns the number of failing scenes, -1 on error. */
+int32_t phy_fluid_batch_step_checked(PhyFluidBatch *b, double dt, int32_t *status);
+int32_t phy_fluid_batch_set_graph(PhyFluidBatch *b, int32_t on);
+int32_t phy_fluid_batch_set_keep_aux(PhyFluidBatch *b, int32_t on);
+int32_t phy_fluid_batch_sync(PhyFluidBatch *b);
+
+/* PHY_FIELD_* of every particle in concatenated index order; returns the
+ * particles written (whole particles that fit in len), 0 on error. */
+uintptr_t phy_fluid_batch_get_field_f32(PhyFluidBatch *b, int32_t field, float *buf, uintptr_t len);
+/* out: 5 per scene (mass, momentum x y z, kinetic energy), FP64 on device. */
+int32_t phy_fluid_batch_diagnostics_f64(PhyFluidBatch *b, double *out);
+/* As phy_fluid_neighbor_stats, concatenated; neighbour indices are scene-local. */
+int32_t phy_fluid_batch_neighbor_stats(PhyFluidBatch *b, uint32_t *counts, uint64_t *sums,
+                                       uint64_t *xors, uintptr_t len);
+
+/* Adjoint tape over every scene (as phy_fluid_tape_*, FP32 cotangents in
+ * concatenated order); dparams: 2 per scene (dL/d stiffness, dL/d visc_k[0]). */
+int32_t phy_fluid_batch_tape_begin(PhyFluidBatch *b, uint32_t checkpoint_every);
+uint32_t phy_fluid_batch_tape_steps(PhyFluidBatch *b);
+int32_t phy_fluid_batch_tape_backward(PhyFluidBatch *b, const float *xbar_final,
+                                      const float *vbar_final, const float *rhobar_last,
+                                      float *xbar0, float *vbar0, double *dparams);
+int32_t phy_fluid_batch_tape_end(PhyFluidBatch *b);
+
+/* As phy_fluid_profile_* (profiled steps run eagerly, not from the graph). */
+int32_t phy_fluid_batch_profile_begin(PhyFluidBatch *b);
+int32_t phy_fluid_batch_profile_end(PhyFluidBatch *b, double *ms4, uint32_t *steps);
+
 #ifdef __cplusplus
 }
 #endif

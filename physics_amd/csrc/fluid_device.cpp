@@ -86,25 +86,52 @@ DevParams make_dev_params(const HostParams& hp) {
     return P;
 }
 
-DeviceFluid::DeviceFluid(const HostParams& hp) : hp_(hp) {
+// Largest concatenated cell table of one world or batch (cells). When the
+// scenes' capacities add up to more, they are scaled down: particles beyond a
+// scene's dense domain go to its overflow segment (exact, but slower).
+constexpr unsigned long long kMaxTableCells = 1ull << 28;
+
+static unsigned long long box_cells(const long long* lo, const long long* hi) {
+    double v = 1.0;
+    for (int a = 0; a < 3; ++a) v *= (double)((unsigned long long)hi[a] - (unsigned long long)lo[a]) + 1.0;
+    return v >= (double)kMaxDenseCells ? (unsigned long long)kMaxDenseCells : (unsigned long long)v;
+}
+
+DeviceFluid::DeviceFluid(const HostParams& hp) : DeviceFluid(std::vector<HostParams>{hp}) {}
+
+DeviceFluid::DeviceFluid(const std::vector<HostParams>& scenes) {
+    if (scenes.empty()) throw std::invalid_argument("phy-fluid: a batch needs at least one scene");
+    if (scenes.size() > (1u << 24)) throw std::invalid_argument("phy-fluid: at most 2^24 scenes");
+    std::vector<DevParams> ps;
+    for (const HostParams& hp : scenes) ps.push_back(make_dev_params(hp));
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
     if (e != hipSuccess || count <= 0)
         throw HipError("phy-fluid: no HIP device available (this library runs on gfx950 only)");
     hip_check(hipGetDevice(&dev_), "hipGetDevice");
     hip_check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate");
-    P_ = make_dev_params(hp_);
-    dom_.ensure(1);
-    diag_part_.ensure((size_t)kDiagBlocks * 5);
-    diag_out_.ensure(5);
-    partial_.ensure(kScanBlocks);
-    for (int a = 0; a < 3; ++a) {
-        bkey_[a] = host_key(P_.lo[a], P_.inv_cell);
-        bkey_[3 + a] = host_key(P_.hi[a], P_.inv_cell);
+    hps_ = scenes;
+    Ps_ = ps;
+    const size_t S = scenes.size();
+    scn_.assign(S, SceneInfo{});
+    for (size_t s = 0; s < S; ++s) {
+        for (int a = 0; a < 3; ++a) {
+            scn_[s].bkey[a] = host_key(Ps_[s].lo[a], Ps_[s].inv_cell);
+            scn_[s].bkey[3 + a] = host_key(Ps_[s].hi[a], Ps_[s].inv_cell);
+        }
     }
+    dom_.ensure(S);
+    table_total_.ensure(1);
+    partial_.ensure(kScanBlocks);
+    diag_out_.ensure(5 * S);
+    std::vector<uint32_t> zero(S, 0u);
+    set_layout(zero.data());
+    size_tables(nullptr);
+    push_params();
 }
 
 DeviceFluid::~DeviceFluid() {
+    drop_graph();
     for (hipEvent_t e : prof_ev_) (void)hipEventDestroy(e);
     if (st_) {
         (void)hipStreamSynchronize(st_);
@@ -112,112 +139,142 @@ DeviceFluid::~DeviceFluid() {
     }
 }
 
-void DeviceFluid::set_params(const HostParams& hp) {
-    P_ = make_dev_params(hp);
-    hp_ = hp;
-    for (int a = 0; a < 3; ++a) {
-        bkey_[a] = host_key(P_.lo[a], P_.inv_cell);
-        bkey_[3 + a] = host_key(P_.hi[a], P_.inv_cell);
+void DeviceFluid::sync() { hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize"); }
+
+// Scene ranges and the tile -> scene map for the given particle counts.
+void DeviceFluid::set_layout(const uint32_t* counts) {
+    uint64_t off = 0;
+    uint32_t t = 0;
+    tile_scene_.clear();
+    for (size_t s = 0; s < scn_.size(); ++s) {
+        const uint32_t n = counts[s];
+        const uint32_t nt = (uint32_t)((n + (uint64_t)kTile - 1) / kTile);
+        scn_[s].off = (uint32_t)off;
+        scn_[s].n = n;
+        scn_[s].tile0 = t;
+        scn_[s].ntiles = nt;
+        tile_scene_.insert(tile_scene_.end(), nt, (uint32_t)s);
+        off += n;
+        t += nt;
+        if (off >= 0x7fffffffull) throw std::invalid_argument("phy-fluid: at most 2^31-1 particles");
     }
-    // the dense table may need to grow for a new bounds box
-    double vol = 1.0;
-    for (int a = 0; a < 3; ++a)
-        vol *= (double)((unsigned long long)bkey_[3 + a] - (unsigned long long)bkey_[a]) + 1.0;
-    unsigned long long need = vol >= (double)kMaxDenseCells ? kMaxDenseCells : (unsigned long long)vol;
-    if (need > cell_cap_) {
-        cell_cap_ = std::min<unsigned long long>(need, kMaxDenseCells);
+    n_ = (size_t)off;
+    ntiles_ = (int)t;
+    d_tiles_.ensure(tile_scene_.size() ? tile_scene_.size() : 1);
+    if (!tile_scene_.empty())
+        hip_check(hipMemcpyAsync(d_tiles_.get(), tile_scene_.data(), tile_scene_.size() * sizeof(uint32_t),
+                                 hipMemcpyHostToDevice, st_), "upload tiles");
+    sync();
+    graph_dirty_ = true;
+}
+
+// Dense table capacity of every scene: the cells of its bounds key box (every
+// finite particle is inside it after one step) and of its particles' key box
+// (pos3: concatenated positions, or null to keep the current capacities as a
+// floor). Grows the concatenated table when needed.
+void DeviceFluid::size_tables(const float* pos3) {
+    const size_t S = scn_.size();
+    std::vector<unsigned long long> caps(S);
+    double total = 0.0;
+    for (size_t s = 0; s < S; ++s) {
+        SceneInfo& si = scn_[s];
+        unsigned long long need = box_cells(si.bkey, si.bkey + 3);
+        if (pos3) {
+            long long plo[3] = {LLONG_MAX, LLONG_MAX, LLONG_MAX}, phi[3] = {LLONG_MIN, LLONG_MIN, LLONG_MIN};
+            bool any = false;
+            for (uint32_t k = 0; k < si.n; ++k) {
+                const float* p = pos3 + 3 * ((size_t)si.off + k);
+                if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) continue;
+                any = true;
+                for (int a = 0; a < 3; ++a) {
+                    long long key = host_key(p[a], Ps_[s].inv_cell);
+                    plo[a] = std::min(plo[a], key);
+                    phi[a] = std::max(phi[a], key);
+                }
+            }
+            if (any) need = std::max(need, box_cells(plo, phi));
+        } else {
+            need = std::max<unsigned long long>(need, si.cap);
+        }
+        caps[s] = std::min<unsigned long long>(std::max<unsigned long long>(need, 1), kMaxDenseCells);
+        total += (double)caps[s] + 1.0;
+    }
+    if (total > (double)kMaxTableCells) {
+        const double f = ((double)kMaxTableCells - (double)S) / (total - (double)S);
+        for (size_t s = 0; s < S; ++s)
+            caps[s] = std::max<unsigned long long>(1, (unsigned long long)((double)caps[s] * f));
+    }
+    unsigned long long need_table = 0;
+    for (size_t s = 0; s < S; ++s) {
+        scn_[s].cap = caps[s];
+        need_table += caps[s] + 1;
+    }
+    if (need_table > table_cap_ || !counts_.get()) {
+        table_cap_ = std::max(need_table, table_cap_);
         counts_.release();
         start_.release();
-        counts_.ensure(cell_cap_ + 1);
-        start_.ensure(cell_cap_ + 2);
-        hip_check(hipMemsetAsync(counts_.get(), 0, (cell_cap_ + 1) * sizeof(uint32_t), st_),
-                  "memset counts");
+        counts_.ensure(table_cap_);
+        start_.ensure(table_cap_ + 1);
+        hip_check(hipMemsetAsync(counts_.get(), 0, table_cap_ * sizeof(uint32_t), st_), "memset counts");
+        sync();
+        graph_dirty_ = true;
     }
+}
+
+void DeviceFluid::push_params() {
+    const size_t S = scn_.size();
+    any_shear_ = false;
+    for (const DevParams& P : Ps_) any_shear_ = any_shear_ || !P.newtonian;
+    d_params_.ensure(S);
+    d_info_.ensure(S);
+    hip_check(hipMemcpyAsync(d_params_.get(), Ps_.data(), S * sizeof(DevParams), hipMemcpyHostToDevice, st_), "params");
+    hip_check(hipMemcpyAsync(d_info_.get(), scn_.data(), S * sizeof(SceneInfo), hipMemcpyHostToDevice, st_), "scenes");
+    sync();
+    graph_dirty_ = true;
+}
+
+void DeviceFluid::set_scene_params(size_t s, const HostParams& hp) {
+    if (s >= scn_.size()) throw std::out_of_range("phy-fluid: scene index");
+    DevParams P = make_dev_params(hp);
+    Ps_[s] = P;
+    hps_[s] = hp;
+    for (int a = 0; a < 3; ++a) {
+        scn_[s].bkey[a] = host_key(P.lo[a], P.inv_cell);
+        scn_[s].bkey[3 + a] = host_key(P.hi[a], P.inv_cell);
+    }
+    size_tables(nullptr);
+    push_params();
     dens_valid_ = aux_valid_ = false;
     tape_invalidate();
 }
 
-void DeviceFluid::sync() { hip_check(hipStreamSynchronize(st_), "hipStreamSynchronize"); }
-
-void DeviceFluid::ensure_capacity(size_t n) {
-    if (n <= cap_n_) return;
-    size_t c = std::max<size_t>(n, 1);
-    posm_.ensure(c); vel_.ensure(c); id_.ensure(c); mat_.ensure(c); body_.ensure(c);
-    posm_s_.ensure(c); velm_s_.ensure(c); body_s_.ensure(c);
-    id_s_.ensure(c); mat_s_.ensure(c); cell_s_.ensure(c);
-    rho_.ensure(c); prs_.ensure(c); mp_.ensure(c);
-    cell_.ensure(c); rank_.ensure(c); tmp_.ensure(c); idtmp_.ensure(c);
-    stage4_.ensure(c); stage1_.ensure(c);
-    if (keep_aux_) { acc_.ensure(c); xsph_.ensure(c); mu_.ensure(c); }
-    cap_n_ = c;
+void DeviceFluid::set_params(const HostParams& hp) {
+    make_dev_params(hp);  // validate before changing anything
+    for (size_t s = 0; s < scn_.size(); ++s) set_scene_params(s, hp);
 }
 
-// Dense table capacity: cells of the bounds key box (every finite particle is
-// inside it after one step) and of the uploaded particles' key box.
-void DeviceFluid::compute_cell_cap(const float* pos3, size_t n) {
-    long long lo[3], hi[3];
-    for (int a = 0; a < 3; ++a) {
-        lo[a] = bkey_[a];
-        hi[a] = bkey_[3 + a];
+void DeviceFluid::ensure_capacity(size_t n) {
+    if (n > cap_n_) {
+        size_t c = std::max<size_t>(n, 1);
+        posm_.ensure(c); vel_.ensure(c); id_.ensure(c); mat_.ensure(c); body_.ensure(c);
+        posm_s_.ensure(c); velm_s_.ensure(c); body_s_.ensure(c);
+        id_s_.ensure(c); mat_s_.ensure(c); cell_s_.ensure(c);
+        rho_.ensure(c); prs_.ensure(c); mp_.ensure(c);
+        cell_.ensure(c); rank_.ensure(c); tmp_.ensure(c); idtmp_.ensure(c);
+        stage4_.ensure(c); stage1_.ensure(c);
+        if (keep_aux_) { acc_.ensure(c); xsph_.ensure(c); mu_.ensure(c); }
+        cap_n_ = c;
+        graph_dirty_ = true;
     }
-    unsigned long long need_b = 1, need_p = 1;
-    {
-        double v = 1.0;
-        for (int a = 0; a < 3; ++a)
-            v *= (double)((unsigned long long)hi[a] - (unsigned long long)lo[a]) + 1.0;
-        need_b = v >= (double)kMaxDenseCells ? kMaxDenseCells : (unsigned long long)v;
-    }
-    long long plo[3] = {LLONG_MAX, LLONG_MAX, LLONG_MAX}, phi[3] = {LLONG_MIN, LLONG_MIN, LLONG_MIN};
-    bool any = false;
-    for (size_t i = 0; i < n; ++i) {
-        const float* p = pos3 + 3 * i;
-        if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) continue;
-        any = true;
-        for (int a = 0; a < 3; ++a) {
-            long long k = host_key(p[a], P_.inv_cell);
-            plo[a] = std::min(plo[a], k);
-            phi[a] = std::max(phi[a], k);
-        }
-    }
-    if (any) {
-        double v = 1.0;
-        for (int a = 0; a < 3; ++a) v *= (double)((unsigned long long)phi[a] - (unsigned long long)plo[a]) + 1.0;
-        need_p = v >= (double)kMaxDenseCells ? kMaxDenseCells : (unsigned long long)v;
-    }
-    unsigned long long need = std::max(need_b, need_p);
-    need = std::min<unsigned long long>(std::max<unsigned long long>(need, 1), kMaxDenseCells);
-    if (need > cell_cap_ || !counts_.get()) {
-        cell_cap_ = std::max(need, cell_cap_);
-        counts_.release();
-        start_.release();
-        counts_.ensure(cell_cap_ + 1);
-        start_.ensure(cell_cap_ + 2);
-        hip_check(hipMemsetAsync(counts_.get(), 0, (cell_cap_ + 1) * sizeof(uint32_t), st_),
-                  "memset counts");
-    }
+    diag_part_.ensure((size_t)std::max(ntiles_, 1) * 5);
 }
 
 void DeviceFluid::upload(size_t n, const float* pos3, const float* vel3, const float* mass,
                          const uint32_t* material, const float* body3) {
+    if (scn_.size() != 1) throw std::logic_error("phy-fluid: upload() needs a one-scene world");
     if (n >= 0x7fffffffull) throw std::invalid_argument("phy-fluid: at most 2^31-1 particles");
-    ensure_capacity(n);
-    compute_cell_cap(pos3, n);
-    std::vector<float4> p4(n), v4(n);
-    std::vector<uint32_t> ids(n), mats(n);
-    for (size_t i = 0; i < n; ++i) {
-        p4[i] = make_float4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], mass ? mass[i] : (float)hp_.mass);
-        v4[i] = vel3 ? make_float4(vel3[3 * i], vel3[3 * i + 1], vel3[3 * i + 2], 0.f)
-                     : make_float4(0.f, 0.f, 0.f, 0.f);
-        ids[i] = (uint32_t)i;
-        mats[i] = material ? material[i] : 0u;
-    }
-    n_ = n;
-    if (n) {
-        hip_check(hipMemcpyAsync(posm_.get(), p4.data(), n * sizeof(float4), hipMemcpyHostToDevice, st_), "upload pos");
-        hip_check(hipMemcpyAsync(vel_.get(), v4.data(), n * sizeof(float4), hipMemcpyHostToDevice, st_), "upload vel");
-        hip_check(hipMemcpyAsync(id_.get(), ids.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st_), "upload id");
-        hip_check(hipMemcpyAsync(mat_.get(), mats.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st_), "upload mat");
-    }
+    const uint32_t c = (uint32_t)n;
+    upload_scenes(&c, pos3, vel3, mass, material);
     has_body_ = false;
     if (body3 && n) {
         std::vector<float4> b4(n);
@@ -225,12 +282,47 @@ void DeviceFluid::upload(size_t n, const float* pos3, const float* vel3, const f
             b4[i] = make_float4(body3[3 * i], body3[3 * i + 1], body3[3 * i + 2], 0.f);
             if (b4[i].x != 0.f || b4[i].y != 0.f || b4[i].z != 0.f) has_body_ = true;
         }
-        if (has_body_)
+        if (has_body_) {
             hip_check(hipMemcpyAsync(body_.get(), b4.data(), n * sizeof(float4), hipMemcpyHostToDevice, st_), "upload body");
+            sync();
+        }
     }
+}
+
+void DeviceFluid::upload_scenes(const uint32_t* counts, const float* pos3, const float* vel3,
+                                const float* mass, const uint32_t* material) {
+    uint64_t total = 0;
+    for (size_t s = 0; s < scn_.size(); ++s) total += counts[s];
+    if (total >= 0x7fffffffull) throw std::invalid_argument("phy-fluid: at most 2^31-1 particles");
+    set_layout(counts);
+    ensure_capacity(n_);
+    size_tables(pos3);
+    push_params();
+    const size_t n = n_;
+    std::vector<float4> p4(n), v4(n);
+    std::vector<uint32_t> ids(n), mats(n);
+    for (size_t s = 0; s < scn_.size(); ++s) {
+        const float m0 = (float)hps_[s].mass;
+        for (uint32_t k = 0; k < scn_[s].n; ++k) {
+            const size_t i = (size_t)scn_[s].off + k;
+            p4[i] = make_float4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], mass ? mass[i] : m0);
+            v4[i] = vel3 ? make_float4(vel3[3 * i], vel3[3 * i + 1], vel3[3 * i + 2], 0.f)
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
+            ids[i] = (uint32_t)i;
+            mats[i] = material ? material[i] : 0u;
+        }
+    }
+    if (n) {
+        hip_check(hipMemcpyAsync(posm_.get(), p4.data(), n * sizeof(float4), hipMemcpyHostToDevice, st_), "upload pos");
+        hip_check(hipMemcpyAsync(vel_.get(), v4.data(), n * sizeof(float4), hipMemcpyHostToDevice, st_), "upload vel");
+        hip_check(hipMemcpyAsync(id_.get(), ids.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st_), "upload id");
+        hip_check(hipMemcpyAsync(mat_.get(), mats.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice, st_), "upload mat");
+    }
+    has_body_ = false;
     sync();  // host vectors go out of scope
     dens_valid_ = aux_valid_ = false;
     tape_invalidate();
+    graph_dirty_ = true;
 }
 
 void DeviceFluid::set_keep_aux(bool on) {
@@ -241,6 +333,20 @@ void DeviceFluid::set_keep_aux(bool on) {
         mu_.ensure(cap_n_);
     }
     if (!on) aux_valid_ = false;
+    graph_dirty_ = true;
+}
+
+SceneTable DeviceFluid::table(Domain* dom) const {
+    SceneTable T;
+    T.info = d_info_.get();
+    T.tile_scene = d_tiles_.get();
+    T.params = d_params_.get();
+    T.dom = dom;
+    T.table_total = table_total_.get();
+    T.nscenes = (int)scn_.size();
+    T.ntiles = ntiles_;
+    T.n = (int)n_;
+    return T;
 }
 
 DeviceFluid::StepBufs DeviceFluid::own_bufs() {
@@ -264,13 +370,13 @@ DeviceFluid::StepBufs DeviceFluid::own_bufs() {
 }
 
 void DeviceFluid::run_domain_and_sort(const StepBufs& s) {
-    const int n = (int)n_;
-    launch_domain(st_, n, posm_.get(), P_.inv_cell, s.dom, cell_cap_, bkey_);
+    const SceneTable T = table(s.dom);
+    launch_domain(st_, T, posm_.get());
     SortBuffers b;
     b.posm_in = posm_.get();
     b.vel_in = vel_.get();
     b.id_in = id_.get();
-    b.mat_in = P_.n_mat > 1 ? mat_.get() : nullptr;
+    b.mat_in = mat_.get();
     b.body_in = has_body_ ? body_.get() : nullptr;
     b.cell = cell_.get();
     b.rank = rank_.get();
@@ -285,10 +391,8 @@ void DeviceFluid::run_domain_and_sort(const StepBufs& s) {
     b.mat_out = s.mat_s;
     b.body_out = s.body_s;
     b.cell_out = s.cell_s;
-    launch_sort(st_, n, b, s.dom, P_.inv_cell);
+    launch_sort(st_, T, b);
 }
-
-void DeviceFluid::run_domain_and_sort() { run_domain_and_sort(own_bufs()); }
 
 hipEvent_t DeviceFluid::ev() {
     hipEvent_t e;
@@ -326,14 +430,14 @@ uint32_t DeviceFluid::profile_end(double ms4[4]) {
     return steps;
 }
 
-// One step into the given buffer set; the new state lands in posm_ / vel_ in
-// this step's cell order.
+// One step into the given buffer set; the new state (x, v, id, material)
+// lands in posm_ / vel_ / id_ / mat_ in this step's cell order.
 void DeviceFluid::step_core(float dt, const StepBufs& s) {
-    const int n = (int)n_;
     if (prof_) ev();
     run_domain_and_sort(s);
     if (prof_) ev();
-    launch_density(st_, n, s.posm_s, s.cell_s, s.start, s.dom, P_, s.rho, s.prs, s.velm_s, s.mp);
+    const SceneTable T = table(s.dom);
+    launch_density(st_, T, s.posm_s, s.cell_s, s.start, s.rho, s.prs, s.velm_s, s.mp);
     ForceArgs a;
     a.posm = s.posm_s;
     a.velm = s.velm_s;
@@ -343,27 +447,62 @@ void DeviceFluid::step_core(float dt, const StepBufs& s) {
     a.cell = s.cell_s;
     a.start = s.start;
     a.id = s.id_s;
-    a.mat = P_.n_mat > 1 ? s.mat_s : nullptr;
+    a.mat = s.mat_s;
     a.body = has_body_ ? s.body_s : nullptr;
     a.posm_out = posm_.get();
     a.vel_out = vel_.get();
+    a.id_out = id_.get();
+    a.mat_out = mat_.get();
     a.acc_out = s.acc;
     a.mu_out = s.mu;
     a.xsph_out = s.xsph;
     a.fvisc_out = s.fvisc;
     if (prof_) ev();
-    launch_force(st_, n, dt, a, s.dom, P_);
+    launch_force(st_, T, dt, a, any_shear_);
     if (prof_) ev();
     hip_check(hipGetLastError(), "step launch");
-    has_body_ = false;  // integrate zeroes body_acc (world.rs:361)
+}
+
+void DeviceFluid::drop_graph() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+}
+
+// Records the whole step (domain, sort, density, force: no memsets, no host
+// synchronisation) into one graph.
+void DeviceFluid::capture(float dt) {
+    drop_graph();
+    hip_check(hipStreamBeginCapture(st_, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    try {
+        step_core(dt, own_bufs());
+    } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(st_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    hip_check(hipStreamEndCapture(st_, &graph_), "hipStreamEndCapture");
+    hip_check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0), "hipGraphInstantiate");
+    graph_dt_ = dt;
+    graph_dirty_ = false;
+}
+
+void DeviceFluid::set_graph(bool on) {
+    graph_on_ = on;
+    if (!on) drop_graph();
 }
 
 void DeviceFluid::step(float dt) {
     if (n_ == 0) return;  // world.rs:184-186
-    step_core(dt, own_bufs());
-    // the new state is in cell order: ids and materials follow the sort
-    id_.swap(id_s_);
-    mat_.swap(mat_s_);
+    if (graph_on_ && !prof_ && !has_body_) {
+        if (graph_dirty_ || !graph_exec_ || !(graph_dt_ == dt)) capture(dt);
+        hip_check(hipGraphLaunch(graph_exec_, st_), "hipGraphLaunch");
+    } else {
+        step_core(dt, own_bufs());
+    }
+    has_body_ = false;  // integrate zeroes body_acc (world.rs:361)
     dens_valid_ = true;
     aux_valid_ = keep_aux_;
     if (taping_) tape_record(dt);
@@ -395,8 +534,9 @@ void DeviceFluid::load_state(const Snapshot& c) {
 
 void DeviceFluid::tape_begin(uint32_t K) {
     if (K == 0) throw std::invalid_argument("tape: checkpoint interval must be >= 1");
-    if (!P_.newtonian || P_.n_mat != 1)
-        throw std::invalid_argument("tape: the adjoint supports one Newtonian material (visc_n = [1])");
+    for (const DevParams& P : Ps_)
+        if (!P.newtonian || P.n_mat != 1)
+            throw std::invalid_argument("tape: the adjoint supports one Newtonian material (visc_n = [1])");
     if (has_body_) throw std::invalid_argument("tape: body_acc is not differentiated");
     tape_end();
     taping_ = true;
@@ -434,8 +574,8 @@ DeviceFluid::StepBufs DeviceFluid::slot_bufs(TapeSlot& t) {
     t.id_s.ensure(n); t.mat_s.ensure(n); t.cell_s.ensure(n);
     t.rho.ensure(n); t.prs.ensure(n); t.mp.ensure(n);
     t.acc.ensure(n); t.xsph.ensure(n); t.fvisc.ensure(n); t.mu.ensure(n);
-    t.start.ensure(cell_cap_ + 2);
-    t.dom.ensure(1);
+    t.start.ensure(table_cap_ + 1);
+    t.dom.ensure(scn_.size());
     StepBufs s;
     s.posm_s = t.posm_s.get(); s.velm_s = t.velm_s.get(); s.body_s = t.body_s.get();
     s.id_s = t.id_s.get(); s.mat_s = t.mat_s.get(); s.cell_s = t.cell_s.get();
@@ -451,7 +591,8 @@ void DeviceFluid::tape_backward(const float* xbar_T, const float* vbar_T, const 
     if (tape_broken_) throw std::logic_error("tape: particles were replaced while recording");
     const size_t T = tape_dt_.size();
     const int n = (int)n_;
-    *dk = *dmu = 0.0;
+    const size_t S = scn_.size();
+    for (size_t s = 0; s < S; ++s) dk[s] = dmu[s] = 0.0;
     if (T == 0 || n == 0) {
         for (size_t k = 0; k < 3 * n_; ++k) {
             if (xbar0) xbar0[k] = xbar_T ? xbar_T[k] : 0.f;
@@ -459,11 +600,11 @@ void DeviceFluid::tape_backward(const float* xbar_T, const float* vbar_T, const 
         }
         return;
     }
-    const int nb = (n + 255) / 256;
+    const size_t nt = (size_t)std::max(ntiles_, 1);
     for (int b = 0; b < 2; ++b) { xbar_[b].ensure(n); vbar_[b].ensure(n); }
     Fb_.ensure(n); Sb_.ensure(n); xb_loc_.ensure(n); vb_loc_.ensure(n); xb_acc_.ensure(n);
     rhob_loc_.ensure(n); rhob_.ensure(n); seed_.ensure(n);
-    adj_mu_.ensure(nb); adj_k_.ensure(nb); adj_tot_.ensure(2);
+    adj_mu_.ensure(nt); adj_k_.ensure(nt); adj_tot_.ensure(2 * S);
     {
         std::vector<float4> xs(n), vs(n);
         for (int i = 0; i < n; ++i) {
@@ -472,17 +613,16 @@ void DeviceFluid::tape_backward(const float* xbar_T, const float* vbar_T, const 
             vs[i] = vbar_T ? make_float4(vbar_T[3 * i], vbar_T[3 * i + 1], vbar_T[3 * i + 2], 0.f)
                            : make_float4(0.f, 0.f, 0.f, 0.f);
         }
+        std::vector<double> z(2 * S, 0.0);
         hip_check(hipMemcpyAsync(xbar_[0].get(), xs.data(), n * sizeof(float4), hipMemcpyHostToDevice, st_), "adj");
         hip_check(hipMemcpyAsync(vbar_[0].get(), vs.data(), n * sizeof(float4), hipMemcpyHostToDevice, st_), "adj");
         if (rhobar_T)
             hip_check(hipMemcpyAsync(seed_.get(), rhobar_T, n * sizeof(float), hipMemcpyHostToDevice, st_), "adj");
-        double z[2] = {0.0, 0.0};
-        hip_check(hipMemcpyAsync(adj_tot_.get(), z, sizeof(z), hipMemcpyHostToDevice, st_), "adj");
+        hip_check(hipMemcpyAsync(adj_tot_.get(), z.data(), 2 * S * sizeof(double), hipMemcpyHostToDevice, st_), "adj");
         sync();
     }
     Snapshot final_state;
     save_state(final_state);
-    const bool keep_aux = keep_aux_;
     const bool prof = prof_;
     prof_ = false;
     int cur = 0;
@@ -495,46 +635,43 @@ void DeviceFluid::tape_backward(const float* xbar_T, const float* vbar_T, const 
     for (size_t seg = nseg; seg-- > 0;) {
         load_state(*ckpt_[seg]);
         const size_t s0 = seg * K, s1 = std::min(T, s0 + K);
-        for (size_t t = s0; t < s1; ++t) {
-            StepBufs sb = slot_bufs(*slots_[t - s0]);
-            step_core(tape_dt_[t], sb);
-            hip_check(hipMemcpyAsync(id_.get(), sb.id_s, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st_), "ids");
-            hip_check(hipMemcpyAsync(mat_.get(), sb.mat_s, n * sizeof(uint32_t), hipMemcpyDeviceToDevice, st_), "mats");
-        }
+        for (size_t t = s0; t < s1; ++t) step_core(tape_dt_[t], slot_bufs(*slots_[t - s0]));
         for (size_t t = s1; t-- > s0;) {
             TapeSlot& sl = *slots_[t - s0];
-            AdjSlot S;
-            S.posm = sl.posm_s.get(); S.velm = sl.velm_s.get(); S.rho = sl.rho.get(); S.prs = sl.prs.get();
-            S.cell = sl.cell_s.get(); S.start = sl.start.get(); S.id = sl.id_s.get(); S.dom = sl.dom.get();
-            S.acc = sl.acc.get(); S.xsph = sl.xsph.get(); S.fvisc = sl.fvisc.get(); S.dt = tape_dt_[t];
+            AdjSlot A;
+            A.posm = sl.posm_s.get(); A.velm = sl.velm_s.get(); A.rho = sl.rho.get(); A.prs = sl.prs.get();
+            A.cell = sl.cell_s.get(); A.start = sl.start.get(); A.id = sl.id_s.get();
+            A.acc = sl.acc.get(); A.xsph = sl.xsph.get(); A.fvisc = sl.fvisc.get(); A.dt = tape_dt_[t];
             AdjBuffers B;
             B.xbar_in = xbar_[cur].get(); B.vbar_in = vbar_[cur].get();
             B.Fb = Fb_.get(); B.Sb = Sb_.get(); B.xb_loc = xb_loc_.get(); B.vb_loc = vb_loc_.get();
             B.rhob_loc = rhob_loc_.get(); B.rhob = rhob_.get(); B.xb_acc = xb_acc_.get();
             B.xbar_out = xbar_[1 - cur].get(); B.vbar_out = vbar_[1 - cur].get();
             B.mu_part = adj_mu_.get(); B.k_part = adj_k_.get(); B.totals = adj_tot_.get();
-            launch_adjoint_step(st_, n, S, P_, B, (t == T - 1 && rhobar_T) ? seed_.get() : nullptr);
+            launch_adjoint_step(st_, table(sl.dom.get()), A, B,
+                                (t == T - 1 && rhobar_T) ? seed_.get() : nullptr);
             hip_check(hipGetLastError(), "adjoint launch");
             cur = 1 - cur;
         }
     }
     std::vector<float4> xs(n), vs(n);
-    double tot[2];
+    std::vector<double> tot(2 * S);
     hip_check(hipMemcpyAsync(xs.data(), xbar_[cur].get(), n * sizeof(float4), hipMemcpyDeviceToHost, st_), "adj out");
     hip_check(hipMemcpyAsync(vs.data(), vbar_[cur].get(), n * sizeof(float4), hipMemcpyDeviceToHost, st_), "adj out");
-    hip_check(hipMemcpyAsync(tot, adj_tot_.get(), sizeof(tot), hipMemcpyDeviceToHost, st_), "adj out");
+    hip_check(hipMemcpyAsync(tot.data(), adj_tot_.get(), 2 * S * sizeof(double), hipMemcpyDeviceToHost, st_), "adj out");
     load_state(final_state);
     sync();
     for (int i = 0; i < n; ++i) {
         if (xbar0) { xbar0[3 * i] = xs[i].x; xbar0[3 * i + 1] = xs[i].y; xbar0[3 * i + 2] = xs[i].z; }
         if (vbar0) { vbar0[3 * i] = vs[i].x; vbar0[3 * i + 1] = vs[i].y; vbar0[3 * i + 2] = vs[i].z; }
     }
-    *dk = tot[0];
-    *dmu = tot[1];
-    keep_aux_ = keep_aux;
+    for (size_t s = 0; s < S; ++s) {
+        dk[s] = tot[2 * s];
+        dmu[s] = tot[2 * s + 1];
+    }
     prof_ = prof;
-    // the recomputation overwrote the last step's rho / p in the own buffers
-    // only if they were shared; the own buffers are untouched by slot steps.
+    // the slot steps wrote only slot buffers and the state, which is restored;
+    // rho / p / aux of the last forward step are untouched
 }
 
 void DeviceFluid::download(Field f, float* out) {
@@ -587,36 +724,56 @@ void DeviceFluid::download(Field f, float* out) {
     throw std::invalid_argument("phy-fluid: unknown field");
 }
 
-int DeviceFluid::last_validate() {
-    if (n_ == 0) return 0;
-    Domain d;
-    hip_check(hipMemcpyAsync(&d, dom_.get(), sizeof(Domain), hipMemcpyDeviceToHost, st_), "watchdog");
+void DeviceFluid::last_validate_all(int32_t* status) {
+    const size_t S = scn_.size();
+    if (n_ == 0) {
+        for (size_t s = 0; s < S; ++s) status[s] = 0;
+        return;
+    }
+    std::vector<Domain> d(S);
+    hip_check(hipMemcpyAsync(d.data(), dom_.get(), S * sizeof(Domain), hipMemcpyDeviceToHost, st_), "watchdog");
     sync();
-    if (d.bad_pos == 0xffffffffu && d.bad_vel == 0xffffffffu) return 0;
-    // first particle in index order with a bad field; pos is checked before vel
-    return d.bad_pos <= d.bad_vel ? 1 : 2;
+    for (size_t s = 0; s < S; ++s) {
+        // first particle in index order with a bad field; pos is checked before vel
+        if (d[s].bad_pos == 0xffffffffu && d[s].bad_vel == 0xffffffffu)
+            status[s] = 0;
+        else
+            status[s] = d[s].bad_pos <= d[s].bad_vel ? 1 : 2;
+    }
 }
 
-void DeviceFluid::diagnostics(double out5[5]) {
-    for (int k = 0; k < 5; ++k) out5[k] = 0.0;
+int DeviceFluid::last_validate() {
+    std::vector<int32_t> st(scn_.size());
+    last_validate_all(st.data());
+    return st[0];
+}
+
+void DeviceFluid::diagnostics(double* out5) {
+    const size_t S = scn_.size();
+    for (size_t k = 0; k < 5 * S; ++k) out5[k] = 0.0;
     if (n_ == 0) return;
-    launch_diag(st_, (int)n_, posm_.get(), vel_.get(), diag_part_.get(), diag_out_.get());
-    hip_check(hipMemcpyAsync(out5, diag_out_.get(), 5 * sizeof(double), hipMemcpyDeviceToHost, st_), "diag");
+    launch_diag(st_, table(dom_.get()), posm_.get(), vel_.get(), diag_part_.get(), diag_out_.get());
+    hip_check(hipMemcpyAsync(out5, diag_out_.get(), 5 * S * sizeof(double), hipMemcpyDeviceToHost, st_), "diag");
     sync();
 }
 
 void DeviceFluid::neighbor_stats(uint32_t* cnt, uint64_t* sum, uint64_t* xr) {
     const int n = (int)n_;
     if (n == 0) return;
-    // sort the current state (the same binning as a step), then count
-    run_domain_and_sort();
+    // sort the current state (the same binning as a step) with a domain array
+    // of its own, so the last step's watchdog survives, then count
+    DevBuf<Domain> dom;
+    dom.ensure(scn_.size());
+    StepBufs sb = own_bufs();
+    sb.dom = dom.get();
+    run_domain_and_sort(sb);
     DevBuf<uint32_t> c;
     DevBuf<unsigned long long> s, x;
     c.ensure(n);
     s.ensure(n);
     x.ensure(n);
-    launch_neighbor_stats(st_, n, posm_s_.get(), cell_s_.get(), start_.get(), id_s_.get(), dom_.get(),
-                          P_, c.get(), s.get(), x.get());
+    launch_neighbor_stats(st_, table(sb.dom), posm_s_.get(), cell_s_.get(), start_.get(), id_s_.get(),
+                          c.get(), s.get(), x.get());
     hip_check(hipMemcpyAsync(cnt, c.get(), n * sizeof(uint32_t), hipMemcpyDeviceToHost, st_), "nstats");
     hip_check(hipMemcpyAsync(sum, s.get(), n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_), "nstats");
     hip_check(hipMemcpyAsync(xr, x.get(), n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_), "nstats");

@@ -85,28 +85,49 @@ enum Field : int {
     F_MU_EFF = 7,
 };
 
+// One or more independent scenes on one GPU. A FluidWorld is a batch of one
+// scene; a batch concatenates its scenes' particles (scene s owns the index
+// range [offset(s), offset(s) + count(s))) and steps all of them with one
+// launch per stage. Each scene keeps its own parameters, bounds, dense cell
+// table and watchdog.
 class DeviceFluid {
    public:
     explicit DeviceFluid(const HostParams& hp);
+    explicit DeviceFluid(const std::vector<HostParams>& scenes);
     ~DeviceFluid();
     DeviceFluid(const DeviceFluid&) = delete;
     DeviceFluid& operator=(const DeviceFluid&) = delete;
 
+    size_t scenes() const { return hps_.size(); }
+    // set_params replaces the parameters of every scene
     void set_params(const HostParams& hp);
-    const HostParams& params() const { return hp_; }
+    void set_scene_params(size_t s, const HostParams& hp);
+    const HostParams& params(size_t s = 0) const { return hps_.at(s); }
     size_t size() const { return n_; }
+    uint32_t offset(size_t s) const { return scn_.at(s).off; }
+    uint32_t count(size_t s) const { return scn_.at(s).n; }
 
-    // Replace all particles (original index order). Null vel / material /
-    // body_acc mean zero / 0 / zero. Positions and velocities are rounded to FP32.
+    // Replace all particles of a one-scene world (original index order). Null
+    // vel / material / body_acc mean zero / 0 / zero. Positions and velocities
+    // are rounded to FP32.
     void upload(size_t n, const float* pos3, const float* vel3, const float* mass,
                 const uint32_t* material, const float* body3);
+    // Replace the particles of every scene: counts[scenes()], then the
+    // scenes' particles concatenated in scene order (null vel / material as above).
+    void upload_scenes(const uint32_t* counts, const float* pos3, const float* vel3,
+                       const float* mass, const uint32_t* material);
 
-    // Advance one step of dt (FP32 on the device).
+    // Advance every scene one step of dt (FP32 on the device).
     void step(float dt);
+    // While on, step() replays one captured hipGraph of the whole step
+    // (recaptured when dt, the particle set, the parameters or the buffers
+    // change). Profiling and steps that consume body_acc run eagerly.
+    void set_graph(bool on);
+    bool graph() const { return graph_on_; }
 
-    // Copy a field to host memory in original index order (3 floats per
-    // particle for vector fields, 1 otherwise). Auxiliary fields (acc, xsph,
-    // mu_eff) hold the last step's values only while keep_aux is on.
+    // Copy a field to host memory in (concatenated) original index order (3
+    // floats per particle for vector fields, 1 otherwise). Auxiliary fields
+    // (acc, xsph, mu_eff) hold the last step's values only while keep_aux is on.
     void download(Field f, float* out);
 
     void set_keep_aux(bool on);
@@ -114,12 +135,15 @@ class DeviceFluid {
     bool aux_valid() const { return aux_valid_; }
     bool density_valid() const { return dens_valid_; }
 
-    // Watchdog result of the last step (subsystem.rs:74-90): 0 ok, 1 pos, 2 vel.
+    // Watchdog of the last step (subsystem.rs:74-90): 0 ok, 1 pos, 2 vel, for
+    // scene 0 / for every scene.
     int last_validate();
-    // FP64 mass, momentum (3) and kinetic energy of the current state.
-    void diagnostics(double out5[5]);
+    void last_validate_all(int32_t* status);
+    // FP64 mass, momentum (3) and kinetic energy of the current state, 5 per scene.
+    void diagnostics(double* out5);
     // Neighbour statistics of the density predicate on the current state
-    // (original order): count, sum of neighbour indices, xor of hashed indices.
+    // (original order): count, sum of neighbour indices, xor of hashed
+    // indices; indices are scene-local.
     void neighbor_stats(uint32_t* cnt, uint64_t* sum, uint64_t* xr);
 
     hipStream_t stream() const { return st_; }
@@ -130,9 +154,10 @@ class DeviceFluid {
     // checkpointed every K steps. tape_backward() takes the cotangents of the
     // final x, v (3 per particle) and of the last step's rho (1 per particle),
     // any of them null for zero, all in original index order, and returns the
-    // cotangents of the initial x, v and dL/d stiffness, dL/d visc_k[0]. Each
-    // segment is recomputed from its checkpoint into per-step slots, then the
-    // adjoint runs backwards through it. The world state is left unchanged.
+    // cotangents of the initial x, v and, per scene, dL/d stiffness and
+    // dL/d visc_k[0]. Each segment is recomputed from its checkpoint into
+    // per-step slots, then the adjoint runs backwards through it. The world
+    // state is left unchanged.
     void tape_begin(uint32_t K);
     void tape_backward(const float* xbar_T, const float* vbar_T, const float* rhobar_T,
                        float* xbar0, float* vbar0, double* dk, double* dmu);
@@ -168,29 +193,40 @@ class DeviceFluid {
         DevBuf<float> rho, prs, mp, mu;
         DevBuf<Domain> dom;
     };
+    void init_stream();
     void ensure_capacity(size_t n);
-    void compute_cell_cap(const float* pos3, size_t n);
+    void set_layout(const uint32_t* counts);
+    void size_tables(const float* pos3);
+    void push_params();
+    SceneTable table(Domain* dom) const;
     StepBufs own_bufs();
     StepBufs slot_bufs(TapeSlot& t);
     void run_domain_and_sort(const StepBufs& s);
-    void run_domain_and_sort();
     void step_core(float dt, const StepBufs& s);
+    void capture(float dt);
+    void drop_graph();
     void save_state(Snapshot& c);
     void load_state(const Snapshot& c);
     void tape_record(float dt);
     void tape_invalidate();
 
-    HostParams hp_;
-    DevParams P_;
+    std::vector<HostParams> hps_;
+    std::vector<DevParams> Ps_;
+    std::vector<SceneInfo> scn_;
+    std::vector<uint32_t> tile_scene_;
+    int ntiles_ = 0;
+    bool any_shear_ = false;
     int dev_ = 0;
     hipStream_t st_ = nullptr;
     size_t n_ = 0;
     size_t cap_n_ = 0;
-    unsigned long long cell_cap_ = 1;
-    long long bkey_[6] = {0, 0, 0, 0, 0, 0};
+    unsigned long long table_cap_ = 0;  // cells of the concatenated table (incl. overflow cells)
     bool keep_aux_ = false, aux_valid_ = false, dens_valid_ = false;
     bool has_body_ = false;
 
+    DevBuf<DevParams> d_params_;
+    DevBuf<SceneInfo> d_info_;
+    DevBuf<uint32_t> d_tiles_, table_total_;
     // current state (cell order after the first step)
     DevBuf<float4> posm_, vel_;
     DevBuf<uint32_t> id_, mat_;
@@ -209,6 +245,11 @@ class DeviceFluid {
     DevBuf<float> stage1_;
     DevBuf<Domain> dom_;
     DevBuf<double> diag_part_, diag_out_;
+
+    bool graph_on_ = false, graph_dirty_ = true;
+    float graph_dt_ = 0.f;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
 
     bool prof_ = false;
     std::vector<hipEvent_t> prof_ev_;  // 4 per profiled step

@@ -11,6 +11,7 @@
 
 #include "world.h"
 
+using phy::FluidBatch;
 using phy::FluidSubsystem;
 using phy::FluidWorld;
 using phy::World;
@@ -34,6 +35,11 @@ R guard(R fail, F&& f) {
 }
 
 World* as_world(PhyWorldHandle* w) { return reinterpret_cast<World*>(w); }
+FluidBatch* as_batch(PhyFluidBatch* b) { return reinterpret_cast<FluidBatch*>(b); }
+FluidBatch* need_batch(PhyFluidBatch* b) {
+    if (!b) throw std::invalid_argument("null batch handle");
+    return as_batch(b);
+}
 PhyWorldHandle* box_world(World* w) { return reinterpret_cast<PhyWorldHandle*>(w); }
 
 // First fluid subsystem of the world, or null.
@@ -488,6 +494,200 @@ int32_t phy_fluid_profile_end(PhyWorldHandle* w, double* ms4, uint32_t* steps) {
         if (!ms4) throw std::invalid_argument("null out");
         uint32_t s = need_fluid(w)->world().device().profile_end(ms4);
         if (steps) *steps = s;
+        return 0;
+    });
+}
+
+int32_t phy_fluid_set_graph(PhyWorldHandle* w, int32_t on) {
+    return guard<int32_t>(-1, [&] {
+        need_fluid(w)->world().device().set_graph(on != 0);
+        return 0;
+    });
+}
+
+// ---------------------------------------------------------------------------
+// batches
+
+PhyFluidBatch* phy_fluid_batch_create(const PhySphParams* params, uint32_t n_scenes) {
+    return guard<PhyFluidBatch*>(nullptr, [&] {
+        if (!params || n_scenes == 0) throw std::invalid_argument("batch: need n_scenes >= 1 parameter sets");
+        std::vector<phy::SphParams> ps;
+        for (uint32_t s = 0; s < n_scenes; ++s) ps.push_back(from_c(params[s]));
+        return reinterpret_cast<PhyFluidBatch*>(new FluidBatch(ps));
+    });
+}
+
+void phy_fluid_batch_destroy(PhyFluidBatch* b) {
+    guard<int>(0, [&] {
+        delete as_batch(b);
+        return 0;
+    });
+}
+
+uint32_t phy_fluid_batch_scenes(PhyFluidBatch* b) {
+    return guard<uint32_t>(0, [&] { return (uint32_t)need_batch(b)->scenes(); });
+}
+
+int32_t phy_fluid_batch_set_params(PhyFluidBatch* b, uint32_t scene, const PhySphParams* params) {
+    return guard<int32_t>(-1, [&] {
+        if (!params) throw std::invalid_argument("null params");
+        need_batch(b)->set_params(scene, from_c(*params));
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_get_params(PhyFluidBatch* b, uint32_t scene, PhySphParams* out) {
+    return guard<int32_t>(-1, [&] {
+        if (!out) throw std::invalid_argument("null out");
+        to_c(need_batch(b)->params(scene), out);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_set_particles_f32(PhyFluidBatch* b, const uint32_t* counts, const float* pos3,
+                                          const float* vel3, const float* mass,
+                                          const uint32_t* material) {
+    return guard<int32_t>(-1, [&] {
+        FluidBatch* B = need_batch(b);
+        if (!counts) throw std::invalid_argument("null counts");
+        uint64_t n = 0;
+        for (size_t s = 0; s < B->scenes(); ++s) n += counts[s];
+        if (n && !pos3) throw std::invalid_argument("null positions");
+        B->set_particles_f32(counts, pos3, vel3, mass, material);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_offsets(PhyFluidBatch* b, uint32_t* offsets) {
+    return guard<int32_t>(-1, [&] {
+        FluidBatch* B = need_batch(b);
+        if (!offsets) throw std::invalid_argument("null out");
+        for (size_t s = 0; s < B->scenes(); ++s) offsets[s] = B->device().offset(s);
+        offsets[B->scenes()] = (uint32_t)B->device().size();
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_step(PhyFluidBatch* b, double dt, uint32_t n) {
+    return guard<int32_t>(-1, [&] {
+        FluidBatch* B = need_batch(b);
+        for (uint32_t i = 0; i < n; ++i) B->step(dt);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_step_checked(PhyFluidBatch* b, double dt, int32_t* status) {
+    return guard<int32_t>(-1, [&] {
+        FluidBatch* B = need_batch(b);
+        if (!status) throw std::invalid_argument("null status");
+        B->step_checked(dt, status);
+        int32_t bad = 0;
+        for (size_t s = 0; s < B->scenes(); ++s) bad += status[s] != 0;
+        return bad;
+    });
+}
+
+int32_t phy_fluid_batch_set_graph(PhyFluidBatch* b, int32_t on) {
+    return guard<int32_t>(-1, [&] {
+        need_batch(b)->device().set_graph(on != 0);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_set_keep_aux(PhyFluidBatch* b, int32_t on) {
+    return guard<int32_t>(-1, [&] {
+        need_batch(b)->device().set_keep_aux(on != 0);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_sync(PhyFluidBatch* b) {
+    return guard<int32_t>(-1, [&] {
+        need_batch(b)->device().sync();
+        return 0;
+    });
+}
+
+uintptr_t phy_fluid_batch_get_field_f32(PhyFluidBatch* b, int32_t field, float* buf, uintptr_t len) {
+    return guard<uintptr_t>(0, [&] {
+        int c = field_comps(field);
+        if (!c) throw std::invalid_argument("unknown field");
+        phy::DeviceFluid& d = need_batch(b)->device();
+        size_t n = d.size();
+        if (!buf || !n) return (uintptr_t)0;
+        std::vector<float> tmp(n * c);
+        d.download((phy::Field)field, tmp.data());
+        uintptr_t np = std::min<uintptr_t>(n, len / c);
+        std::memcpy(buf, tmp.data(), np * c * sizeof(float));
+        return np;
+    });
+}
+
+int32_t phy_fluid_batch_diagnostics_f64(PhyFluidBatch* b, double* out) {
+    return guard<int32_t>(-1, [&] {
+        if (!out) throw std::invalid_argument("null out");
+        need_batch(b)->device().diagnostics(out);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_neighbor_stats(PhyFluidBatch* b, uint32_t* counts, uint64_t* sums,
+                                       uint64_t* xors, uintptr_t len) {
+    return guard<int32_t>(-1, [&] {
+        phy::DeviceFluid& d = need_batch(b)->device();
+        if (!counts || !sums || !xors || len < d.size()) throw std::invalid_argument("buffers too short");
+        d.neighbor_stats(counts, sums, xors);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_tape_begin(PhyFluidBatch* b, uint32_t checkpoint_every) {
+    return guard<int32_t>(-1, [&] {
+        need_batch(b)->device().tape_begin(checkpoint_every);
+        return 0;
+    });
+}
+
+uint32_t phy_fluid_batch_tape_steps(PhyFluidBatch* b) {
+    return guard<uint32_t>(0, [&] { return (uint32_t)need_batch(b)->device().tape_steps(); });
+}
+
+int32_t phy_fluid_batch_tape_backward(PhyFluidBatch* b, const float* xbar_final, const float* vbar_final,
+                                      const float* rhobar_last, float* xbar0, float* vbar0,
+                                      double* dparams) {
+    return guard<int32_t>(-1, [&] {
+        phy::DeviceFluid& d = need_batch(b)->device();
+        const size_t S = d.scenes();
+        std::vector<double> dk(S), dmu(S);
+        d.tape_backward(xbar_final, vbar_final, rhobar_last, xbar0, vbar0, dk.data(), dmu.data());
+        if (dparams)
+            for (size_t s = 0; s < S; ++s) {
+                dparams[2 * s] = dk[s];
+                dparams[2 * s + 1] = dmu[s];
+            }
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_tape_end(PhyFluidBatch* b) {
+    return guard<int32_t>(-1, [&] {
+        need_batch(b)->device().tape_end();
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_profile_begin(PhyFluidBatch* b) {
+    return guard<int32_t>(-1, [&] {
+        need_batch(b)->device().profile_begin();
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_profile_end(PhyFluidBatch* b, double* ms4, uint32_t* steps) {
+    return guard<int32_t>(-1, [&] {
+        if (!ms4) throw std::invalid_argument("null out");
+        uint32_t n = need_batch(b)->device().profile_end(ms4);
+        if (steps) *steps = n;
         return 0;
     });
 }

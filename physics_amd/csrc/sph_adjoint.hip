@@ -43,15 +43,17 @@ __device__ __forceinline__ double block_sum_f64(double v, double* sh) {
 }
 
 // Integrate + bounds backwards (world.rs:352-397) and the acc = F / rho split.
-__global__ __launch_bounds__(256) void k_adj_local(int n, AdjSlot S, DevParams P,
+__global__ __launch_bounds__(256) void k_adj_local(SceneTable T, AdjSlot S,
                                                    const float4* __restrict__ xbar_in,
                                                    const float4* __restrict__ vbar_in, float4* Fb,
                                                    float4* Sb, float4* xb_loc, float4* vb_loc,
                                                    float* rhob_loc, double* mu_part) {
     __shared__ double sh[4];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const TileLoc TL = tile_loc(T);
+    const int i = TL.i;
+    const DevParams P = T.params[TL.s];
     double mu_b = 0.0;
-    if (i < n) {
+    if (TL.act) {
         const uint32_t id = S.id[i];
         const float4 xo = xbar_in[id], vo = vbar_in[id];
         const float4 xs = S.xsph[i];
@@ -86,7 +88,7 @@ __global__ __launch_bounds__(256) void k_adj_local(int n, AdjSlot S, DevParams P
         rhob_loc[i] = rb;
     }
     mu_b = block_sum_f64(mu_b, sh);
-    if (threadIdx.x == 0) mu_part[blockIdx.x] = mu_b;
+    if (threadIdx.x == 0) mu_part[TL.t] = mu_b;
 }
 
 struct AdjNbr {
@@ -104,7 +106,7 @@ struct ForceAdjAcc {
 };
 
 template <int L>
-__global__ __launch_bounds__(256) void k_adj_force(int n, AdjSlot S, DevParams P,
+__global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
                                                    const float4* __restrict__ Fb,
                                                    const float4* __restrict__ Sb,
                                                    const float4* __restrict__ xb_loc,
@@ -116,9 +118,12 @@ __global__ __launch_bounds__(256) void k_adj_force(int n, AdjSlot S, DevParams P
     __shared__ uint32_t lst[L][256];
     __shared__ double sh[4];
     const int tid = threadIdx.x;
-    const int a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + tid;
-    const bool act = a < n;
-    const Domain d = *S.dom;
+    const TileLoc TL = tile_loc(T);
+    const int a = TL.i;
+    const bool act = TL.act;
+    const Domain d = T.dom[TL.s];
+    const DevParams P = T.params[TL.s];
+    const uint32_t* __restrict__ start = S.start + d.toff;
     float4 pa = make_float4(0.f, 0.f, 0.f, 0.f), va = pa, Fa = pa, sa = pa;
     uint32_t ca = d.ncells;
     if (act) {
@@ -132,7 +137,7 @@ __global__ __launch_bounds__(256) void k_adj_force(int n, AdjSlot S, DevParams P
     const float h = P.h, h2 = P.h2;
     ForceAdjAcc A = {};
     gather<L>(
-        act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, S.start, S.posm, lst, tid,
+        act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, start, S.posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_accept(P, (uint32_t)a, pa.x, pa.y, pa.z, j, pj); },
         [&](uint32_t j) { return AdjNbr{S.posm[j], S.velm[j], Fb[j], Sb[j]}; },
         [&](const AdjNbr& nb, bool valid) {
@@ -222,19 +227,22 @@ __global__ __launch_bounds__(256) void k_adj_force(int n, AdjSlot S, DevParams P
         vbar_out[id] = make_float4(vb[0], vb[1], vb[2], 0.f);
     }
     kb = block_sum_f64(kb, sh);
-    if (tid == 0) k_part[blockIdx.x] = kb;
+    if (tid == 0) k_part[TL.t] = kb;
 }
 
 template <int L>
-__global__ __launch_bounds__(256) void k_adj_density(int n, AdjSlot S, DevParams P,
+__global__ __launch_bounds__(256) void k_adj_density(SceneTable T, AdjSlot S,
                                                      const float* __restrict__ rhob,
                                                      const float4* __restrict__ xb_acc,
                                                      float4* xbar_out) {
     __shared__ uint32_t lst[L][256];
     const int tid = threadIdx.x;
-    const int a = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + tid;
-    const bool act = a < n;
-    const Domain d = *S.dom;
+    const TileLoc TL = tile_loc(T);
+    const int a = TL.i;
+    const bool act = TL.act;
+    const Domain d = T.dom[TL.s];
+    const DevParams P = T.params[TL.s];
+    const uint32_t* __restrict__ start = S.start + d.toff;
     float4 pa = make_float4(0.f, 0.f, 0.f, 0.f);
     uint32_t ca = d.ncells;
     if (act) {
@@ -247,7 +255,7 @@ __global__ __launch_bounds__(256) void k_adj_density(int n, AdjSlot S, DevParams
         float rb;
     };
     gather<L>(
-        act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, S.start, S.posm, lst, tid,
+        act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, start, S.posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return density_accept(P, pa.x, pa.y, pa.z, pj); },
         [&](uint32_t j) { return DN{S.posm[j], rhob[j]}; },
         [&](const DN& nb, bool valid) {
@@ -272,32 +280,48 @@ __global__ __launch_bounds__(256) void k_adj_density(int n, AdjSlot S, DevParams
                                     xa.z - (ra * B1[2] + ma * B2[2]), 0.f);
 }
 
-// Adds the block partials of one step to the running FP64 totals in a fixed order.
-__global__ void k_adj_reduce(int nb, const double* __restrict__ mu_part,
-                             const double* __restrict__ k_part, double* totals) {
-    if (threadIdx.x != 0) return;
+// Adds one step's tile partials to the running FP64 totals of each scene
+// (one workgroup per scene, fixed summation shape).
+__global__ __launch_bounds__(256) void k_adj_reduce(SceneTable T, const double* __restrict__ mu_part,
+                                                    const double* __restrict__ k_part,
+                                                    double* totals) {
+    __shared__ double sh[2][256];
+    const SceneInfo si = T.info[blockIdx.x];
     double km = 0.0, kk = 0.0;
-    for (int b = 0; b < nb; ++b) {
-        km += mu_part[b];
-        kk += k_part[b];
+    for (uint32_t t = threadIdx.x; t < si.ntiles; t += 256) {
+        km += mu_part[si.tile0 + t];
+        kk += k_part[si.tile0 + t];
     }
-    totals[0] += kk;
-    totals[1] += km;
+    sh[0][threadIdx.x] = kk;
+    sh[1][threadIdx.x] = km;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
+            sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        totals[2 * blockIdx.x] += sh[0][0];
+        totals[2 * blockIdx.x + 1] += sh[1][0];
+    }
 }
 
-static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
-
-void launch_adjoint_step(hipStream_t st, int n, const AdjSlot& S, const DevParams& P,
-                         const AdjBuffers& B, const float* rhob_seed) {
-    const int nb = nblk(n, 256);
-    hipLaunchKernelGGL(k_adj_local, dim3(nb), dim3(256), 0, st, n, S, P, B.xbar_in, B.vbar_in, B.Fb,
-                       B.Sb, B.xb_loc, B.vb_loc, B.rhob_loc, B.mu_part);
-    hipLaunchKernelGGL(k_adj_force<kForceList>, dim3(nb), dim3(256), 0, st, n, S, P, B.Fb, B.Sb,
-                       B.xb_loc, B.vb_loc, B.rhob_loc, rhob_seed, B.rhob, B.xb_acc, B.vbar_out,
-                       B.k_part);
-    hipLaunchKernelGGL(k_adj_density<kDensityList>, dim3(nb), dim3(256), 0, st, n, S, P, B.rhob,
-                       B.xb_acc, B.xbar_out);
-    hipLaunchKernelGGL(k_adj_reduce, dim3(1), dim3(64), 0, st, nb, B.mu_part, B.k_part, B.totals);
+void launch_adjoint_step(hipStream_t st, const SceneTable& T, const AdjSlot& S, const AdjBuffers& B,
+                         const float* rhob_seed) {
+    if (T.ntiles > 0) {
+        const int nb = T.ntiles;
+        hipLaunchKernelGGL(k_adj_local, dim3(nb), dim3(256), 0, st, T, S, B.xbar_in, B.vbar_in, B.Fb,
+                           B.Sb, B.xb_loc, B.vb_loc, B.rhob_loc, B.mu_part);
+        hipLaunchKernelGGL(k_adj_force<kForceList>, dim3(nb), dim3(256), 0, st, T, S, B.Fb, B.Sb,
+                           B.xb_loc, B.vb_loc, B.rhob_loc, rhob_seed, B.rhob, B.xb_acc, B.vbar_out,
+                           B.k_part);
+        hipLaunchKernelGGL(k_adj_density<kDensityList>, dim3(nb), dim3(256), 0, st, T, S, B.rhob,
+                           B.xb_acc, B.xbar_out);
+    }
+    hipLaunchKernelGGL(k_adj_reduce, dim3(T.nscenes), dim3(256), 0, st, T, B.mu_part, B.k_part,
+                       B.totals);
 }
 
 }  // namespace phy

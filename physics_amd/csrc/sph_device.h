@@ -68,6 +68,27 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
     return x * per + min(x, rem) + k;
 }
 
+// This workgroup's tile, its scene and this lane's particle slot. The tile
+// index is XCD-remapped (xcd_block), so consecutive tiles of a scene share an
+// L2. Lanes past the end of the scene are inactive and must not touch i.
+struct TileLoc {
+    int t;       // tile
+    uint32_t s;  // scene
+    int i;       // particle slot in the concatenated arrays
+    bool act;
+};
+
+__device__ __forceinline__ TileLoc tile_loc(const SceneTable& T) {
+    TileLoc L;
+    L.t = xcd_block(blockIdx.x, gridDim.x);
+    L.s = T.tile_scene[L.t];
+    const SceneInfo* si = T.info + L.s;
+    const uint32_t local = (uint32_t)(L.t - (int)si->tile0) * (uint32_t)kTile + threadIdx.x;
+    L.act = local < si->n;
+    L.i = (int)(si->off + local);
+    return L;
+}
+
 __device__ __forceinline__ long long wave_min_ll(long long v) {
     for (int o = 32; o > 0; o >>= 1) {
         long long w = __shfl_xor(v, o, 64);

@@ -8,7 +8,6 @@
 namespace phy {
 
 constexpr int kScanBlocks = 1024;  // fixed grid of the three-phase cell scan
-constexpr int kDiagBlocks = 512;   // fixed grid of the FP64 diagnostics reduction
 constexpr int kForceList = 32;     // per-lane LDS neighbour list length of k_force
 constexpr int kDensityList = 32;   // per-lane LDS neighbour list length of k_density
 
@@ -47,8 +46,10 @@ struct ForceArgs {
     const uint32_t* id;
     const uint32_t* mat;      // may be null when one material
     const float4* body;       // may be null
-    float4* posm_out;
+    float4* posm_out;         // new state, in this step's cell order
     float4* vel_out;
+    uint32_t* id_out;         // original index of each new-state slot
+    uint32_t* mat_out;        // material of each new-state slot (with mat)
     float4* acc_out;          // optional (null: not stored)
     float* mu_out;
     float4* xsph_out;         // w: wall-clamp flags (bit per axis)
@@ -64,7 +65,6 @@ struct AdjSlot {
     const uint32_t* cell;
     const uint32_t* start;
     const uint32_t* id;
-    const Domain* dom;
     const float4* acc;
     const float4* xsph;       // w: wall-clamp flags
     const float4* fvisc;
@@ -83,29 +83,29 @@ struct AdjBuffers {
     float4* xb_acc;
     float4* xbar_out;         // cotangents of the step's input x, v (original order)
     float4* vbar_out;
-    double* mu_part;          // per-block FP64 partials
+    double* mu_part;          // per-tile FP64 partials
     double* k_part;
-    double* totals;           // running {dL/dk, dL/dmu}
+    double* totals;           // running {dL/dk, dL/dmu} per scene
 };
 
-void launch_adjoint_step(hipStream_t st, int n, const AdjSlot& S, const DevParams& P,
-                         const AdjBuffers& B, const float* rhob_seed);
+// T.dom must be the domain array of the recorded step.
+void launch_adjoint_step(hipStream_t st, const SceneTable& T, const AdjSlot& S, const AdjBuffers& B,
+                         const float* rhob_seed);
 
-void launch_domain(hipStream_t st, int n, const float4* posm, double inv, Domain* d,
-                   unsigned long long cap, const long long bk[6]);
-void launch_sort(hipStream_t st, int n, const SortBuffers& b, const Domain* d, double inv);
-void launch_density(hipStream_t st, int n, const float4* posm, const uint32_t* cellv,
-                    const uint32_t* start, const Domain* d, const DevParams& P, float* rho,
-                    float* prs, float4* velm, float* mp);
-void launch_force(hipStream_t st, int n, float dt, const ForceArgs& a, Domain* d,
-                  const DevParams& P);
-void launch_diag(hipStream_t st, int n, const float4* posm, const float4* vel, double* partial,
-                 double* out);
+// Every launch covers all scenes of T; T.dom / T.table_total belong to the
+// step's buffer set. Diagnostics write 5 doubles per scene; partial holds
+// 5 per tile.
+void launch_domain(hipStream_t st, const SceneTable& T, const float4* posm);
+void launch_sort(hipStream_t st, const SceneTable& T, const SortBuffers& b);
+void launch_density(hipStream_t st, const SceneTable& T, const float4* posm, const uint32_t* cellv,
+                    const uint32_t* start, float* rho, float* prs, float4* velm, float* mp);
+void launch_force(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a, bool shear);
+void launch_diag(hipStream_t st, const SceneTable& T, const float4* posm, const float4* vel,
+                 double* partial, double* out);
 void launch_unsort4(hipStream_t st, int n, const uint32_t* id, const float4* in, float4* out);
 void launch_unsort1(hipStream_t st, int n, const uint32_t* id, const float* in, float* out);
-void launch_neighbor_stats(hipStream_t st, int n, const float4* posm, const uint32_t* cellv,
-                           const uint32_t* start, const uint32_t* id, const Domain* d,
-                           const DevParams& P, uint32_t* cnt, unsigned long long* sum,
-                           unsigned long long* xr);
+void launch_neighbor_stats(hipStream_t st, const SceneTable& T, const float4* posm,
+                           const uint32_t* cellv, const uint32_t* start, const uint32_t* id,
+                           uint32_t* cnt, unsigned long long* sum, unsigned long long* xr);
 
 }  // namespace phy

@@ -16,6 +16,12 @@
 // Kahan's algorithm, so repeated runs are bit-identical. The only atomics are
 // integer ones (cell counts, min/max of keys, watchdog indices) whose results do
 // not depend on arrival order.
+//
+// Every kernel serves a batch of independent scenes in one launch: the
+// per-particle kernels run one workgroup per 256-particle tile and look up the
+// tile's scene (parameters, domain, cell-table offset) in a SceneTable; each
+// scene has its own dense cell table inside one concatenated table, scanned in
+// one pass. A single world is a batch of one scene.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <math.h>
@@ -29,7 +35,10 @@ namespace phy {
 // ---------------------------------------------------------------------------
 // domain
 
-__global__ void k_domain_reset(Domain* d) {
+__global__ __launch_bounds__(256) void k_domain_reset(Domain* dom, int nscenes) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nscenes) return;
+    Domain* d = dom + s;
     for (int a = 0; a < 3; ++a) {
         d->amin[a] = 0x7fffffffffffffffll;
         d->amax[a] = (long long)0x8000000000000000ull;
@@ -38,50 +47,80 @@ __global__ void k_domain_reset(Domain* d) {
     d->bad_vel = 0xffffffffu;
 }
 
-__global__ __launch_bounds__(256) void k_aabb(int n, const float4* __restrict__ posm, double inv,
-                                              Domain* d) {
+// Key AABB per scene. Each workgroup walks a contiguous range of tiles and
+// flushes its running min / max with one set of atomics per scene it touched.
+__global__ __launch_bounds__(256) void k_aabb(SceneTable T, const float4* __restrict__ posm) {
     __shared__ long long sh[6][4];
-    long long mn[3] = {0x7fffffffffffffffll, 0x7fffffffffffffffll, 0x7fffffffffffffffll};
-    long long mx[3] = {(long long)0x8000000000000000ull, (long long)0x8000000000000000ull,
-                       (long long)0x8000000000000000ull};
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-        float4 p = posm[s];
-        if (!finite3(p.x, p.y, p.z)) continue;
-        long long k[3] = {key1(p.x, inv), key1(p.y, inv), key1(p.z, inv)};
+    const int per = (T.ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int t0 = (int)blockIdx.x * per;
+    const int t1 = min(T.ntiles, t0 + per);
+    long long mn[3], mx[3];
+    auto reset = [&]() {
         for (int a = 0; a < 3; ++a) {
-            mn[a] = k[a] < mn[a] ? k[a] : mn[a];
-            mx[a] = k[a] > mx[a] ? k[a] : mx[a];
+            mn[a] = 0x7fffffffffffffffll;
+            mx[a] = (long long)0x8000000000000000ull;
         }
-    }
-    for (int a = 0; a < 3; ++a) {
-        mn[a] = wave_min_ll(mn[a]);
-        mx[a] = wave_max_ll(mx[a]);
-    }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0)
+    };
+    auto flush = [&](int s) {  // block-uniform: every thread calls it
         for (int a = 0; a < 3; ++a) {
-            sh[a][w] = mn[a];
-            sh[3 + a][w] = mx[a];
+            mn[a] = wave_min_ll(mn[a]);
+            mx[a] = wave_max_ll(mx[a]);
         }
-    __syncthreads();
-    if (threadIdx.x < 3) {
-        const int a = threadIdx.x;
-        long long lo = sh[a][0], hi = sh[3 + a][0];
-        for (int k = 1; k < 4; ++k) {
-            lo = sh[a][k] < lo ? sh[a][k] : lo;
-            hi = sh[3 + a][k] > hi ? sh[3 + a][k] : hi;
+        const int w = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0)
+            for (int a = 0; a < 3; ++a) {
+                sh[a][w] = mn[a];
+                sh[3 + a][w] = mx[a];
+            }
+        __syncthreads();
+        if (threadIdx.x < 3) {
+            const int a = threadIdx.x;
+            long long lo = sh[a][0], hi = sh[3 + a][0];
+            for (int k = 1; k < 4; ++k) {
+                lo = sh[a][k] < lo ? sh[a][k] : lo;
+                hi = sh[3 + a][k] > hi ? sh[3 + a][k] : hi;
+            }
+            if (lo <= hi) {
+                atomicMin(&T.dom[s].amin[a], lo);
+                atomicMax(&T.dom[s].amax[a], hi);
+            }
         }
-        atomicMin(&d->amin[a], lo);
-        atomicMax(&d->amax[a], hi);
+        __syncthreads();
+    };
+    reset();
+    int cur = -1;
+    double inv = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        const int s = (int)T.tile_scene[t];
+        if (s != cur) {
+            if (cur >= 0) flush(cur);
+            reset();
+            cur = s;
+            inv = T.params[s].inv_cell;
+        }
+        const SceneInfo* si = T.info + s;
+        const uint32_t local = (uint32_t)(t - (int)si->tile0) * (uint32_t)kTile + threadIdx.x;
+        if (local < si->n) {
+            const float4 p = posm[si->off + local];
+            if (finite3(p.x, p.y, p.z)) {
+                const long long k[3] = {key1(p.x, inv), key1(p.y, inv), key1(p.z, inv)};
+                for (int a = 0; a < 3; ++a) {
+                    mn[a] = k[a] < mn[a] ? k[a] : mn[a];
+                    mx[a] = k[a] > mx[a] ? k[a] : mx[a];
+                }
+            }
+        }
     }
+    if (cur >= 0) flush(cur);
 }
 
-// One thread: turn the key AABB into the dense table domain, clipped to the
-// table capacity. Particles outside the domain are binned to the overflow cell.
-__global__ void k_domain_finalize(Domain* d, unsigned long long cap, long long bk0, long long bk1,
-                                  long long bk2, long long bK0, long long bK1, long long bK2) {
+// Turns one scene's key AABB into its dense table domain, clipped to the
+// scene's table capacity. Particles outside the domain are binned to the
+// scene's overflow cell.
+__device__ void finalize_domain(Domain* d, unsigned long long cap, const long long* bk) {
     long long lo[3], hi[3];
-    long long blo[3] = {bk0, bk1, bk2}, bhi[3] = {bK0, bK1, bK2};
+    const long long* blo = bk;
+    const long long* bhi = bk + 3;
     bool any = d->amin[0] <= d->amax[0];
     for (int a = 0; a < 3; ++a) {
         lo[a] = any ? d->amin[a] : blo[a];
@@ -121,6 +160,36 @@ __global__ void k_domain_finalize(Domain* d, unsigned long long cap, long long b
     d->ncells = (uint32_t)nc;
 }
 
+// One workgroup: every scene's domain, then the scenes' table offsets as an
+// exclusive scan of (ncells + 1) in scene order, and the table's total size.
+__global__ __launch_bounds__(256) void k_domain_finalize(SceneTable T) {
+    __shared__ uint32_t sh[4];
+    uint32_t carry = 0;
+    const int tid = threadIdx.x;
+    for (int base = 0; base < T.nscenes; base += 256) {
+        const int s = base + tid;
+        uint32_t cells = 0;
+        if (s < T.nscenes) {
+            finalize_domain(T.dom + s, T.info[s].cap, T.info[s].bkey);
+            cells = T.dom[s].ncells + 1u;
+        }
+        uint32_t x = cells;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o, 64);
+            if ((tid & 63) >= o) x += y;
+        }
+        const int w = tid >> 6;
+        if ((tid & 63) == 63) sh[w] = x;
+        __syncthreads();
+        uint32_t before = 0;
+        for (int k = 0; k < w; ++k) before += sh[k];
+        if (s < T.nscenes) T.dom[s].toff = carry + before + x - cells;
+        carry += sh[0] + sh[1] + sh[2] + sh[3];
+        __syncthreads();
+    }
+    if (tid == 0) *T.table_total = carry;
+}
+
 // ---------------------------------------------------------------------------
 // counting sort
 
@@ -128,17 +197,18 @@ __global__ void k_domain_finalize(Domain* d, unsigned long long cap, long long b
 // of lanes: one atomic per run (wave-aggregated), each lane's arrival rank is
 // the run's base plus its position in the run. The arrival order is not
 // deterministic; k_reorder replaces it by the original-index order.
-__global__ __launch_bounds__(256) void k_bin(int n, const float4* __restrict__ posm, double inv,
-                                             const Domain* __restrict__ dom, uint32_t* cell,
-                                             uint32_t* rank, uint32_t* counts) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_bin(SceneTable T, const float4* __restrict__ posm,
+                                             uint32_t* cell, uint32_t* rank, uint32_t* counts) {
+    const TileLoc L = tile_loc(T);
+    const int s = L.i;
     const int lane = threadIdx.x & 63;
-    const bool valid = s < n;
-    const Domain d = *dom;
+    const bool valid = L.act;
+    const Domain d = T.dom[L.s];
+    const double inv = T.params[L.s].inv_cell;
     uint32_t c = 0xffffffffu;
     if (valid) {
         float4 p = posm[s];
-        c = cell_of(p.x, p.y, p.z, inv, d);
+        c = d.toff + cell_of(p.x, p.y, p.z, inv, d);
     }
     const uint32_t prev = __shfl_up(c, 1, 64);
     const bool head = lane == 0 || prev != c;
@@ -156,9 +226,9 @@ __global__ __launch_bounds__(256) void k_bin(int n, const float4* __restrict__ p
     }
 }
 
-// Three-phase exclusive scan over counts[0 .. ncells] (ncells + 1 entries, the
-// last one the overflow cell) with a fixed grid; ncells is read on the device.
-// Phase 3 also clears counts for the next step.
+// Three-phase exclusive scan over the concatenated cell table (every scene's
+// cells and overflow cell) with a fixed grid; the table size is read on the
+// device. Phase 3 also clears counts for the next step.
 __device__ __forceinline__ void chunk_of(uint32_t total, int b, int nb, uint32_t& lo, uint32_t& hi) {
     uint32_t per = (total + nb - 1) / nb;
     lo = min(total, per * (uint32_t)b);
@@ -177,10 +247,10 @@ __device__ __forceinline__ uint32_t block_sum_u32(uint32_t v, uint32_t* sh) {
 }
 
 __global__ __launch_bounds__(256) void k_scan_partial(const uint32_t* __restrict__ counts,
-                                                      const Domain* __restrict__ dom,
+                                                      const uint32_t* __restrict__ table_total,
                                                       uint32_t* partial) {
     __shared__ uint32_t sh[4];
-    uint32_t total = dom->ncells + 1;
+    uint32_t total = *table_total;
     uint32_t lo, hi;
     chunk_of(total, blockIdx.x, gridDim.x, lo, hi);
     uint32_t v = 0;
@@ -191,7 +261,8 @@ __global__ __launch_bounds__(256) void k_scan_partial(const uint32_t* __restrict
 
 // single block of 1024 threads; nb <= 1024
 __global__ __launch_bounds__(1024) void k_scan_top(uint32_t* partial, int nb,
-                                                   const Domain* __restrict__ dom, uint32_t* start) {
+                                                   const uint32_t* __restrict__ table_total,
+                                                   uint32_t* start) {
     __shared__ uint32_t sh[1024];
     int t = threadIdx.x;
     uint32_t v = t < nb ? partial[t] : 0;
@@ -204,14 +275,15 @@ __global__ __launch_bounds__(1024) void k_scan_top(uint32_t* partial, int nb,
         __syncthreads();
     }
     if (t < nb) partial[t] = sh[t] - v;  // exclusive
-    if (t == 1023) start[dom->ncells + 1] = sh[1023];
+    if (t == 1023) start[*table_total] = sh[1023];
 }
 
-__global__ __launch_bounds__(256) void k_scan_final(uint32_t* counts, const Domain* __restrict__ dom,
+__global__ __launch_bounds__(256) void k_scan_final(uint32_t* counts,
+                                                    const uint32_t* __restrict__ table_total,
                                                     const uint32_t* __restrict__ partial,
                                                     uint32_t* start) {
     __shared__ uint32_t sh[256];
-    uint32_t total = dom->ncells + 1;
+    uint32_t total = *table_total;
     uint32_t lo, hi;
     chunk_of(total, blockIdx.x, gridDim.x, lo, hi);
     uint32_t carry = partial[blockIdx.x];
@@ -256,15 +328,19 @@ __global__ __launch_bounds__(256) void k_scatter(int n, const uint32_t* __restri
 
 // Slot q of the scatter holds some member of its cell in arrival order; its
 // final slot is the cell start plus the number of members with a smaller
-// original index (ascending index inside a cell, grid.rs:37-38).
+// original index (ascending index inside a cell, grid.rs:37-38). Slots of a
+// scene stay inside the scene's particle range, so the tile names the scene;
+// the stored cell index is the scene-local one.
 __global__ __launch_bounds__(256) void k_reorder(
-    int n, const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ idtmp,
+    SceneTable T, const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ idtmp,
     const uint32_t* __restrict__ cell, const uint32_t* __restrict__ start,
     const float4* __restrict__ posm_in, const float4* __restrict__ vel_in,
     const uint32_t* __restrict__ mat_in, const float4* __restrict__ body_in, float4* posm_out,
     float4* velm_out, uint32_t* id_out, uint32_t* mat_out, float4* body_out, uint32_t* cell_out) {
-    int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
+    const TileLoc L = tile_loc(T);
+    if (!L.act) return;
+    const int q = L.i;
+    const uint32_t toff = T.dom[L.s].toff;
     uint32_t s = tmp[q];
     uint32_t my = idtmp[q];
     uint32_t c = cell[s];
@@ -277,7 +353,7 @@ __global__ __launch_bounds__(256) void k_reorder(
     v.w = 0.f;
     velm_out[dst] = v;
     id_out[dst] = my;
-    cell_out[dst] = c;
+    cell_out[dst] = c - toff;
     if (mat_in) mat_out[dst] = mat_in[s];
     if (body_in) body_out[dst] = body_in[s];
 }
@@ -291,17 +367,19 @@ __global__ __launch_bounds__(256) void k_reorder(
 // compaction limits it to true neighbours.
 
 template <int L>
-__global__ __launch_bounds__(256) void k_density(int n, const float4* __restrict__ posm,
+__global__ __launch_bounds__(256) void k_density(SceneTable T, const float4* __restrict__ posm,
                                                  const uint32_t* __restrict__ cellv,
-                                                 const uint32_t* __restrict__ start,
-                                                 const Domain* __restrict__ dom, DevParams P,
+                                                 const uint32_t* __restrict__ start_all,
                                                  float* rho_out, float* p_out, float4* velm,
                                                  float* mp_out) {
     __shared__ uint32_t lst[L][256];
     const int tid = threadIdx.x;
-    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + tid;
-    const bool act = i < n;
-    const Domain d = *dom;
+    const TileLoc TL = tile_loc(T);
+    const int i = TL.i;
+    const bool act = TL.act;
+    const Domain d = T.dom[TL.s];
+    const DevParams P = T.params[TL.s];
+    const uint32_t* __restrict__ start = start_all + d.toff;
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
     uint32_t ci = d.ncells;
     if (act) {
@@ -404,17 +482,20 @@ __device__ __forceinline__ void pair_force(ForceAcc<SHEAR>& A, const DevParams& 
 
 template <int L, bool SHEAR>
 __global__ __launch_bounds__(256) void k_force(
-    int n, float dt, const float4* __restrict__ posm, const float4* __restrict__ velm,
+    SceneTable T, float dt, const float4* __restrict__ posm, const float4* __restrict__ velm,
     const float* __restrict__ rho, const float* __restrict__ prs, const float* __restrict__ mp,
-    const uint32_t* __restrict__ cellv, const uint32_t* __restrict__ start,
+    const uint32_t* __restrict__ cellv, const uint32_t* __restrict__ start_all,
     const uint32_t* __restrict__ idv, const uint32_t* __restrict__ matv,
-    const float4* __restrict__ body, Domain* dom, DevParams P, float4* posm_out, float4* vel_out,
-    float4* acc_out, float* mu_out, float4* xsph_out, float4* fvisc_out) {
+    const float4* __restrict__ body, float4* posm_out, float4* vel_out, uint32_t* id_out,
+    uint32_t* mat_out, float4* acc_out, float* mu_out, float4* xsph_out, float4* fvisc_out) {
     __shared__ uint32_t lst[L][256];
     const int tid = threadIdx.x;
-    const int i = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + tid;
-    const bool act = i < n;
-    const Domain d = *dom;
+    const TileLoc TL = tile_loc(T);
+    const int i = TL.i;
+    const bool act = TL.act;
+    const Domain d = T.dom[TL.s];
+    const DevParams P = T.params[TL.s];
+    const uint32_t* __restrict__ start = start_all + d.toff;
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi;
     float rho_i = 1.f, Pi = 0.f;
     uint32_t ci = d.ncells;
@@ -439,18 +520,22 @@ __global__ __launch_bounds__(256) void k_force(
 
     // power-law effective viscosity (world.rs:319-327)
     int m = 0;
-    if (P.n_mat > 1) {
-        uint32_t mt = matv[i];
-        m = mt < (uint32_t)P.n_mat ? (int)mt : 0;
+    if (matv) {
+        const uint32_t mt = matv[i];
+        mat_out[i] = mt;  // the new state keeps this step's cell order
+        if (P.n_mat > 1) m = mt < (uint32_t)P.n_mat ? (int)mt : 0;
     }
-    float kc = P.visc_k[m];
+    // indexed through the table (a dynamic index into the register copy P
+    // would move that copy to scratch memory)
+    const DevParams* __restrict__ Pm = T.params + TL.s;
+    float kc = Pm->visc_k[m];
     float mu;
     if (!SHEAR) {
         mu = kc;  // powf(x, 0) == 1 for every x
     } else {
         float sh = A.s[9];
         float sreg = sh > P.shear_min ? sh : P.shear_min;
-        mu = kc * powf(sreg, P.visc_n[m] - 1.f);
+        mu = kc * powf(sreg, Pm->visc_n[m] - 1.f);
     }
     float acc[3] = {0.f, 0.f, 0.f}, fvis[3] = {0.f, 0.f, 0.f};
     if (rho_i > 0.f) {
@@ -498,48 +583,64 @@ __global__ __launch_bounds__(256) void k_force(
 
     // watchdog: smallest original index with a non-finite position / velocity
     uint32_t id = idv[i];
+    id_out[i] = id;
     uint32_t bp = finite3(x[0], x[1], x[2]) ? 0xffffffffu : id;
     uint32_t bv = finite3(v[0], v[1], v[2]) ? 0xffffffffu : id;
-    if (bp != 0xffffffffu) atomicMin(&dom->bad_pos, bp);
-    if (bv != 0xffffffffu) atomicMin(&dom->bad_vel, bv);
+    if (bp != 0xffffffffu) atomicMin(&T.dom[TL.s].bad_pos, bp);
+    if (bv != 0xffffffffu) atomicMin(&T.dom[TL.s].bad_vel, bv);
 }
 
 // ---------------------------------------------------------------------------
-// FP64 diagnostics (subsystem.rs:149-163): mass, momentum, kinetic energy.
-// Block partials in a fixed order, then one block sums the partials in order.
+// FP64 diagnostics (subsystem.rs:149-163): mass, momentum, kinetic energy per
+// scene. Tile partials by a fixed tree, then per scene a fixed-shape sum of its
+// tiles' partials, so the result does not depend on scheduling.
 
-__global__ __launch_bounds__(256) void k_diag_partial(int n, const float4* __restrict__ posm,
-                                                      const float4* __restrict__ vel,
-                                                      double* partial) {
-    __shared__ double sh[5][256];
-    double acc[5] = {0, 0, 0, 0, 0};
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-        double m = posm[s].w;
-        float4 v = vel[s];
-        double vx = v.x, vy = v.y, vz = v.z;
-        acc[0] += m;
-        acc[1] += vx * m;
-        acc[2] += vy * m;
-        acc[3] += vz * m;
-        acc[4] += (((vx * vx + vy * vy) + vz * vz) * m) / 2.0;
-    }
-    for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] = acc[k];
+__device__ __forceinline__ double block_tree_sum(double v, double* sh) {
+    sh[threadIdx.x] = v;
     __syncthreads();
     for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o)
-            for (int k = 0; k < 5; ++k) sh[k][threadIdx.x] += sh[k][threadIdx.x + o];
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0)
-        for (int k = 0; k < 5; ++k) partial[blockIdx.x * 5 + k] = sh[k][0];
+    const double r = sh[0];
+    __syncthreads();
+    return r;
 }
 
-__global__ void k_diag_final(int nb, const double* __restrict__ partial, double* out) {
-    if (threadIdx.x != 0) return;
+__global__ __launch_bounds__(256) void k_diag_partial(SceneTable T, const float4* __restrict__ posm,
+                                                      const float4* __restrict__ vel,
+                                                      double* partial) {
+    __shared__ double sh[256];
+    const TileLoc L = tile_loc(T);
     double acc[5] = {0, 0, 0, 0, 0};
-    for (int b = 0; b < nb; ++b)
-        for (int k = 0; k < 5; ++k) acc[k] += partial[b * 5 + k];
-    for (int k = 0; k < 5; ++k) out[k] = acc[k];
+    if (L.act) {
+        const double m = posm[L.i].w;
+        const float4 v = vel[L.i];
+        const double vx = v.x, vy = v.y, vz = v.z;
+        acc[0] = m;
+        acc[1] = vx * m;
+        acc[2] = vy * m;
+        acc[3] = vz * m;
+        acc[4] = (((vx * vx + vy * vy) + vz * vz) * m) / 2.0;
+    }
+    for (int k = 0; k < 5; ++k) {
+        const double t = block_tree_sum(acc[k], sh);
+        if (threadIdx.x == 0) partial[(size_t)L.t * 5 + k] = t;
+    }
+}
+
+// one workgroup per scene
+__global__ __launch_bounds__(256) void k_diag_final(SceneTable T, const double* __restrict__ partial,
+                                                    double* out) {
+    __shared__ double sh[256];
+    const SceneInfo si = T.info[blockIdx.x];
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (uint32_t t = threadIdx.x; t < si.ntiles; t += 256)
+        for (int k = 0; k < 5; ++k) acc[k] += partial[(size_t)(si.tile0 + t) * 5 + k];
+    for (int k = 0; k < 5; ++k) {
+        const double r = block_tree_sum(acc[k], sh);
+        if (threadIdx.x == 0) out[(size_t)blockIdx.x * 5 + k] = r;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -558,17 +659,22 @@ __global__ __launch_bounds__(256) void k_unsort1(int n, const uint32_t* __restri
 
 // neighbour statistics of the density predicate, for exact set comparison:
 // count, sum of original indices, xor of SplitMix64-hashed original indices.
-__global__ __launch_bounds__(256) void k_neighbor_stats(int n, const float4* __restrict__ posm,
+// Indices are scene-local (original index minus the scene's offset); the
+// outputs are written in concatenated original order.
+__global__ __launch_bounds__(256) void k_neighbor_stats(SceneTable T, const float4* __restrict__ posm,
                                                         const uint32_t* __restrict__ cellv,
-                                                        const uint32_t* __restrict__ start,
+                                                        const uint32_t* __restrict__ start_all,
                                                         const uint32_t* __restrict__ idv,
-                                                        const Domain* __restrict__ dom,
-                                                        DevParams P, uint32_t* cnt_out,
+                                                        uint32_t* cnt_out,
                                                         unsigned long long* sum_out,
                                                         unsigned long long* xor_out) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    Domain d = *dom;
+    const TileLoc L = tile_loc(T);
+    if (!L.act) return;
+    const int i = L.i;
+    const Domain d = T.dom[L.s];
+    const DevParams P = T.params[L.s];
+    const uint32_t* __restrict__ start = start_all + d.toff;
+    const uint32_t off = T.info[L.s].off;
     float4 pi = posm[i];
     uint32_t c = 0;
     unsigned long long s1 = 0, x1 = 0;
@@ -577,7 +683,7 @@ __global__ __launch_bounds__(256) void k_neighbor_stats(int n, const float4* __r
         float dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
         float s = dx * dx + dy * dy + dz * dz;
         if (classify(P, s, pi.x, pi.y, pi.z, pj.x, pj.y, pj.z) == 1) {
-            unsigned long long id = idv[j];
+            unsigned long long id = idv[j] - off;
             ++c;
             s1 += id;
             unsigned long long z = id + 0x9E3779B97F4A7C15ull;
@@ -597,56 +703,60 @@ __global__ __launch_bounds__(256) void k_neighbor_stats(int n, const float4* __r
 
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
-void launch_domain(hipStream_t st, int n, const float4* posm, double inv, Domain* d,
-                   unsigned long long cap, const long long bk[6]) {
-    hipLaunchKernelGGL(k_domain_reset, dim3(1), dim3(1), 0, st, d);
-    if (n > 0) {
-        int nb = nblk(n, 256);
-        if (nb > 512) nb = 512;
-        hipLaunchKernelGGL(k_aabb, dim3(nb), dim3(256), 0, st, n, posm, inv, d);
+void launch_domain(hipStream_t st, const SceneTable& T, const float4* posm) {
+    hipLaunchKernelGGL(k_domain_reset, dim3(nblk(T.nscenes, 256)), dim3(256), 0, st, T.dom, T.nscenes);
+    if (T.ntiles > 0) {
+        int nb = T.ntiles < 512 ? T.ntiles : 512;
+        hipLaunchKernelGGL(k_aabb, dim3(nb), dim3(256), 0, st, T, posm);
     }
-    hipLaunchKernelGGL(k_domain_finalize, dim3(1), dim3(1), 0, st, d, cap, bk[0], bk[1], bk[2],
-                       bk[3], bk[4], bk[5]);
+    hipLaunchKernelGGL(k_domain_finalize, dim3(1), dim3(256), 0, st, T);
 }
 
-void launch_sort(hipStream_t st, int n, const SortBuffers& b, const Domain* d, double inv) {
-    hipLaunchKernelGGL(k_bin, dim3(nblk(n, 256)), dim3(256), 0, st, n, b.posm_in, inv, d, b.cell,
-                       b.rank, b.counts);
-    hipLaunchKernelGGL(k_scan_partial, dim3(kScanBlocks), dim3(256), 0, st, b.counts, d, b.partial);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, b.partial, kScanBlocks, d, b.start);
-    hipLaunchKernelGGL(k_scan_final, dim3(kScanBlocks), dim3(256), 0, st, b.counts, d, b.partial,
+void launch_sort(hipStream_t st, const SceneTable& T, const SortBuffers& b) {
+    if (T.ntiles > 0)
+        hipLaunchKernelGGL(k_bin, dim3(T.ntiles), dim3(256), 0, st, T, b.posm_in, b.cell, b.rank,
+                           b.counts);
+    hipLaunchKernelGGL(k_scan_partial, dim3(kScanBlocks), dim3(256), 0, st, b.counts, T.table_total,
+                       b.partial);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, b.partial, kScanBlocks, T.table_total,
                        b.start);
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(n, 256)), dim3(256), 0, st, n, b.cell, b.rank, b.start,
-                       b.id_in, b.tmp, b.idtmp);
-    hipLaunchKernelGGL(k_reorder, dim3(nblk(n, 256)), dim3(256), 0, st, n, b.tmp, b.idtmp, b.cell,
-                       b.start, b.posm_in, b.vel_in, b.mat_in, b.body_in, b.posm_out, b.velm_out,
-                       b.id_out, b.mat_out, b.body_out, b.cell_out);
+    hipLaunchKernelGGL(k_scan_final, dim3(kScanBlocks), dim3(256), 0, st, b.counts, T.table_total,
+                       b.partial, b.start);
+    if (T.n > 0)
+        hipLaunchKernelGGL(k_scatter, dim3(nblk(T.n, 256)), dim3(256), 0, st, T.n, b.cell, b.rank,
+                           b.start, b.id_in, b.tmp, b.idtmp);
+    if (T.ntiles > 0)
+        hipLaunchKernelGGL(k_reorder, dim3(T.ntiles), dim3(256), 0, st, T, b.tmp, b.idtmp, b.cell,
+                           b.start, b.posm_in, b.vel_in, b.mat_in, b.body_in, b.posm_out, b.velm_out,
+                           b.id_out, b.mat_out, b.body_out, b.cell_out);
 }
 
-void launch_density(hipStream_t st, int n, const float4* posm, const uint32_t* cellv,
-                    const uint32_t* start, const Domain* d, const DevParams& P, float* rho,
-                    float* prs, float4* velm, float* mp) {
-    hipLaunchKernelGGL(k_density<kDensityList>, dim3(nblk(n, 256)), dim3(256), 0, st, n, posm, cellv,
-                       start, d, P, rho, prs, velm, mp);
+void launch_density(hipStream_t st, const SceneTable& T, const float4* posm, const uint32_t* cellv,
+                    const uint32_t* start, float* rho, float* prs, float4* velm, float* mp) {
+    if (T.ntiles == 0) return;
+    hipLaunchKernelGGL(k_density<kDensityList>, dim3(T.ntiles), dim3(256), 0, st, T, posm, cellv,
+                       start, rho, prs, velm, mp);
 }
 
-void launch_force(hipStream_t st, int n, float dt, const ForceArgs& a, Domain* d,
-                  const DevParams& P) {
-    if (P.newtonian)
-        hipLaunchKernelGGL((k_force<kForceList, false>), dim3(nblk(n, 256)), dim3(256), 0, st, n, dt,
-                           a.posm, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body, d,
-                           P, a.posm_out, a.vel_out, a.acc_out, a.mu_out, a.xsph_out, a.fvisc_out);
+void launch_force(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a, bool shear) {
+    if (T.ntiles == 0) return;
+    if (!shear)
+        hipLaunchKernelGGL((k_force<kForceList, false>), dim3(T.ntiles), dim3(256), 0, st, T, dt,
+                           a.posm, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body,
+                           a.posm_out, a.vel_out, a.id_out, a.mat_out, a.acc_out, a.mu_out,
+                           a.xsph_out, a.fvisc_out);
     else
-        hipLaunchKernelGGL((k_force<kForceList, true>), dim3(nblk(n, 256)), dim3(256), 0, st, n, dt,
-                           a.posm, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body, d,
-                           P, a.posm_out, a.vel_out, a.acc_out, a.mu_out, a.xsph_out, a.fvisc_out);
+        hipLaunchKernelGGL((k_force<kForceList, true>), dim3(T.ntiles), dim3(256), 0, st, T, dt,
+                           a.posm, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body,
+                           a.posm_out, a.vel_out, a.id_out, a.mat_out, a.acc_out, a.mu_out,
+                           a.xsph_out, a.fvisc_out);
 }
 
-void launch_diag(hipStream_t st, int n, const float4* posm, const float4* vel, double* partial,
-                 double* out) {
-    int nb = kDiagBlocks;
-    hipLaunchKernelGGL(k_diag_partial, dim3(nb), dim3(256), 0, st, n, posm, vel, partial);
-    hipLaunchKernelGGL(k_diag_final, dim3(1), dim3(64), 0, st, nb, partial, out);
+void launch_diag(hipStream_t st, const SceneTable& T, const float4* posm, const float4* vel,
+                 double* partial, double* out) {
+    if (T.ntiles > 0)
+        hipLaunchKernelGGL(k_diag_partial, dim3(T.ntiles), dim3(256), 0, st, T, posm, vel, partial);
+    hipLaunchKernelGGL(k_diag_final, dim3(T.nscenes), dim3(256), 0, st, T, partial, out);
 }
 
 void launch_unsort4(hipStream_t st, int n, const uint32_t* id, const float4* in, float4* out) {
@@ -656,12 +766,12 @@ void launch_unsort1(hipStream_t st, int n, const uint32_t* id, const float* in, 
     hipLaunchKernelGGL(k_unsort1, dim3(nblk(n, 256)), dim3(256), 0, st, n, id, in, out);
 }
 
-void launch_neighbor_stats(hipStream_t st, int n, const float4* posm, const uint32_t* cellv,
-                           const uint32_t* start, const uint32_t* id, const Domain* d,
-                           const DevParams& P, uint32_t* cnt, unsigned long long* sum,
-                           unsigned long long* xr) {
-    hipLaunchKernelGGL(k_neighbor_stats, dim3(nblk(n, 256)), dim3(256), 0, st, n, posm, cellv,
-                       start, id, d, P, cnt, sum, xr);
+void launch_neighbor_stats(hipStream_t st, const SceneTable& T, const float4* posm,
+                           const uint32_t* cellv, const uint32_t* start, const uint32_t* id,
+                           uint32_t* cnt, unsigned long long* sum, unsigned long long* xr) {
+    if (T.ntiles == 0) return;
+    hipLaunchKernelGGL(k_neighbor_stats, dim3(T.ntiles), dim3(256), 0, st, T, posm, cellv, start, id,
+                       cnt, sum, xr);
 }
 
 }  // namespace phy

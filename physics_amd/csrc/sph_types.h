@@ -16,6 +16,9 @@
 namespace phy {
 
 constexpr int kMaxMaterials = 16;
+// Particles per tile: the per-particle kernels run one 256-lane workgroup per
+// tile, and a tile never spans two scenes.
+constexpr int kTile = 256;
 // Largest dense cell table (cells) one world may allocate; particles whose cell
 // falls outside the table go to the overflow segment (exact, but O(n) per visit).
 constexpr uint64_t kMaxDenseCells = 1ull << 26;
@@ -41,22 +44,47 @@ struct DevParams {
     float visc_n[kMaxMaterials];
 };
 
-// Dense grid domain for one step. Keys are the reference's i64 cell keys
-// (grid.rs:26-33); the table covers kmin .. kmin + dims - 1 per axis with z the
-// fastest-varying axis, so the three z-adjacent cells of a neighbour row are one
-// contiguous run of sorted particles. Cell index ncells is the overflow segment.
+// One independent scene of a batch (a single FluidWorld is a batch of one).
+// Scenes occupy consecutive particle ranges [off, off + n) of the concatenated
+// device arrays and consecutive tile ranges [tile0, tile0 + ntiles).
+struct SceneInfo {
+    uint32_t off, n;
+    uint32_t tile0, ntiles;
+    unsigned long long cap;  // dense cell-table capacity of the scene (cells)
+    long long bkey[6];       // cell keys of the scene's bounds box: lo xyz, hi xyz
+};
+
+// Dense grid domain of one scene for one step. Keys are the reference's i64
+// cell keys (grid.rs:26-33); the table covers kmin .. kmin + dims - 1 per axis
+// with z the fastest-varying axis, so the three z-adjacent cells of a neighbour
+// row are one contiguous run of sorted particles. Local cell index ncells is
+// the scene's overflow segment. The scene's cells 0 .. ncells sit at toff in
+// the concatenated cell table of the step.
 struct Domain {
     long long kmin[3];
     long long dims[3];
     uint32_t ncells;
-    uint32_t pad;
+    uint32_t toff;
     // AABB of the finite keys of the current positions (filled by k_aabb).
     long long amin[3];
     long long amax[3];
-    // Watchdog (subsystem.rs:74-90): smallest original index with a non-finite
-    // position / velocity after the step; 0xffffffff when none.
+    // Watchdog (subsystem.rs:74-90): smallest particle index (concatenated
+    // original order) with a non-finite position / velocity after the step;
+    // 0xffffffff when none.
     uint32_t bad_pos;
     uint32_t bad_vel;
+};
+
+// Device view of a batch, passed by value to every per-particle kernel.
+struct SceneTable {
+    const SceneInfo* info;        // [nscenes]
+    const uint32_t* tile_scene;   // [ntiles] scene of each tile
+    const DevParams* params;      // [nscenes]
+    Domain* dom;                  // [nscenes], per step-buffer set
+    uint32_t* table_total;        // cells of this step's concatenated table (incl. overflow cells)
+    int nscenes;
+    int ntiles;
+    int n;                        // particles over all scenes
 };
 
 }  // namespace phy

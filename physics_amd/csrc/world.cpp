@@ -368,6 +368,36 @@ double FluidSubsystem::kinetic_energy() {
 
 // ---------------------------------------------------------------------------
 
+// ---------------------------------------------------------------------------
+
+static std::vector<HostParams> hosts_of(const std::vector<SphParams>& ps) {
+    std::vector<HostParams> h;
+    for (const SphParams& p : ps) h.push_back(p.to_host());
+    return h;
+}
+
+FluidBatch::FluidBatch(const std::vector<SphParams>& params)
+    : params_(params), dev_(hosts_of(params)) {
+    dev_.set_graph(true);
+}
+
+void FluidBatch::set_params(size_t s, const SphParams& p) {
+    dev_.set_scene_params(s, p.to_host());
+    params_.at(s) = p;
+}
+
+void FluidBatch::set_particles_f32(const uint32_t* counts, const float* pos3, const float* vel3,
+                                   const float* mass, const uint32_t* material) {
+    dev_.upload_scenes(counts, pos3, vel3, mass, material);
+}
+
+void FluidBatch::step_checked(double dt, int32_t* status) {
+    dev_.step((float)dt);
+    dev_.last_validate_all(status);
+}
+
+// ---------------------------------------------------------------------------
+
 void World::step_skipping_seeded(double dt, const std::vector<bool>& skip, uint64_t seed) {
     seed_ = seed;
     started_ = true;

@@ -148,6 +148,32 @@ class FluidSubsystem : public Subsystem {
     std::unique_ptr<FluidWorld> world_;
 };
 
+// Many independent FluidWorlds of uneven size stepped together on one GPU (no
+// counterpart in the reference). Scene s behaves as FluidWorld(params[s])
+// holding particles [offset(s), offset(s) + count(s)) of the concatenated
+// arrays; all scenes share dt and advance with one launch per stage, the
+// step replayed from a captured hipGraph.
+class FluidBatch {
+   public:
+    explicit FluidBatch(const std::vector<SphParams>& params);
+    size_t scenes() const { return params_.size(); }
+    const SphParams& params(size_t s) const { return params_.at(s); }
+    void set_params(size_t s, const SphParams& p);
+    // counts[scenes()]; FP32 arrays concatenated in scene order (vel, mass,
+    // material may be null: zero, SphParams::mass, 0)
+    void set_particles_f32(const uint32_t* counts, const float* pos3, const float* vel3,
+                           const float* mass, const uint32_t* material);
+    void step(double dt) { dev_.step((float)dt); }
+    // step, then FluidSubsystem::validate of every scene (subsystem.rs:74-90):
+    // status[s] = 0 ok, 1 non-finite position, 2 non-finite velocity
+    void step_checked(double dt, int32_t* status);
+    DeviceFluid& device() { return dev_; }
+
+   private:
+    std::vector<SphParams> params_;
+    DeviceFluid dev_;
+};
+
 class World {
    public:
     World() = default;

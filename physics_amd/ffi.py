@@ -39,7 +39,14 @@ EXPORTS = [
     "phy_fluid_set_keep_aux", "phy_fluid_set_body_acc", "phy_fluid_diagnostics_f64",
     "phy_fluid_neighbor_stats", "phy_world_step_n", "phy_fluid_sync", "phy_fluid_profile_begin",
     "phy_fluid_profile_end", "phy_fluid_tape_begin", "phy_fluid_tape_steps", "phy_fluid_tape_backward",
-    "phy_fluid_tape_end",
+    "phy_fluid_tape_end", "phy_fluid_set_graph", "phy_fluid_batch_create", "phy_fluid_batch_destroy",
+    "phy_fluid_batch_scenes", "phy_fluid_batch_set_params", "phy_fluid_batch_get_params",
+    "phy_fluid_batch_set_particles_f32", "phy_fluid_batch_offsets", "phy_fluid_batch_step",
+    "phy_fluid_batch_step_checked", "phy_fluid_batch_set_graph", "phy_fluid_batch_set_keep_aux",
+    "phy_fluid_batch_sync", "phy_fluid_batch_get_field_f32", "phy_fluid_batch_diagnostics_f64",
+    "phy_fluid_batch_neighbor_stats", "phy_fluid_batch_tape_begin", "phy_fluid_batch_tape_steps",
+    "phy_fluid_batch_tape_backward", "phy_fluid_batch_tape_end", "phy_fluid_batch_profile_begin",
+    "phy_fluid_batch_profile_end",
 ]
 
 
@@ -157,6 +164,28 @@ def lib():
         "phy_fluid_tape_steps": ([H], C.c_uint32),
         "phy_fluid_tape_backward": ([H, dp, dp, dp, dp, dp, dp], C.c_int32),
         "phy_fluid_tape_end": ([H], C.c_int32),
+        "phy_fluid_set_graph": ([H, C.c_int32], C.c_int32),
+        "phy_fluid_batch_create": ([C.POINTER(PhySphParams), C.c_uint32], H),
+        "phy_fluid_batch_destroy": ([H], None),
+        "phy_fluid_batch_scenes": ([H], C.c_uint32),
+        "phy_fluid_batch_set_params": ([H, C.c_uint32, C.POINTER(PhySphParams)], C.c_int32),
+        "phy_fluid_batch_get_params": ([H, C.c_uint32, C.POINTER(PhySphParams)], C.c_int32),
+        "phy_fluid_batch_set_particles_f32": ([H, u32p, fp, fp, fp, u32p], C.c_int32),
+        "phy_fluid_batch_offsets": ([H, u32p], C.c_int32),
+        "phy_fluid_batch_step": ([H, C.c_double, C.c_uint32], C.c_int32),
+        "phy_fluid_batch_step_checked": ([H, C.c_double, C.POINTER(C.c_int32)], C.c_int32),
+        "phy_fluid_batch_set_graph": ([H, C.c_int32], C.c_int32),
+        "phy_fluid_batch_set_keep_aux": ([H, C.c_int32], C.c_int32),
+        "phy_fluid_batch_sync": ([H], C.c_int32),
+        "phy_fluid_batch_get_field_f32": ([H, C.c_int32, fp, up], up),
+        "phy_fluid_batch_diagnostics_f64": ([H, dp], C.c_int32),
+        "phy_fluid_batch_neighbor_stats": ([H, u32p, u64p, u64p, up], C.c_int32),
+        "phy_fluid_batch_tape_begin": ([H, C.c_uint32], C.c_int32),
+        "phy_fluid_batch_tape_steps": ([H], C.c_uint32),
+        "phy_fluid_batch_tape_backward": ([H, fp, fp, fp, fp, fp, dp], C.c_int32),
+        "phy_fluid_batch_tape_end": ([H], C.c_int32),
+        "phy_fluid_batch_profile_begin": ([H], C.c_int32),
+        "phy_fluid_batch_profile_end": ([H, dp, u32p], C.c_int32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -326,6 +355,11 @@ class FluidWorld(World):
             raise PhyError(last_error())
         return ms, int(steps.value)
 
+    def set_graph(self, on: bool = True) -> None:
+        """Replay the step from a captured hipGraph (bit-identical results)."""
+        if lib().phy_fluid_set_graph(self._h, 1 if on else 0) != 0:
+            raise PhyError(last_error())
+
     def tape_begin(self, checkpoint_every: int) -> None:
         """Start recording a rollout for reverse-mode gradients."""
         if lib().phy_fluid_tape_begin(self._h, int(checkpoint_every)) != 0:
@@ -364,3 +398,140 @@ class FluidWorld(World):
         if rc != 0:
             raise PhyError(last_error())
         return c, s, x
+
+
+def _u32(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+class FluidBatch:
+    """Independent scenes (one FluidWorld each) stepped together on the GPU.
+
+    ``params``: one PhySphParams per scene. Particles are passed and returned
+    concatenated in scene order; ``offsets()`` gives each scene's range.
+    """
+
+    def __init__(self, params):
+        params = [p.copy() for p in params]
+        arr = (PhySphParams * len(params))(*params)
+        h = lib().phy_fluid_batch_create(arr, len(params))
+        if not h:
+            raise PhyError(last_error())
+        self._h = h
+        self.nscenes = len(params)
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().phy_fluid_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _ok(self, rc) -> None:
+        if rc != 0:
+            raise PhyError(last_error())
+
+    def set_params(self, scene: int, params: PhySphParams) -> None:
+        self._ok(lib().phy_fluid_batch_set_params(self._h, scene, C.byref(params)))
+
+    def get_params(self, scene: int) -> PhySphParams:
+        p = PhySphParams()
+        self._ok(lib().phy_fluid_batch_get_params(self._h, scene, C.byref(p)))
+        return p
+
+    def set_particles_f32(self, counts, pos, vel=None, mass=None, material=None) -> None:
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        n = int(counts.sum())
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(n, 3)
+        vel = None if vel is None else np.ascontiguousarray(vel, dtype=np.float32).reshape(n, 3)
+        mass = None if mass is None else np.ascontiguousarray(
+            np.broadcast_to(np.asarray(mass, dtype=np.float32), (n,)))
+        mat = None if material is None else np.ascontiguousarray(material, dtype=np.uint32)
+        self._ok(lib().phy_fluid_batch_set_particles_f32(self._h, _u32(counts), _fp(pos), _fp(vel),
+                                                          _fp(mass), _u32(mat)))
+
+    def offsets(self) -> np.ndarray:
+        o = np.zeros(self.nscenes + 1, dtype=np.uint32)
+        self._ok(lib().phy_fluid_batch_offsets(self._h, _u32(o)))
+        return o
+
+    def __len__(self) -> int:
+        return int(self.offsets()[-1])
+
+    def step(self, dt: float, n: int = 1) -> None:
+        self._ok(lib().phy_fluid_batch_step(self._h, dt, n))
+
+    def step_checked(self, dt: float) -> np.ndarray:
+        st = np.zeros(self.nscenes, dtype=np.int32)
+        rc = lib().phy_fluid_batch_step_checked(self._h, dt, st.ctypes.data_as(C.POINTER(C.c_int32)))
+        if rc < 0:
+            raise PhyError(last_error())
+        return st
+
+    def set_graph(self, on: bool = True) -> None:
+        self._ok(lib().phy_fluid_batch_set_graph(self._h, 1 if on else 0))
+
+    def set_keep_aux(self, on: bool = True) -> None:
+        self._ok(lib().phy_fluid_batch_set_keep_aux(self._h, 1 if on else 0))
+
+    def sync(self) -> None:
+        self._ok(lib().phy_fluid_batch_sync(self._h))
+
+    def get(self, field: str) -> np.ndarray:
+        n = len(self)
+        comps = 3 if field in VEC_FIELDS else 1
+        out = np.zeros(n * comps, dtype=np.float32)
+        got = lib().phy_fluid_batch_get_field_f32(self._h, FIELDS[field], _fp(out), out.size)
+        if got != n:
+            raise PhyError(f"get({field}) wrote {got} of {n}: {last_error()}")
+        return out.reshape(n, 3) if comps == 3 else out
+
+    def diagnostics(self) -> np.ndarray:
+        out = np.zeros((self.nscenes, 5))
+        self._ok(lib().phy_fluid_batch_diagnostics_f64(self._h, _dp(out)))
+        return out
+
+    def neighbor_stats(self):
+        n = len(self)
+        c = np.zeros(n, dtype=np.uint32)
+        s = np.zeros(n, dtype=np.uint64)
+        x = np.zeros(n, dtype=np.uint64)
+        self._ok(lib().phy_fluid_batch_neighbor_stats(
+            self._h, _u32(c), s.ctypes.data_as(C.POINTER(C.c_uint64)),
+            x.ctypes.data_as(C.POINTER(C.c_uint64)), n))
+        return c, s, x
+
+    def tape_begin(self, checkpoint_every: int) -> None:
+        self._ok(lib().phy_fluid_batch_tape_begin(self._h, int(checkpoint_every)))
+
+    def tape_steps(self) -> int:
+        return int(lib().phy_fluid_batch_tape_steps(self._h))
+
+    def tape_backward(self, xbar=None, vbar=None, rhobar=None):
+        """-> (x0_bar, v0_bar, dL/d stiffness per scene, dL/d visc_k[0] per scene)."""
+        n = len(self)
+        xb = None if xbar is None else np.ascontiguousarray(xbar, dtype=np.float32).reshape(n, 3)
+        vb = None if vbar is None else np.ascontiguousarray(vbar, dtype=np.float32).reshape(n, 3)
+        rb = None if rhobar is None else np.ascontiguousarray(rhobar, dtype=np.float32).reshape(n)
+        x0 = np.zeros((n, 3), dtype=np.float32)
+        v0 = np.zeros((n, 3), dtype=np.float32)
+        dp = np.zeros((self.nscenes, 2))
+        self._ok(lib().phy_fluid_batch_tape_backward(self._h, _fp(xb), _fp(vb), _fp(rb), _fp(x0),
+                                                      _fp(v0), _dp(dp)))
+        return x0, v0, dp[:, 0].copy(), dp[:, 1].copy()
+
+    def tape_end(self) -> None:
+        self._ok(lib().phy_fluid_batch_tape_end(self._h))
+
+    def profile_begin(self) -> None:
+        self._ok(lib().phy_fluid_batch_profile_begin(self._h))
+
+    def profile_end(self):
+        ms = np.zeros(4)
+        steps = C.c_uint32(0)
+        self._ok(lib().phy_fluid_batch_profile_end(self._h, _dp(ms), C.byref(steps)))
+        return ms, int(steps.value)

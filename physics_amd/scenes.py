@@ -133,3 +133,33 @@ def rollout_loss_cotangents(x: np.ndarray, rho: np.ndarray, rest_density: float 
     c = x.mean(0)
     L = float(c @ c + ((rho - rest_density) ** 2).mean())
     return L, np.tile(2.0 * c / n, (n, 1)), 2.0 * (rho - rest_density) / n
+
+
+def batch_counts(nscenes: int, seed: int, lo: int = 2000, hi: int = 64000) -> np.ndarray:
+    """Particle counts round(exp(U[ln lo, ln hi])) of a batch, drawn from the seed."""
+    u = rng_f64(seed, nscenes)
+    return np.rint(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.int64)
+
+
+def batch_scene(n: int, seed: int, vel_sigma: float = 0.1) -> Scene:
+    """One scene of a batched run: the first n particles (fill order x, y, z)
+    of an a x a x a lattice block, a = ceil(cbrt(n)), spacing h/2, +-1 %
+    jitter, N(0, vel_sigma) velocities, in the corner of its own tank of twice
+    the block's width per axis."""
+    a = max(1, int(math.ceil(round(n ** (1.0 / 3.0), 9))))
+    while a ** 3 < n:
+        a += 1
+    pos = lattice_block(a, a, a, S32, (0.0, 0.0, 0.0), 0.01, seed)[:n]
+    u1 = rng_f64(seed + 1000, 3 * n)
+    u2 = rng_f64(seed + 2000, 3 * n)
+    z = np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+    vel = (vel_sigma * z).reshape(n, 3).astype(np.float32)
+    side = 2.0 * a * S32
+    return Scene(pos=pos, vel=vel, mass=lattice_mass(H32, S32), h=H32,
+                 bounds_min=(0.0, 0.0, 0.0), bounds_max=(side, side, side))
+
+
+def batch_scenes(nscenes: int = 256, seed: int = 7, lo: int = 2000, hi: int = 64000):
+    """The batched-scenes workload: nscenes independent dam breaks with
+    log-uniform particle counts in [lo, hi]."""
+    return [batch_scene(int(n), seed * 100003 + s) for s, n in enumerate(batch_counts(nscenes, seed, lo, hi))]
