@@ -47,6 +47,13 @@ def parse():
     ap.add_argument("--adj-side", type=int, default=32)
     ap.add_argument("--adj-steps", type=int, default=100)
     ap.add_argument("--adj-k", type=int, default=10)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--no-inverse", action="store_true")
+    ap.add_argument("--inv-rollouts", type=int, default=16)
+    ap.add_argument("--inv-steps", type=int, default=200)
+    ap.add_argument("--inv-k", type=int, default=20)
     return ap.parse_args()
 
 
@@ -145,6 +152,10 @@ def main():
     if not a.no_adjoint:
         out["adjoint"] = adjoint_leg(a)
         out["adjoint_forward_ratio"] = out["adjoint"]["ratio"]
+    if not a.no_batch:
+        out["batched_scenes"] = batch_leg(a)
+    if not a.no_inverse:
+        out["inverse_design"] = inverse_leg(a)
 
     if not a.no_cpu and snaps:
         import oracle as O
@@ -227,6 +238,142 @@ def adjoint_leg(a):
         "loss": L,
         "dL_dstiffness": dk,
         "dL_dviscosity": dmu,
+    }
+
+
+def _concat(scs):
+    pos = np.concatenate([sc.pos for sc in scs])
+    vel = np.concatenate([sc.vel for sc in scs])
+    mass = np.concatenate([np.full(len(sc.pos), sc.mass, dtype=np.float32) for sc in scs])
+    return [len(sc.pos) for sc in scs], pos, vel, mass
+
+
+def batch_leg(a):
+    """Independent dam breaks of log-uniform size in one batch: every stage is
+    one launch over all scenes and each step is one hipGraph replay. Also a
+    one-step f64-oracle check of a sample of scenes after the timed run."""
+    from physics_amd import ffi, scenes
+    scs = scenes.batch_scenes(a.batch_scenes, seed=7)
+    counts, pos, vel, mass = _concat(scs)
+    params = [scenes.apply_scene(ffi.PhySphParams.defaults(), sc) for sc in scs]
+    dt = float(np.float32(scenes.DT))
+    b = ffi.FluidBatch(params)
+    b.set_particles_f32(counts, pos, vel, mass)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(dt, a.batch_steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    # stage split of a few eager (event-profiled) steps at the end state
+    b.profile_begin()
+    b.step(dt, 5)
+    ms, k = b.profile_end()
+    res = {
+        "config": f"{len(scs)} scenes, {sum(counts)} particles (log-uniform 2k..64k per scene, "
+                  f"min {min(counts)}, max {max(counts)}), {a.batch_steps} FP32 steps from rest "
+                  f"shape + N(0, 0.1) velocities, dt {dt:.6g}, hipGraph replay",
+        "scene_steps_per_s": len(scs) * a.batch_steps / wall,
+        "particle_steps_per_s": sum(counts) * a.batch_steps / wall,
+        "ms_per_step": wall / a.batch_steps * 1e3,
+        "eager_stage_ms_per_step_at_end": {"sort": ms[0] / k, "density": ms[1] / k, "force": ms[2] / k},
+    }
+    if not a.no_cpu:
+        import oracle as O
+        off = b.offsets()
+        x0, v0, m = b.get("pos"), b.get("vel"), b.get("mass")
+        b.step(dt)
+        x1, v1, r1 = b.get("pos"), b.get("vel"), b.get("rho")
+        worst = {"rho": 0.0, "vel": 0.0, "dx": 0.0}
+        sample = list(range(0, len(scs), max(1, len(scs) // 16)))
+        for s in sample:
+            lo, hi = off[s], off[s + 1]
+            o = O.World(scenes.apply_scene(O.Params.defaults(), scs[s]))
+            o.set_particles(x0[lo:hi].astype(np.float64), v0[lo:hi].astype(np.float64),
+                            m[lo:hi].astype(np.float64))
+            o.step(dt)
+            xo = o.get("pos")
+            ulp = np.abs(np.spacing(xo.astype(np.float32)).astype(np.float64))
+            exc = np.maximum(np.abs(x1[lo:hi] - xo) - ulp, 0.0)
+            disp = np.sqrt(((xo - x0[lo:hi]) ** 2).sum(1)).max()
+            e = {"rho": np.abs(r1[lo:hi] - o.get("rho")).max() / np.abs(o.get("rho")).max(),
+                 "vel": np.sqrt(((v1[lo:hi] - o.get("vel")) ** 2).sum(1)).max()
+                 / np.sqrt((o.get("vel") ** 2).sum(1)).max(),
+                 "dx": np.sqrt((exc ** 2).sum(1)).max() / disp}
+            for f in worst:
+                worst[f] = max(worst[f], float(e[f]))
+        res["oracle_one_step_error_worst_of_sample"] = worst
+        res["oracle_sample_scenes"] = len(sample)
+    b.close()
+    return res
+
+
+def _rollout_cotangents(x, rho, off):
+    from physics_amd import scenes
+    losses = []
+    xbar = np.zeros_like(x)
+    rhobar = np.zeros_like(rho)
+    for s in range(len(off) - 1):
+        lo, hi = off[s], off[s + 1]
+        L, xbar[lo:hi], rhobar[lo:hi] = scenes.rollout_loss_cotangents(x[lo:hi], rho[lo:hi])
+        losses.append(L)
+    return losses, xbar, rhobar
+
+
+def inverse_leg(a):
+    """Batched differentiable rollouts with per-rollout viscosity: recorded
+    forward (checkpoint every K) + reverse-mode gradient of every rollout."""
+    from physics_amd import ffi, scenes
+    ro = scenes.inverse_design_rollouts(a.inv_rollouts, seed=5)
+    scs = [sc for sc, _ in ro]
+    params = []
+    for sc, mu in ro:
+        p = scenes.apply_scene(ffi.PhySphParams.defaults(), sc)
+        p.set_visc([mu], [1.0])
+        params.append(p)
+    counts, pos, vel, mass = _concat(scs)
+    dt = float(np.float32(scenes.DT))
+    free0, total = ffi.device_mem_info()
+    b = ffi.FluidBatch(params)
+    b.set_particles_f32(counts, pos, vel, mass)
+    off = b.offsets()
+    # warm-up: kernels loaded, graph captured, tape buffers allocated
+    b.tape_begin(a.inv_k)
+    b.step(dt, 2)
+    b.tape_backward(np.zeros((len(pos), 3), np.float32), None, None)
+    b.tape_end()
+    b.set_particles_f32(counts, pos, vel, mass)
+    b.sync()
+    t0 = time.perf_counter()
+    b.step(dt, a.inv_steps)
+    b.sync()
+    t_plain = time.perf_counter() - t0
+    b.set_particles_f32(counts, pos, vel, mass)
+    b.sync()
+    t0 = time.perf_counter()
+    b.tape_begin(a.inv_k)
+    b.step(dt, a.inv_steps)
+    b.sync()
+    t_fwd = time.perf_counter() - t0
+    losses, xbar, rhobar = _rollout_cotangents(b.get("pos"), b.get("rho"), off)
+    t0 = time.perf_counter()
+    _, v0b, dk, dmu = b.tape_backward(xbar, None, rhobar)
+    t_bwd = time.perf_counter() - t0
+    free1, _ = ffi.device_mem_info()
+    b.tape_end()
+    b.close()
+    return {
+        "config": f"{len(scs)} rollouts x {counts[0]} particles (64x64x32 blocks, free fall), "
+                  f"visc_k[0] ~ U[0.5, 2] x 3.5 per rollout, {a.inv_steps} FP32 steps, "
+                  f"checkpoint every {a.inv_k}",
+        "rollouts_per_s_forward_plus_adjoint": len(scs) / (t_fwd + t_bwd),
+        "forward_ms": t_plain * 1e3,
+        "recorded_forward_ms": t_fwd * 1e3,
+        "backward_ms": t_bwd * 1e3,
+        "adjoint_forward_ratio": t_bwd / t_plain,
+        "hbm_in_use_after_backward_GB": (free0 - free1) / 1e9,
+        "dL_dviscosity": [float(x) for x in dmu],
+        "dL_dstiffness": [float(x) for x in dk],
+        "loss": losses,
     }
 
 

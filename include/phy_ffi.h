@@ -242,6 +242,10 @@ int32_t phy_fluid_profile_end(PhyWorldHandle *w, double *ms4, uint32_t *steps);
  * Results are bit-identical to eager steps. 0 ok, -1 error. */
 int32_t phy_fluid_set_graph(PhyWorldHandle *w, int32_t on);
 
+/* Device memory of the current HIP device (hipMemGetInfo): free and total
+ * bytes. 0 ok, -1 error. */
+int32_t phy_device_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
+
 /* ---- Batches of independent scenes (no counterpart in the reference) ----
  * A batch holds n_scenes independent FluidWorlds of uneven particle counts in
  * one set of device arrays: scene s owns particles [offsets[s], offsets[s+1])

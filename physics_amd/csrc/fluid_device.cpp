@@ -574,13 +574,16 @@ DeviceFluid::StepBufs DeviceFluid::slot_bufs(TapeSlot& t) {
     t.id_s.ensure(n); t.mat_s.ensure(n); t.cell_s.ensure(n);
     t.rho.ensure(n); t.prs.ensure(n); t.mp.ensure(n);
     t.acc.ensure(n); t.xsph.ensure(n); t.fvisc.ensure(n); t.mu.ensure(n);
-    t.start.ensure(table_cap_ + 1);
+    if (!compact_slots()) t.start.ensure(table_cap_ + 1);
     t.dom.ensure(scn_.size());
     StepBufs s;
     s.posm_s = t.posm_s.get(); s.velm_s = t.velm_s.get(); s.body_s = t.body_s.get();
     s.id_s = t.id_s.get(); s.mat_s = t.mat_s.get(); s.cell_s = t.cell_s.get();
     s.rho = t.rho.get(); s.prs = t.prs.get(); s.mp = t.mp.get();
-    s.start = t.start.get(); s.dom = t.dom.get();
+    // with compact slots the sort uses the world's own full-size table and
+    // the used part is copied into the slot after the step
+    s.start = compact_slots() ? start_.get() : t.start.get();
+    s.dom = t.dom.get();
     s.acc = t.acc.get(); s.mu = t.mu.get(); s.xsph = t.xsph.get(); s.fvisc = t.fvisc.get();
     return s;
 }
@@ -635,7 +638,18 @@ void DeviceFluid::tape_backward(const float* xbar_T, const float* vbar_T, const 
     for (size_t seg = nseg; seg-- > 0;) {
         load_state(*ckpt_[seg]);
         const size_t s0 = seg * K, s1 = std::min(T, s0 + K);
-        for (size_t t = s0; t < s1; ++t) step_core(tape_dt_[t], slot_bufs(*slots_[t - s0]));
+        for (size_t t = s0; t < s1; ++t) {
+            TapeSlot& sl = *slots_[t - s0];
+            step_core(tape_dt_[t], slot_bufs(sl));
+            if (compact_slots()) {
+                uint32_t total = 0;
+                hip_check(hipMemcpyAsync(&total, table_total_.get(), sizeof(uint32_t), hipMemcpyDeviceToHost, st_), "table size");
+                sync();
+                sl.start.ensure((size_t)total + 1);
+                hip_check(hipMemcpyAsync(sl.start.get(), start_.get(), ((size_t)total + 1) * sizeof(uint32_t),
+                                         hipMemcpyDeviceToDevice, st_), "slot table");
+            }
+        }
         for (size_t t = s1; t-- > s0;) {
             TapeSlot& sl = *slots_[t - s0];
             AdjSlot A;

@@ -201,6 +201,10 @@ class DeviceFluid {
     SceneTable table(Domain* dom) const;
     StepBufs own_bufs();
     StepBufs slot_bufs(TapeSlot& t);
+    // Tape slots keep only the used part of each step's cell table when the
+    // table's capacity is large (huge bounds boxes), at one host sync per
+    // recomputed step; small tables are kept whole, with no sync.
+    bool compact_slots() const { return table_cap_ > (1ull << 22); }
     void run_domain_and_sort(const StepBufs& s);
     void step_core(float dt, const StepBufs& s);
     void capture(float dt);

@@ -505,6 +505,16 @@ int32_t phy_fluid_set_graph(PhyWorldHandle* w, int32_t on) {
     });
 }
 
+int32_t phy_device_mem_info(uint64_t* free_bytes, uint64_t* total_bytes) {
+    return guard<int32_t>(-1, [&] {
+        size_t f = 0, t = 0;
+        phy::hip_check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+        if (free_bytes) *free_bytes = f;
+        if (total_bytes) *total_bytes = t;
+        return 0;
+    });
+}
+
 // ---------------------------------------------------------------------------
 // batches
 

@@ -46,7 +46,7 @@ EXPORTS = [
     "phy_fluid_batch_sync", "phy_fluid_batch_get_field_f32", "phy_fluid_batch_diagnostics_f64",
     "phy_fluid_batch_neighbor_stats", "phy_fluid_batch_tape_begin", "phy_fluid_batch_tape_steps",
     "phy_fluid_batch_tape_backward", "phy_fluid_batch_tape_end", "phy_fluid_batch_profile_begin",
-    "phy_fluid_batch_profile_end",
+    "phy_fluid_batch_profile_end", "phy_device_mem_info",
 ]
 
 
@@ -186,6 +186,7 @@ def lib():
         "phy_fluid_batch_tape_end": ([H], C.c_int32),
         "phy_fluid_batch_profile_begin": ([H], C.c_int32),
         "phy_fluid_batch_profile_end": ([H, dp, u32p], C.c_int32),
+        "phy_device_mem_info": ([u64p, u64p], C.c_int32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -398,6 +399,14 @@ class FluidWorld(World):
         if rc != 0:
             raise PhyError(last_error())
         return c, s, x
+
+
+def device_mem_info():
+    """(free, total) bytes of the current HIP device."""
+    f, t = C.c_uint64(0), C.c_uint64(0)
+    if lib().phy_device_mem_info(C.byref(f), C.byref(t)) != 0:
+        raise PhyError(last_error())
+    return int(f.value), int(t.value)
 
 
 def _u32(a):

@@ -163,3 +163,26 @@ def batch_scenes(nscenes: int = 256, seed: int = 7, lo: int = 2000, hi: int = 64
     """The batched-scenes workload: nscenes independent dam breaks with
     log-uniform particle counts in [lo, hi]."""
     return [batch_scene(int(n), seed * 100003 + s) for s, n in enumerate(batch_counts(nscenes, seed, lo, hi))]
+
+
+def inverse_design_rollouts(nrollouts: int = 16, seed: int = 5, nx: int = 64, ny: int = 64,
+                            nz: int = 32, base_visc: float = 3.5):
+    """The batched inverse-design workload: nrollouts blocks of nx x ny x nz
+    particles (spacing h/2, +-1 % jitter, N(0, 0.1) velocities) in free fall
+    with the walls out of reach (+-100 m), each with its own visc_k[0] drawn
+    uniform in [0.5, 2] x base_visc. Returns [(Scene, visc_k0)]."""
+    u = rng_f64(seed, nrollouts)
+    out = []
+    for r in range(nrollouts):
+        s = seed * 7919 + r
+        origin = (-0.5 * nx * S32, -0.5 * ny * S32, -0.5 * nz * S32)
+        pos = lattice_block(nx, ny, nz, S32, origin, 0.01, s)
+        n = len(pos)
+        u1 = rng_f64(s + 1000, 3 * n)
+        u2 = rng_f64(s + 2000, 3 * n)
+        z = np.sqrt(-2.0 * np.log1p(-u1)) * np.cos(2.0 * np.pi * u2)
+        vel = (0.1 * z).reshape(n, 3).astype(np.float32)
+        sc = Scene(pos=pos, vel=vel, mass=lattice_mass(H32, S32), h=H32,
+                   bounds_min=(-100.0, -100.0, -100.0), bounds_max=(100.0, 100.0, 100.0))
+        out.append((sc, base_visc * (0.5 + 1.5 * float(u[r]))))
+    return out
