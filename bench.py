@@ -29,6 +29,20 @@ if ROOT not in sys.path:
 # 60 B, density 20 B, fused force + integrate 56 B (read x, v, rho, p; write x', v')
 ALG_BYTES = {"sort": 60, "density": 20, "force": 56}
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled by a plain copy's counts) on the default 1000-step workload
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_force_1M_dambreak_hbm_traffic.md")
+
+
+def measured_traffic(kernel: str):
+    """Bytes per launch of `kernel` from the committed counter summary, or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def parse():
@@ -112,6 +126,12 @@ def main():
     achieved = ALG_BYTES[dom] * n / t_dom / 1e9
     x = g.get("pos")
     assert np.isfinite(x).all(), "non-finite state in the benchmark run"
+    # the counter summary describes the default workload only
+    default_run = n_side == 100 and a.seed == 1 and a.warmup + K == 1000
+    traffic = measured_traffic(f"k_{dom}") if default_run and dom != "sort" else None
+    traffic_src = ("rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE mean over the 1000 dispatches of this "
+                   "workload, scaled by a 1 GiB copy's counts (profiles/k_force_1M_dambreak_hbm_traffic.md)"
+                   if traffic else None)
 
     out = {
         "metric": "SPH particle-steps/s at 1M (FP32 fwd) + adjoint/forward time ratio, 1 GPU",
@@ -144,7 +164,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": traffic_src,
             "alg_bytes_per_particle": ALG_BYTES[dom],
         },
     }

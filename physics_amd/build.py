@@ -12,6 +12,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libphy_ffi.so")
+# counter-calibration copy program (rocprofv3 --pmc reference point)
+COPY_EXE = os.path.join(HERE, "hbm_copy")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
@@ -60,6 +62,11 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    cmd = [HIPCC, *COMMON, os.path.join(CSRC, "hbm_copy.hip"), "-o", COPY_EXE + ".tmp"]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for hbm_copy.hip:\n" + r.stdout.decode(errors="replace"))
+    os.replace(COPY_EXE + ".tmp", COPY_EXE)
     return LIB
 
 
