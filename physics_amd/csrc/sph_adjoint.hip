@@ -138,11 +138,12 @@ __global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
     ForceAdjAcc A = {};
     gather<L>(
         act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, start, S.posm, lst, tid,
-        [&](uint32_t j, const float4& pj) { return force_accept(P, (uint32_t)a, pa.x, pa.y, pa.z, j, pj); },
+        [&](uint32_t j, const float4& pj) { return force_maybe(P, pa.x, pa.y, pa.z, pj); },
         [&](uint32_t j) { return AdjNbr{S.posm[j], S.velm[j], Fb[j], Sb[j]}; },
-        [&](const AdjNbr& nb, bool valid) {
+        [&](uint32_t j, const AdjNbr& nb, bool listed) {
             const float Dx = nb.p.x - pa.x, Dy = nb.p.y - pa.y, Dz = nb.p.z - pa.z;
             float s = Dx * Dx + Dy * Dy + Dz * Dz;
+            const bool valid = force_valid(P, listed, (uint32_t)a, j, pa.x, pa.y, pa.z, nb.p, s);
             s = s < 1.17549435e-38f ? 1.17549435e-38f : s;
             // padding slot (the particle itself, D = 0): evaluate the kernels at
             // r = h/2 so every term stays finite before its zero weight applies
@@ -256,11 +257,12 @@ __global__ __launch_bounds__(256) void k_adj_density(SceneTable T, AdjSlot S,
     };
     gather<L>(
         act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, start, S.posm, lst, tid,
-        [&](uint32_t j, const float4& pj) { return density_accept(P, pa.x, pa.y, pa.z, pj); },
+        [&](uint32_t j, const float4& pj) { return density_maybe(P, pa.x, pa.y, pa.z, pj); },
         [&](uint32_t j) { return DN{S.posm[j], rhob[j]}; },
-        [&](const DN& nb, bool valid) {
+        [&](uint32_t j, const DN& nb, bool listed) {
             const float Dx = nb.p.x - pa.x, Dy = nb.p.y - pa.y, Dz = nb.p.z - pa.z;
             const float s = Dx * Dx + Dy * Dy + Dz * Dz;
+            const bool valid = density_valid(P, listed, pa.x, pa.y, pa.z, nb.p, s);
             const float q = P.h2 - s;
             const float Wpr = valid ? -6.f * P.c6 * q * q : 0.f;  // W'(r) / r
             const float c1 = Wpr * nb.p.w, c2 = Wpr * nb.rb;

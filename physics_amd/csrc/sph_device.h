@@ -290,10 +290,14 @@ struct Cursor {
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
-// test(j, pj) -> bool (FP32 filter, exact f64 band); load(j) -> T;
-// proc(T, valid) accumulates (an invalid slot must contribute exactly nothing);
-// flush() after every drain. self_j is a valid index used to pad drain slots.
-// Overflow-segment candidates additionally need reference key adjacency.
+// test(j, pj) -> bool, a conservative FP32 filter (every true neighbour
+// passes; the exact decision is proc's); load(j) -> T; proc(j, T, listed)
+// accumulates (a slot that is not listed, or that proc's exact test rejects,
+// must contribute exactly nothing); flush() after every drain. self_j is a
+// valid index used to pad drain slots. Overflow-segment candidates
+// additionally need reference key adjacency. Appends are branch-free: every
+// candidate is written at the list head and the head advances only when the
+// filter passes.
 template <int L, class Test, class Load, class Proc, class Flush>
 __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi, float zi,
                                        uint32_t self_j, const Domain& d, double inv,
@@ -309,11 +313,15 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
             const uint32_t j1 = v1 ? lst[k + 1][tid] : self_j;
             auto t0 = load(j0);
             auto t1 = load(j1);
-            proc(t0, v0);
-            proc(t1, v1);
+            proc(j0, t0, v0);
+            proc(j1, t1, v1);
         }
         cnt = 0;
         flush();
+    };
+    auto push = [&](uint32_t j, bool ok) {
+        lst[cnt][tid] = j;
+        cnt += ok ? 1 : 0;
     };
     auto ov_ok = [&](const float4& pj) {
         return keys_adjacent(xi, yi, zi, pj.x, pj.y, pj.z, inv);
@@ -331,21 +339,23 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 const uint32_t last = b - 1;
                 const uint32_t j1 = min(j + 1, last), j2 = min(j + 2, last), j3 = min(j + 3, last);
                 const float4 p0 = posm[j], p1 = posm[j1], p2 = posm[j2], p3 = posm[j3];
-                const bool t0 = act && test(j, p0);
-                const bool t1 = act && j + 1 <= last && test(j1, p1);
-                const bool t2 = act && j + 2 <= last && test(j2, p2);
-                const bool t3 = act && j + 3 <= last && test(j3, p3);
-                if (t0) lst[cnt++][tid] = j;
-                if (t1) lst[cnt++][tid] = j1;
-                if (t2) lst[cnt++][tid] = j2;
-                if (t3) lst[cnt++][tid] = j3;
+                // non-short-circuit: the four loads issue together, no branches
+                const bool t0 = test(j, p0) & act;
+                const bool t1 = test(j1, p1) & (act & (j + 1 <= last));
+                const bool t2 = test(j2, p2) & (act & (j + 2 <= last));
+                const bool t3 = test(j3, p3) & (act & (j + 3 <= last));
+                // cnt <= L - 4 here, so the four writes stay inside the list
+                push(j, t0);
+                push(j1, t1);
+                push(j2, t2);
+                push(j3, t3);
                 if (__ballot(cnt > L - 4) != 0ull) drain();
             }
         }
         const uint32_t o0 = uni(start[d.ncells]), o1 = uni(start[d.ncells + 1]);
         for (uint32_t j = o0; j < o1; ++j) {
             const float4 pj = posm[j];
-            if (act && ov_ok(pj) && test(j, pj)) lst[cnt++][tid] = j;
+            push(j, act && ov_ok(pj) && test(j, pj));
             if (__ballot(cnt == L) != 0ull) drain();
         }
     } else {
@@ -355,7 +365,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
             if (act && cur.j < cur.jend) {
                 const uint32_t j = cur.j;
                 const float4 pj = posm[j];
-                if ((!cur.in_overflow() || ov_ok(pj)) && test(j, pj)) lst[cnt++][tid] = j;
+                push(j, (!cur.in_overflow() || ov_ok(pj)) && test(j, pj));
                 ++cur.j;
                 if (cur.j >= cur.jend) cur.next_run(d, start);
             }
@@ -365,35 +375,63 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
     drain();
 }
 
-// Density-pass neighbour test (world.rs:225-229): r < h, self included.
-__device__ __forceinline__ bool density_accept(const DevParams& P, float xi, float yi, float zi,
-                                               const float4& pj) {
-    float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
-    float s = dx * dx + dy * dy + dz * dz;
-    bool in = s < P.s_lo;
-    if (!in && !(s > P.s_hi)) {  // f64 band or NaN: decide exactly (rare)
-        double e = exact_s64(xi, yi, zi, pj.x, pj.y, pj.z);
-        in = e < P.s_thresh;
-    }
-    return in;
+// Candidate filters of the compaction. Both are conservative: |s32 - s64| <=
+// 6u s for any FP32 evaluation order, so every pair with r < h has
+// s32 < s_thresh (1 + 6u) < s_hi. The exact decision is made per listed slot.
+//
+// Density pass (world.rs:225-229): r < h, self included; a NaN distance fails.
+__device__ __forceinline__ bool density_maybe(const DevParams& P, float xi, float yi, float zi,
+                                              const float4& pj) {
+    const float dx = xi - pj.x, dy = yi - pj.y, dz = zi - pj.z;
+    return dx * dx + dy * dy + dz * dz < P.s_hi;
 }
 
-// Force-pass neighbour test (world.rs:292-299): j != i, skip r >= h and r <= 0;
-// a NaN distance falls through (contributes NaN, as in the reference).
-__device__ __forceinline__ bool force_accept(const DevParams& P, uint32_t i, float xi, float yi,
-                                             float zi, uint32_t j, const float4& pj) {
-    float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
-    float s = dx * dx + dy * dy + dz * dz;
-    // s > 0 rejects the particle itself and coincident points (r <= 0)
-    bool ok = s < P.s_lo && s > 0.f;
-    bool rare = (!(s < P.s_lo) && !(s > P.s_hi)) || (s == 0.f && j != i);
-    if (rare) {
-        // f64 band, NaN distance or an underflowed distinct pair
-        double e = exact_s64(xi, yi, zi, pj.x, pj.y, pj.z);
-        bool nz = dx != 0.f || dy != 0.f || dz != 0.f;
-        ok = j != i && ((e < P.s_thresh && nz) || e != e);
+// Force pass (world.rs:292-299): a NaN distance passes (it contributes NaN).
+__device__ __forceinline__ bool force_maybe(const DevParams& P, float xi, float yi, float zi,
+                                            const float4& pj) {
+    const float dx = pj.x - xi, dy = pj.y - yi, dz = pj.z - zi;
+    return !(dx * dx + dy * dy + dz * dz >= P.s_hi);
+}
+
+// The reference's f64 squared distance from f64 coordinate differences,
+// rounded exactly like the oracle's (no contraction).
+__device__ __forceinline__ double sq_exact(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    return (dx * dx + dy * dy) + dz * dz;
+}
+
+// Exact density-pass decision (r < h, self included) for a listed slot with
+// FP32 squared distance s32; the f64 band and NaN take the exact predicate.
+__device__ __forceinline__ bool density_valid(const DevParams& P, bool listed, float xi, float yi,
+                                              float zi, const float4& pj, float s32) {
+    const bool fast = s32 < P.s_lo;
+    const bool need = listed && !fast;
+    bool ex = false;
+    if (__ballot(need) != 0ull) {
+        if (need) ex = exact_s64(xi, yi, zi, pj.x, pj.y, pj.z) < P.s_thresh;
     }
-    return ok;
+    return listed && (fast || ex);
+}
+
+// Exact force-pass decision for a listed slot j of particle i (world.rs:
+// 292-299): skip j == i, r >= h and r <= 0; a NaN distance is kept. s32 is the
+// slot's FP32 squared distance. Clear cases are decided in FP32; the f64 band,
+// NaN and coincident distinct points take the exact f64 predicate, entered only
+// when some lane of the wave needs it.
+__device__ __forceinline__ bool force_valid(const DevParams& P, bool listed, uint32_t i, uint32_t j,
+                                            float xi, float yi, float zi, const float4& pj,
+                                            float s32) {
+    const bool fast = s32 < P.s_lo && s32 > 0.f;  // s32 > 0: distinct points, so j != i
+    const bool need = listed && j != i && !fast;
+    bool ex = false;
+    if (__ballot(need) != 0ull) {
+        if (need) {
+            const double e = exact_s64(xi, yi, zi, pj.x, pj.y, pj.z);
+            const bool nz = xi != pj.x || yi != pj.y || zi != pj.z;
+            ex = (e < P.s_thresh && nz) || e != e;
+        }
+    }
+    return listed && (fast || ex);
 }
 
 }  // namespace phy

@@ -390,13 +390,15 @@ __global__ __launch_bounds__(256) void k_density(SceneTable T, const float4* __r
     double sum = 0.0;
     gather<L>(
         act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, start, posm, lst, tid,
-        [&](uint32_t j, const float4& pj) { return density_accept(P, pi.x, pi.y, pi.z, pj); },
+        [&](uint32_t j, const float4& pj) { return density_maybe(P, pi.x, pi.y, pi.z, pj); },
         [&](uint32_t j) { return posm[j]; },
-        [&](const float4& pj, bool valid) {
-            double dx = xi - (double)pj.x, dy = yi - (double)pj.y, dz = zi - (double)pj.z;
-            double s = (dx * dx + dy * dy) + dz * dz;
-            double q = P.h2_64 - s;
-            double t = (double)pj.w * (q * q * q);
+        [&](uint32_t j, const float4& pj, bool listed) {
+            // the reference's own f64 distance decides r < h exactly (NaN fails)
+            const double dx = xi - (double)pj.x, dy = yi - (double)pj.y, dz = zi - (double)pj.z;
+            const double s = sq_exact(dx, dy, dz);
+            const bool valid = listed && s < P.s_thresh;
+            const double q = P.h2_64 - s;
+            const double t = (double)pj.w * (q * q * q);
             sum += valid ? t : 0.0;
         },
         [&]() {});
@@ -438,13 +440,15 @@ struct NbrData {
 };
 
 template <bool SHEAR>
-__device__ __forceinline__ void pair_force(ForceAcc<SHEAR>& A, const DevParams& P, float4 pi,
-                                           float4 vi, float Pi, const NbrData& nb, bool valid) {
+__device__ __forceinline__ void pair_force(ForceAcc<SHEAR>& A, const DevParams& P, uint32_t i,
+                                           uint32_t j, float4 pi, float4 vi, float Pi,
+                                           const NbrData& nb, bool listed) {
     const float4 pj = nb.p, vj = nb.v;
     float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+    float s = dx * dx + dy * dy + dz * dz;
+    const bool valid = force_valid(P, listed, i, j, pi.x, pi.y, pi.z, pj, s);
     // distinct FP32 points closer than ~1e-19 underflow s; keep r tiny but finite.
     // (not fmaxf: a NaN distance must stay NaN, as in the reference)
-    float s = dx * dx + dy * dy + dz * dz;
     s = s < 1.17549435e-38f ? 1.17549435e-38f : s;
     float inv_r = __builtin_amdgcn_rsqf(s);
     float r = s * inv_r;
@@ -458,7 +462,8 @@ __device__ __forceinline__ void pair_force(ForceAcc<SHEAR>& A, const DevParams& 
     float bb = mr * hr;
     // XSPH: (v_j - v_i) * (m_j / rho_j * poly6(r))
     float w = mr * (q * q * q);
-    // an invalid (padding) slot holds the particle itself: finite, zero weight
+    // a rejected or padding slot (the particle itself, a band pair outside
+    // r < h) is finite here and gets zero weight
     a = valid ? a : 0.f;
     bb = valid ? bb : 0.f;
     w = valid ? w : 0.f;
@@ -510,11 +515,11 @@ __global__ __launch_bounds__(256) void k_force(
     A.zero();
     gather<L>(
         act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, start, posm, lst, tid,
-        [&](uint32_t j, const float4& pj) {
-            return force_accept(P, (uint32_t)i, pi.x, pi.y, pi.z, j, pj);
-        },
+        [&](uint32_t j, const float4& pj) { return force_maybe(P, pi.x, pi.y, pi.z, pj); },
         [&](uint32_t j) { return NbrData{posm[j], velm[j], mp[j]}; },
-        [&](const NbrData& nb, bool valid) { pair_force<SHEAR>(A, P, pi, vi, Pi, nb, valid); },
+        [&](uint32_t j, const NbrData& nb, bool listed) {
+            pair_force<SHEAR>(A, P, (uint32_t)i, j, pi, vi, Pi, nb, listed);
+        },
         [&]() { A.merge(); });
     if (!act) return;
 
