@@ -242,6 +242,35 @@ int32_t phy_fluid_profile_end(PhyWorldHandle *w, double *ms4, uint32_t *steps);
  * Results are bit-identical to eager steps. 0 ok, -1 error. */
 int32_t phy_fluid_set_graph(PhyWorldHandle *w, int32_t on);
 
+/* Launch choices of the gather kernels: per-lane LDS neighbour-list length
+ * of the density and of the force pass (8, 16 or 32; default 16), and the
+ * launch form (split = 1: each pass is one launch for the waves whose lanes
+ * share a grid cell plus one for the others; 0: one launch). The force pass's
+ * compensated sums are blocked per list drain, so force_list is part of its
+ * summation order: keep it fixed (for example through the autotuner's cache)
+ * when runs must repeat bit for bit. */
+typedef struct PhyTuning {
+    int32_t density_list;
+    int32_t force_list;
+    int32_t split;
+} PhyTuning;
+
+int32_t phy_fluid_get_tuning(PhyWorldHandle *w, PhyTuning *out);
+int32_t phy_fluid_set_tuning(PhyWorldHandle *w, const PhyTuning *t);
+
+/* Autotuner. With cache_path naming a file that holds a choice for this
+ * device and workload class (architecture, log2 particle count, log2 scene
+ * count, non-Newtonian flag), that choice is applied and 1 is returned.
+ * Otherwise `steps` steps of dt are timed from the current state for every
+ * candidate (list lengths 8, 16, 32, each in one and in two launches), the
+ * fastest combination is applied (and stored in cache_path when given), the
+ * state is restored, and 0 is returned. rho / p / aux fields are undefined
+ * until the next step. out (may be NULL) receives the applied choice; ms12
+ * (may be NULL) the timed milliseconds per step: density for lists 8, 16, 32
+ * in one launch, then in two, then force in the same order. -1 on error. */
+int32_t phy_fluid_autotune(PhyWorldHandle *w, double dt, uint32_t steps, const char *cache_path,
+                           PhyTuning *out, double *ms12);
+
 /* Device memory of the current HIP device (hipMemGetInfo): free and total
  * bytes. 0 ok, -1 error. */
 int32_t phy_device_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
@@ -299,6 +328,12 @@ int32_t phy_fluid_batch_tape_backward(PhyFluidBatch *b, const float *xbar_final,
                                       const float *vbar_final, const float *rhobar_last,
                                       float *xbar0, float *vbar0, double *dparams);
 int32_t phy_fluid_batch_tape_end(PhyFluidBatch *b);
+
+/* As phy_fluid_get_tuning / set_tuning / autotune, for a whole batch. */
+int32_t phy_fluid_batch_get_tuning(PhyFluidBatch *b, PhyTuning *out);
+int32_t phy_fluid_batch_set_tuning(PhyFluidBatch *b, const PhyTuning *t);
+int32_t phy_fluid_batch_autotune(PhyFluidBatch *b, double dt, uint32_t steps, const char *cache_path,
+                                 PhyTuning *out, double *ms12);
 
 /* As phy_fluid_profile_* (profiled steps run eagerly, not from the graph). */
 int32_t phy_fluid_batch_profile_begin(PhyFluidBatch *b);

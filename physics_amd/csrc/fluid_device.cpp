@@ -8,6 +8,9 @@
 #include <cfloat>
 #include <climits>
 #include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <sstream>
 
 namespace phy {
 
@@ -437,7 +440,7 @@ void DeviceFluid::step_core(float dt, const StepBufs& s) {
     run_domain_and_sort(s);
     if (prof_) ev();
     const SceneTable T = table(s.dom);
-    launch_density(st_, T, s.posm_s, s.cell_s, s.start, s.rho, s.prs, s.velm_s, s.mp);
+    launch_density(st_, T, s.posm_s, s.cell_s, s.start, s.rho, s.prs, s.velm_s, s.mp, tuning_);
     ForceArgs a;
     a.posm = s.posm_s;
     a.velm = s.velm_s;
@@ -458,7 +461,7 @@ void DeviceFluid::step_core(float dt, const StepBufs& s) {
     a.xsph_out = s.xsph;
     a.fvisc_out = s.fvisc;
     if (prof_) ev();
-    launch_force(st_, T, dt, a, any_shear_);
+    launch_force(st_, T, dt, a, any_shear_, tuning_);
     if (prof_) ev();
     hip_check(hipGetLastError(), "step launch");
 }
@@ -492,6 +495,134 @@ void DeviceFluid::capture(float dt) {
 void DeviceFluid::set_graph(bool on) {
     graph_on_ = on;
     if (!on) drop_graph();
+}
+
+void DeviceFluid::set_tuning(const Tuning& t) {
+    if (!tuning_valid(t)) throw std::invalid_argument("tuning: list lengths must be 16, 32 or 64");
+    tuning_ = t;
+    graph_dirty_ = true;
+}
+
+std::string DeviceFluid::tuning_key() const {
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, dev_), "hipGetDeviceProperties");
+    std::string arch = prop.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));
+    auto lg = [](size_t v) {
+        int k = 0;
+        while (v > 1) {
+            v >>= 1;
+            ++k;
+        }
+        return k;
+    };
+    std::ostringstream o;
+    o << arch << " n" << lg(n_) << " s" << lg(scn_.size()) << " shear" << (any_shear_ ? 1 : 0);
+    return o.str();
+}
+
+// Cache lines: "<key> density_list=<L> force_list=<L> split=<0|1>", one per key.
+static bool read_tuning(const std::string& path, const std::string& key, Tuning& t) {
+    std::ifstream f(path);
+    std::string line;
+    while (std::getline(f, line)) {
+        if (line.compare(0, key.size() + 1, key + " ") != 0) continue;
+        Tuning c;
+        if (std::sscanf(line.c_str() + key.size(), " density_list=%d force_list=%d split=%d",
+                        &c.density_list, &c.force_list, &c.split) == 3 &&
+            tuning_valid(c)) {
+            t = c;
+            return true;
+        }
+    }
+    return false;
+}
+
+static void write_tuning(const std::string& path, const std::string& key, const Tuning& t) {
+    std::vector<std::string> keep;
+    {
+        std::ifstream f(path);
+        std::string line;
+        while (std::getline(f, line))
+            if (!line.empty() && line.compare(0, key.size() + 1, key + " ") != 0) keep.push_back(line);
+    }
+    std::ostringstream o;
+    o << key << " density_list=" << t.density_list << " force_list=" << t.force_list
+      << " split=" << t.split;
+    keep.push_back(o.str());
+    const std::string tmp = path + ".tmp";
+    {
+        std::ofstream f(tmp, std::ios::trunc);
+        if (!f) throw std::runtime_error("tuning cache: cannot write " + tmp);
+        for (const std::string& l : keep) f << l << "\n";
+    }
+    if (std::rename(tmp.c_str(), path.c_str()) != 0)
+        throw std::runtime_error("tuning cache: cannot replace " + path);
+}
+
+bool DeviceFluid::autotune(float dt, uint32_t steps, const char* cache_path, double* ms) {
+    if (taping_) throw std::logic_error("autotune: not while recording a tape");
+    if (prof_) throw std::logic_error("autotune: not while profiling");
+    if (has_body_) throw std::logic_error("autotune: a pending body_acc would be consumed");
+    const std::string key = tuning_key();
+    if (cache_path && *cache_path) {
+        Tuning t;
+        if (read_tuning(cache_path, key, t)) {
+            set_tuning(t);
+            return true;
+        }
+    }
+    if (n_ == 0 || steps == 0) return false;
+    Snapshot snap;
+    save_state(snap);
+    // trial k runs list length cand[k % 3] in both passes, launch form k / 3;
+    // the density and force passes are timed separately and chosen apart
+    const int cand[3] = {8, 16, 32};
+    double dms[6], fms[6];
+    for (int k = 0; k < 6; ++k) {
+        load_state(snap);
+        tuning_.density_list = tuning_.force_list = cand[k % 3];
+        tuning_.split = k / 3;
+        profile_begin();
+        for (uint32_t s = 0; s < steps; ++s) step_core(dt, own_bufs());
+        double m4[4];
+        const uint32_t nst = profile_end(m4);
+        dms[k] = m4[1] / nst;
+        fms[k] = m4[2] / nst;
+    }
+    // one launch form for both passes: the one with the smaller best sum
+    int best_split = 0;
+    double best_sum = 0.0;
+    int bd = 0, bf = 0;
+    for (int sp = 0; sp < 2; ++sp) {
+        int d0 = 3 * sp, f0 = 3 * sp;
+        for (int k = 1; k < 3; ++k) {
+            if (dms[3 * sp + k] < dms[d0]) d0 = 3 * sp + k;
+            if (fms[3 * sp + k] < fms[f0]) f0 = 3 * sp + k;
+        }
+        const double sum = dms[d0] + fms[f0];
+        if (sp == 0 || sum < best_sum) {
+            best_sum = sum;
+            best_split = sp;
+            bd = d0;
+            bf = f0;
+        }
+    }
+    load_state(snap);
+    sync();
+    Tuning t;
+    t.density_list = cand[bd % 3];
+    t.force_list = cand[bf % 3];
+    t.split = best_split;
+    set_tuning(t);
+    dens_valid_ = aux_valid_ = false;
+    if (ms)
+        for (int k = 0; k < 6; ++k) {
+            ms[k] = dms[k];
+            ms[6 + k] = fms[k];
+        }
+    if (cache_path && *cache_path) write_tuning(cache_path, key, t);
+    return false;
 }
 
 void DeviceFluid::step(float dt) {

@@ -165,6 +165,24 @@ class DeviceFluid {
     size_t tape_steps() const { return tape_dt_.size(); }
     bool taping() const { return taping_; }
 
+    // Launch choices of the gather kernels (sph_kernels.h Tuning).
+    const Tuning& tuning() const { return tuning_; }
+    void set_tuning(const Tuning& t);
+    // Autotuner: times `steps` eager steps of the current state for each
+    // candidate (list length 8, 16, 32 in one and in two launches; density and
+    // force timed separately from HIP events), keeps the launch form with the
+    // smaller best sum and the fastest list length of each pass in it, then
+    // restores the state. rho / p / aux are undefined until the next step.
+    // With a cache path, a stored choice for this device and workload class is
+    // applied without timing (returns true), and a new choice is stored, so
+    // repeated runs keep one summation order. ms[12] (optional): density
+    // milliseconds per step of the six trials (lists 8, 16, 32 combined, then
+    // split), then force.
+    bool autotune(float dt, uint32_t steps, const char* cache_path, double* ms);
+    // "gfx950 n20 s0 shear0": device architecture, log2 particles, log2 scenes,
+    // non-Newtonian flag (the cache key)
+    std::string tuning_key() const;
+
     // Per-stage HIP-event timing on this world's stream: while enabled, every
     // step records events around sort (domain + counting sort), density and
     // force; profile_end() synchronises and returns the summed milliseconds
@@ -250,6 +268,7 @@ class DeviceFluid {
     DevBuf<Domain> dom_;
     DevBuf<double> diag_part_, diag_out_;
 
+    Tuning tuning_;
     bool graph_on_ = false, graph_dirty_ = true;
     float graph_dt_ = 0.f;
     hipGraph_t graph_ = nullptr;

@@ -505,6 +505,78 @@ int32_t phy_fluid_set_graph(PhyWorldHandle* w, int32_t on) {
     });
 }
 
+static int32_t tune(phy::DeviceFluid& d, double dt, uint32_t steps, const char* cache_path,
+                    PhyTuning* out, double* ms12) {
+    const bool cached = d.autotune((float)dt, steps, cache_path, ms12);
+    if (out) {
+        out->density_list = d.tuning().density_list;
+        out->force_list = d.tuning().force_list;
+        out->split = d.tuning().split;
+    }
+    return cached ? 1 : 0;
+}
+
+static phy::Tuning from_c(const PhyTuning* t) {
+    if (!t) throw std::invalid_argument("null tuning");
+    phy::Tuning r;
+    r.density_list = t->density_list;
+    r.force_list = t->force_list;
+    r.split = t->split;
+    return r;
+}
+
+int32_t phy_fluid_get_tuning(PhyWorldHandle* w, PhyTuning* out) {
+    return guard<int32_t>(-1, [&] {
+        if (!out) throw std::invalid_argument("null out");
+        const phy::Tuning& t = need_fluid(w)->world().device().tuning();
+        out->density_list = t.density_list;
+        out->force_list = t.force_list;
+        out->split = t.split;
+        return 0;
+    });
+}
+
+int32_t phy_fluid_set_tuning(PhyWorldHandle* w, const PhyTuning* t) {
+    return guard<int32_t>(-1, [&] {
+        need_fluid(w)->world().device().set_tuning(from_c(t));
+        return 0;
+    });
+}
+
+int32_t phy_fluid_autotune(PhyWorldHandle* w, double dt, uint32_t steps, const char* cache_path,
+                           PhyTuning* out, double* ms12) {
+    return guard<int32_t>(-1, [&] {
+        FluidWorld& f = need_fluid(w)->world();
+        f.flush();
+        int32_t r = tune(f.device(), dt, steps, cache_path, out, ms12);
+        f.mark_state_changed();
+        return r;
+    });
+}
+
+int32_t phy_fluid_batch_get_tuning(PhyFluidBatch* b, PhyTuning* out) {
+    return guard<int32_t>(-1, [&] {
+        if (!out) throw std::invalid_argument("null out");
+        const phy::Tuning& t = need_batch(b)->device().tuning();
+        out->density_list = t.density_list;
+        out->force_list = t.force_list;
+        out->split = t.split;
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_set_tuning(PhyFluidBatch* b, const PhyTuning* t) {
+    return guard<int32_t>(-1, [&] {
+        need_batch(b)->device().set_tuning(from_c(t));
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_autotune(PhyFluidBatch* b, double dt, uint32_t steps, const char* cache_path,
+                                 PhyTuning* out, double* ms12) {
+    return guard<int32_t>(-1, [&] { return tune(need_batch(b)->device(), dt, steps, cache_path, out, ms12); });
+}
+
 int32_t phy_device_mem_info(uint64_t* free_bytes, uint64_t* total_bytes) {
     return guard<int32_t>(-1, [&] {
         size_t f = 0, t = 0;

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
     const float ma = pa.w, ira = sa.w, mra = va.w;  // m_a, 1/rho_a, m_a/rho_a
     const float h = P.h, h2 = P.h2;
     ForceAdjAcc A = {};
-    gather<L>(
+    gather<L, kAllWaves, 2>(
         act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, start, S.posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_maybe(P, pa.x, pa.y, pa.z, pj); },
         [&](uint32_t j) { return AdjNbr{S.posm[j], S.velm[j], Fb[j], Sb[j]}; },
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(256) void k_adj_density(SceneTable T, AdjSlot S,
         float4 p;
         float rb;
     };
-    gather<L>(
+    gather<L, kAllWaves, 2>(
         act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, start, S.posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return density_maybe(P, pa.x, pa.y, pa.z, pj); },
         [&](uint32_t j) { return DN{S.posm[j], rhob[j]}; },

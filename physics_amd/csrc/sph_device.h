@@ -290,6 +290,15 @@ struct Cursor {
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// Element idx of an array with a wave-uniform base: a 32-bit byte offset from
+// the base (arrays here hold < 2^27 elements of <= 16 B), so the load can use
+// the scalar-base + 32-bit-offset address form instead of a 64-bit per-lane
+// address.
+template <class T>
+__device__ __forceinline__ T ldg(const T* __restrict__ base, uint32_t idx) {
+    return *(const T*)((const char*)base + (idx * (uint32_t)sizeof(T)));
+}
+
 // test(j, pj) -> bool, a conservative FP32 filter (every true neighbour
 // passes; the exact decision is proc's); load(j) -> T; proc(j, T, listed)
 // accumulates (a slot that is not listed, or that proc's exact test rejects,
@@ -298,23 +307,44 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_re
 // additionally need reference key adjacency. Appends are branch-free: every
 // candidate is written at the list head and the head advances only when the
 // filter passes.
-template <int L, class Test, class Load, class Proc, class Flush>
+// Which waves a gather launch serves: every wave, only waves whose active
+// lanes all sit in one dense cell (uniform scan), or only the others (per-lane
+// scan). The forward passes launch the last two separately, so the uniform
+// kernel does not carry the per-lane cursor's registers.
+enum GatherMode { kAllWaves = 0, kUniformWaves = 1, kOtherWaves = 2 };
+
+__device__ __forceinline__ bool wave_uniform(bool act, uint32_t ci, const Domain& d) {
+    const uint32_t ci0 = uni(ci);
+    return ci0 < d.ncells && __ballot(act && ci != ci0) == 0ull;
+}
+
+// S: list slots drained per round (1 or 2; 2 keeps two neighbour loads in
+// flight, 1 needs fewer registers).
+template <int L, int MODE, int S, class Test, class Load, class Proc, class Flush>
 __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi, float zi,
                                        uint32_t self_j, const Domain& d, double inv,
                                        const uint32_t* __restrict__ start,
                                        const float4* __restrict__ posm, uint32_t (*lst)[256],
                                        int tid, Test&& test, Load&& load, Proc&& proc,
                                        Flush&& flush) {
+    static_assert(S == 1 || S == 2, "drain width");
     int cnt = 0;
     auto drain = [&]() {
-        for (int k = 0; __ballot(k < cnt) != 0ull; k += 2) {
-            const bool v0 = k < cnt, v1 = k + 1 < cnt;
-            const uint32_t j0 = v0 ? lst[k][tid] : self_j;
-            const uint32_t j1 = v1 ? lst[k + 1][tid] : self_j;
-            auto t0 = load(j0);
-            auto t1 = load(j1);
-            proc(j0, t0, v0);
-            proc(j1, t1, v1);
+        for (int k = 0; __ballot(k < cnt) != 0ull; k += S) {
+            if (S == 2) {
+                const bool v0 = k < cnt, v1 = k + 1 < cnt;
+                const uint32_t j0 = v0 ? lst[k][tid] : self_j;
+                const uint32_t j1 = v1 ? lst[k + 1][tid] : self_j;
+                auto t0 = load(j0);
+                auto t1 = load(j1);
+                proc(j0, t0, v0);
+                proc(j1, t1, v1);
+            } else {
+                const bool v0 = k < cnt;
+                const uint32_t j0 = v0 ? lst[k][tid] : self_j;
+                auto t0 = load(j0);
+                proc(j0, t0, v0);
+            }
         }
         cnt = 0;
         flush();
@@ -327,8 +357,10 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
         return keys_adjacent(xi, yi, zi, pj.x, pj.y, pj.z, inv);
     };
     const uint32_t ci0 = uni(ci);
-    const bool uniform = ci0 < d.ncells && __ballot(act && ci != ci0) == 0ull;
-    if (uniform) {
+    const bool uniform = MODE == kUniformWaves ? true
+                         : MODE == kOtherWaves ? false
+                                               : wave_uniform(act, ci, d);
+    if (MODE != kOtherWaves && uniform) {
         const CellCoord cc = decode(ci0, d);
         for (int row = 0; row < 9; ++row) {
             uint32_t a, b;
@@ -358,18 +390,32 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
             push(j, act && ov_ok(pj) && test(j, pj));
             if (__ballot(cnt == L) != 0ull) drain();
         }
-    } else {
+    } else if (MODE != kUniformWaves) {
         Cursor cur;
         cur.init(act, ci, xi, yi, zi, d, inv, start);
+        // up to four consecutive candidates of the lane's current run per
+        // round: four independent loads in flight instead of one dependent
+        // load per candidate
         while (__ballot(act && cur.j < cur.jend) != 0ull) {
             if (act && cur.j < cur.jend) {
-                const uint32_t j = cur.j;
-                const float4 pj = posm[j];
-                push(j, (!cur.in_overflow() || ov_ok(pj)) && test(j, pj));
-                ++cur.j;
+                const uint32_t j = cur.j, last = cur.jend - 1;
+                const uint32_t j1 = min(j + 1, last), j2 = min(j + 2, last), j3 = min(j + 3, last);
+                const float4 p0 = ldg(posm, j), p1 = ldg(posm, j1), p2 = ldg(posm, j2),
+                             p3 = ldg(posm, j3);
+                const bool ov = cur.in_overflow();
+                const bool t0 = (!ov || ov_ok(p0)) && test(j, p0);
+                const bool t1 = (j + 1 <= last) && (!ov || ov_ok(p1)) && test(j1, p1);
+                const bool t2 = (j + 2 <= last) && (!ov || ov_ok(p2)) && test(j2, p2);
+                const bool t3 = (j + 3 <= last) && (!ov || ov_ok(p3)) && test(j3, p3);
+                // cnt <= L - 4 here, so the four writes stay inside the list
+                push(j, t0);
+                push(j1, t1);
+                push(j2, t2);
+                push(j3, t3);
+                cur.j = min(j + 4, cur.jend);
                 if (cur.j >= cur.jend) cur.next_run(d, start);
             }
-            if (__ballot(cnt == L) != 0ull) drain();
+            if (__ballot(cnt > L - 4) != 0ull) drain();
         }
     }
     drain();

@@ -8,8 +8,23 @@
 namespace phy {
 
 constexpr int kScanBlocks = 1024;  // fixed grid of the three-phase cell scan
-constexpr int kForceList = 32;     // per-lane LDS neighbour list length of k_force
-constexpr int kDensityList = 32;   // per-lane LDS neighbour list length of k_density
+constexpr int kForceList = 16;     // default per-lane LDS neighbour list length of k_force
+constexpr int kDensityList = 16;   // default per-lane LDS neighbour list length of k_density
+
+// Launch choices of the gather kernels, picked by the autotuner. The list
+// length sets the LDS per workgroup (L x 1 KiB, hence occupancy) against the
+// drain count. The force pass's compensated sums are blocked per drain, so its
+// list length is part of the summation order; the density pass's order does
+// not depend on it.
+struct Tuning {
+    int density_list = kDensityList;  // 8, 16 or 32
+    int force_list = kForceList;      // 8, 16 or 32
+    // 1: each pass is two launches, one for waves whose lanes share a dense
+    // cell (compiled without the per-lane scan, so with fewer registers and
+    // more waves in flight) and one for the others; 0: one launch for all.
+    int split = 0;
+};
+bool tuning_valid(const Tuning& t);
 
 struct SortBuffers {
     // inputs (current order)
@@ -98,8 +113,10 @@ void launch_adjoint_step(hipStream_t st, const SceneTable& T, const AdjSlot& S, 
 void launch_domain(hipStream_t st, const SceneTable& T, const float4* posm);
 void launch_sort(hipStream_t st, const SceneTable& T, const SortBuffers& b);
 void launch_density(hipStream_t st, const SceneTable& T, const float4* posm, const uint32_t* cellv,
-                    const uint32_t* start, float* rho, float* prs, float4* velm, float* mp);
-void launch_force(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a, bool shear);
+                    const uint32_t* start, float* rho, float* prs, float4* velm, float* mp,
+                    const Tuning& tu);
+void launch_force(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a, bool shear,
+                  const Tuning& tu);
 void launch_diag(hipStream_t st, const SceneTable& T, const float4* posm, const float4* vel,
                  double* partial, double* out);
 void launch_unsort4(hipStream_t st, int n, const uint32_t* id, const float4* in, float4* out);

@@ -366,7 +366,7 @@ __global__ __launch_bounds__(256) void k_reorder(
 // rho / (rho - rho0). MI355X executes f64 FMA at the unpacked FP32 rate, and the
 // compaction limits it to true neighbours.
 
-template <int L>
+template <int L, int MODE>
 __global__ __launch_bounds__(256) void k_density(SceneTable T, const float4* __restrict__ posm,
                                                  const uint32_t* __restrict__ cellv,
                                                  const uint32_t* __restrict__ start_all,
@@ -378,20 +378,18 @@ __global__ __launch_bounds__(256) void k_density(SceneTable T, const float4* __r
     const int i = TL.i;
     const bool act = TL.act;
     const Domain d = T.dom[TL.s];
+    uint32_t ci = act ? cellv[i] : d.ncells;
+    if (MODE != kAllWaves && wave_uniform(act, ci, d) != (MODE == kUniformWaves)) return;
     const DevParams P = T.params[TL.s];
     const uint32_t* __restrict__ start = start_all + d.toff;
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
-    uint32_t ci = d.ncells;
-    if (act) {
-        pi = posm[i];
-        ci = cellv[i];
-    }
+    if (act) pi = posm[i];
     const double xi = pi.x, yi = pi.y, zi = pi.z;
     double sum = 0.0;
-    gather<L>(
+    gather<L, MODE, (MODE == kUniformWaves ? 1 : 2)>(
         act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, start, posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return density_maybe(P, pi.x, pi.y, pi.z, pj); },
-        [&](uint32_t j) { return posm[j]; },
+        [&](uint32_t j) { return ldg(posm, j); },
         [&](uint32_t j, const float4& pj, bool listed) {
             // the reference's own f64 distance decides r < h exactly (NaN fails)
             const double dx = xi - (double)pj.x, dy = yi - (double)pj.y, dz = zi - (double)pj.z;
@@ -485,7 +483,7 @@ __device__ __forceinline__ void pair_force(ForceAcc<SHEAR>& A, const DevParams& 
     }
 }
 
-template <int L, bool SHEAR>
+template <int L, bool SHEAR, int MODE>
 __global__ __launch_bounds__(256) void k_force(
     SceneTable T, float dt, const float4* __restrict__ posm, const float4* __restrict__ velm,
     const float* __restrict__ rho, const float* __restrict__ prs, const float* __restrict__ mp,
@@ -499,24 +497,24 @@ __global__ __launch_bounds__(256) void k_force(
     const int i = TL.i;
     const bool act = TL.act;
     const Domain d = T.dom[TL.s];
+    uint32_t ci = act ? cellv[i] : d.ncells;
+    if (MODE != kAllWaves && wave_uniform(act, ci, d) != (MODE == kUniformWaves)) return;
     const DevParams P = T.params[TL.s];
     const uint32_t* __restrict__ start = start_all + d.toff;
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi;
     float rho_i = 1.f, Pi = 0.f;
-    uint32_t ci = d.ncells;
     if (act) {
         pi = posm[i];
         vi = velm[i];
         rho_i = rho[i];
         Pi = prs[i] / (rho_i * rho_i);
-        ci = cellv[i];
     }
     ForceAcc<SHEAR> A;
     A.zero();
-    gather<L>(
+    gather<L, MODE, (MODE == kUniformWaves ? 1 : 2)>(
         act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, start, posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_maybe(P, pi.x, pi.y, pi.z, pj); },
-        [&](uint32_t j) { return NbrData{posm[j], velm[j], mp[j]}; },
+        [&](uint32_t j) { return NbrData{ldg(posm, j), ldg(velm, j), ldg(mp, j)}; },
         [&](uint32_t j, const NbrData& nb, bool listed) {
             pair_force<SHEAR>(A, P, (uint32_t)i, j, pi, vi, Pi, nb, listed);
         },
@@ -736,25 +734,67 @@ void launch_sort(hipStream_t st, const SceneTable& T, const SortBuffers& b) {
                            b.id_out, b.mat_out, b.body_out, b.cell_out);
 }
 
-void launch_density(hipStream_t st, const SceneTable& T, const float4* posm, const uint32_t* cellv,
-                    const uint32_t* start, float* rho, float* prs, float4* velm, float* mp) {
-    if (T.ntiles == 0) return;
-    hipLaunchKernelGGL(k_density<kDensityList>, dim3(T.ntiles), dim3(256), 0, st, T, posm, cellv,
-                       start, rho, prs, velm, mp);
+bool tuning_valid(const Tuning& t) {
+    auto ok = [](int L) { return L == 8 || L == 16 || L == 32; };
+    return ok(t.density_list) && ok(t.force_list) && (t.split == 0 || t.split == 1);
 }
 
-void launch_force(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a, bool shear) {
+template <int L>
+static void density_l(hipStream_t st, const SceneTable& T, bool split, const float4* posm,
+                      const uint32_t* cellv, const uint32_t* start, float* rho, float* prs,
+                      float4* velm, float* mp) {
+    if (split) {
+        hipLaunchKernelGGL((k_density<L, kUniformWaves>), dim3(T.ntiles), dim3(256), 0, st, T, posm,
+                           cellv, start, rho, prs, velm, mp);
+        hipLaunchKernelGGL((k_density<L, kOtherWaves>), dim3(T.ntiles), dim3(256), 0, st, T, posm,
+                           cellv, start, rho, prs, velm, mp);
+    } else {
+        hipLaunchKernelGGL((k_density<L, kAllWaves>), dim3(T.ntiles), dim3(256), 0, st, T, posm, cellv,
+                           start, rho, prs, velm, mp);
+    }
+}
+
+void launch_density(hipStream_t st, const SceneTable& T, const float4* posm, const uint32_t* cellv,
+                    const uint32_t* start, float* rho, float* prs, float4* velm, float* mp,
+                    const Tuning& tu) {
     if (T.ntiles == 0) return;
-    if (!shear)
-        hipLaunchKernelGGL((k_force<kForceList, false>), dim3(T.ntiles), dim3(256), 0, st, T, dt,
-                           a.posm, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body,
-                           a.posm_out, a.vel_out, a.id_out, a.mat_out, a.acc_out, a.mu_out,
-                           a.xsph_out, a.fvisc_out);
-    else
-        hipLaunchKernelGGL((k_force<kForceList, true>), dim3(T.ntiles), dim3(256), 0, st, T, dt,
-                           a.posm, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body,
-                           a.posm_out, a.vel_out, a.id_out, a.mat_out, a.acc_out, a.mu_out,
-                           a.xsph_out, a.fvisc_out);
+    const bool sp = tu.split != 0;
+    switch (tu.density_list) {
+        case 8: density_l<8>(st, T, sp, posm, cellv, start, rho, prs, velm, mp); break;
+        case 16: density_l<16>(st, T, sp, posm, cellv, start, rho, prs, velm, mp); break;
+        default: density_l<32>(st, T, sp, posm, cellv, start, rho, prs, velm, mp); break;
+    }
+}
+
+template <int L, bool SHEAR, int MODE>
+static void force_one(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a) {
+    hipLaunchKernelGGL((k_force<L, SHEAR, MODE>), dim3(T.ntiles), dim3(256), 0, st, T, dt, a.posm,
+                       a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body, a.posm_out,
+                       a.vel_out, a.id_out, a.mat_out, a.acc_out, a.mu_out, a.xsph_out, a.fvisc_out);
+}
+
+template <int L, bool SHEAR>
+static void force_l(hipStream_t st, const SceneTable& T, bool split, float dt, const ForceArgs& a) {
+    if (split) {
+        force_one<L, SHEAR, kUniformWaves>(st, T, dt, a);
+        force_one<L, SHEAR, kOtherWaves>(st, T, dt, a);
+    } else {
+        force_one<L, SHEAR, kAllWaves>(st, T, dt, a);
+    }
+}
+
+void launch_force(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a, bool shear,
+                  const Tuning& tu) {
+    if (T.ntiles == 0) return;
+    const bool sp = tu.split != 0;
+    switch (tu.force_list * 2 + (shear ? 1 : 0)) {
+        case 16: force_l<8, false>(st, T, sp, dt, a); break;
+        case 17: force_l<8, true>(st, T, sp, dt, a); break;
+        case 32: force_l<16, false>(st, T, sp, dt, a); break;
+        case 33: force_l<16, true>(st, T, sp, dt, a); break;
+        case 65: force_l<32, true>(st, T, sp, dt, a); break;
+        default: force_l<32, false>(st, T, sp, dt, a); break;
+    }
 }
 
 void launch_diag(hipStream_t st, const SceneTable& T, const float4* posm, const float4* vel,

@@ -117,6 +117,11 @@ class FluidWorld {
     void diagnostics(double out5[5]);
     void neighbor_stats(uint32_t* cnt, uint64_t* sum, uint64_t* xr);
     DeviceFluid& device() { return dev_; }
+    // push pending host-side particle edits to the device
+    void flush() { flush_host(); }
+    // the device state was changed outside step() (autotuner trials): the last
+    // step's watchdog no longer describes it
+    void mark_state_changed() { last_was_step_ = false; }
 
    private:
     void flush_host();  // push pending host particles to the device

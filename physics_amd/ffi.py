@@ -46,7 +46,9 @@ EXPORTS = [
     "phy_fluid_batch_sync", "phy_fluid_batch_get_field_f32", "phy_fluid_batch_diagnostics_f64",
     "phy_fluid_batch_neighbor_stats", "phy_fluid_batch_tape_begin", "phy_fluid_batch_tape_steps",
     "phy_fluid_batch_tape_backward", "phy_fluid_batch_tape_end", "phy_fluid_batch_profile_begin",
-    "phy_fluid_batch_profile_end", "phy_device_mem_info",
+    "phy_fluid_batch_profile_end", "phy_device_mem_info", "phy_fluid_get_tuning",
+    "phy_fluid_set_tuning", "phy_fluid_autotune", "phy_fluid_batch_get_tuning",
+    "phy_fluid_batch_set_tuning", "phy_fluid_batch_autotune",
 ]
 
 
@@ -89,6 +91,13 @@ class PhySphParams(C.Structure):
         q = PhySphParams()
         C.memmove(C.byref(q), C.byref(self), C.sizeof(PhySphParams))
         return q
+
+
+class PhyTuning(C.Structure):
+    """Per-lane LDS neighbour-list lengths of the density and force passes and
+    the launch form (split: uniform-cell waves and the others in two launches)."""
+
+    _fields_ = [("density_list", C.c_int32), ("force_list", C.c_int32), ("split", C.c_int32)]
 
 
 _lib = None
@@ -187,6 +196,14 @@ def lib():
         "phy_fluid_batch_profile_begin": ([H], C.c_int32),
         "phy_fluid_batch_profile_end": ([H, dp, u32p], C.c_int32),
         "phy_device_mem_info": ([u64p, u64p], C.c_int32),
+        "phy_fluid_get_tuning": ([H, C.POINTER(PhyTuning)], C.c_int32),
+        "phy_fluid_set_tuning": ([H, C.POINTER(PhyTuning)], C.c_int32),
+        "phy_fluid_autotune": ([H, C.c_double, C.c_uint32, C.c_char_p, C.POINTER(PhyTuning), dp],
+                               C.c_int32),
+        "phy_fluid_batch_get_tuning": ([H, C.POINTER(PhyTuning)], C.c_int32),
+        "phy_fluid_batch_set_tuning": ([H, C.POINTER(PhyTuning)], C.c_int32),
+        "phy_fluid_batch_autotune": ([H, C.c_double, C.c_uint32, C.c_char_p, C.POINTER(PhyTuning), dp],
+                                     C.c_int32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -361,6 +378,21 @@ class FluidWorld(World):
         if lib().phy_fluid_set_graph(self._h, 1 if on else 0) != 0:
             raise PhyError(last_error())
 
+    def tuning(self):
+        t = PhyTuning()
+        if lib().phy_fluid_get_tuning(self._h, C.byref(t)) != 0:
+            raise PhyError(last_error())
+        return int(t.density_list), int(t.force_list), int(t.split)
+
+    def set_tuning(self, density_list: int, force_list: int, split: int = 0) -> None:
+        t = PhyTuning(density_list, force_list, split)
+        if lib().phy_fluid_set_tuning(self._h, C.byref(t)) != 0:
+            raise PhyError(last_error())
+
+    def autotune(self, dt: float, steps: int = 5, cache_path: str | None = None):
+        """-> (cached?, (density_list, force_list, split), ms[12] of the timed candidates)."""
+        return _autotune(lib().phy_fluid_autotune, self._h, dt, steps, cache_path)
+
     def tape_begin(self, checkpoint_every: int) -> None:
         """Start recording a rollout for reverse-mode gradients."""
         if lib().phy_fluid_tape_begin(self._h, int(checkpoint_every)) != 0:
@@ -399,6 +431,16 @@ class FluidWorld(World):
         if rc != 0:
             raise PhyError(last_error())
         return c, s, x
+
+
+def _autotune(fn, h, dt, steps, cache_path):
+    t = PhyTuning()
+    ms = np.full(12, np.nan)
+    path = None if cache_path is None else os.fsencode(cache_path)
+    rc = fn(h, dt, int(steps), path, C.byref(t), _dp(ms))
+    if rc < 0:
+        raise PhyError(last_error())
+    return bool(rc == 1), (int(t.density_list), int(t.force_list), int(t.split)), ms
 
 
 def device_mem_info():
@@ -486,6 +528,18 @@ class FluidBatch:
 
     def set_keep_aux(self, on: bool = True) -> None:
         self._ok(lib().phy_fluid_batch_set_keep_aux(self._h, 1 if on else 0))
+
+    def tuning(self):
+        t = PhyTuning()
+        self._ok(lib().phy_fluid_batch_get_tuning(self._h, C.byref(t)))
+        return int(t.density_list), int(t.force_list), int(t.split)
+
+    def set_tuning(self, density_list: int, force_list: int, split: int = 0) -> None:
+        t = PhyTuning(density_list, force_list, split)
+        self._ok(lib().phy_fluid_batch_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, dt: float, steps: int = 5, cache_path: str | None = None):
+        return _autotune(lib().phy_fluid_batch_autotune, self._h, dt, steps, cache_path)
 
     def sync(self) -> None:
         self._ok(lib().phy_fluid_batch_sync(self._h))
