@@ -86,8 +86,13 @@ uintptr_t phy_world_get_fluid_velocities(PhyWorldHandle *w, double *buf, uintptr
 uintptr_t phy_world_rigid_count(PhyWorldHandle *w);
 uintptr_t phy_world_get_rigid_transforms(PhyWorldHandle *w, double *buf, uintptr_t len);
 
-/* JSON save / load (phy-io archive). Not provided by this library: -1 / NULL,
- * the documented failure values. Replaces lib.rs:517-542 / phy_ffi.h:112, 115. */
+/* JSON save / load of the phy-io world archive (crates/phy-io/src/archive.rs:
+ * version, time, tagged subsystems; f64 scalars as {"__f64": bits}, Vec3 as
+ * u64 bit patterns). Save writes every fluid subsystem (other kinds are not
+ * hosted here); load rebuilds the world and returns NULL on malformed input, a
+ * non-finite scalar or a non-fluid subsystem. Positions and velocities load
+ * rounded to FP32. 0 / -1 and handle / NULL. Replaces lib.rs:517-542 /
+ * phy_ffi.h:112, 115. */
 int32_t phy_world_save(PhyWorldHandle *w, const char *path);
 PhyWorldHandle *phy_world_load(const char *path);
 
@@ -270,6 +275,33 @@ int32_t phy_fluid_set_tuning(PhyWorldHandle *w, const PhyTuning *t);
  * in one launch, then in two, then force in the same order. -1 on error. */
 int32_t phy_fluid_autotune(PhyWorldHandle *w, double dt, uint32_t steps, const char *cache_path,
                            PhyTuning *out, double *ms12);
+
+/* Flat export: FluidWorld::to_gpu_flat over Grid::to_flat (world.rs:782-859,
+ * grid.rs:62-175, SphFlatData in sph/gpu_flat.rs:14-30), the reference's
+ * hand-off format for GPU consumers, with the grid built from the current
+ * positions (as after build_grid). Cells: the occupied key box padded to
+ * [min - 1, max + 2] per axis, x fastest (c = i + nc0 (j + nc1 k) relative to
+ * grid_min); cell_start: ncells + 1 prefix sums; sorted: particle indices,
+ * ascending inside a cell. pos4 / vel4: (x, y, z, 0); scalar4: (rho, p, mass,
+ * material); 4 floats per particle. info is always filled; the arrays are
+ * written and 0 returned when n_cap >= n and cell_cap >= ncells + 1, else 1
+ * (call again with larger buffers); -1 on error. An empty world has one cell
+ * and cell_start = {0, 0}. */
+typedef struct PhyFlatInfo {
+    uint64_t n;
+    uint64_t ncells;
+    int64_t grid_min[3];
+    uint64_t nc[3];
+    float h, rest_density, stiffness, shear_min;
+    float gravity[3];
+    uint32_t n_visc;
+    float visc_k[PHY_MAX_MATERIALS];
+    float visc_n[PHY_MAX_MATERIALS];
+} PhyFlatInfo;
+
+int32_t phy_fluid_flat_export(PhyWorldHandle *w, PhyFlatInfo *info, float *pos4, float *vel4,
+                              float *scalar4, uintptr_t n_cap, int32_t *cell_start,
+                              uintptr_t cell_cap, int32_t *sorted);
 
 /* Device memory of the current HIP device (hipMemGetInfo): free and total
  * bytes. 0 ok, -1 error. */

@@ -125,6 +125,9 @@ def lib():
         vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.or_world_time_steps.argtypes = [vp, C.c_double, C.c_int]
     L.or_world_time_steps.restype = C.c_double
+    L.or_world_flat_grid.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
+                                     C.POINTER(C.c_int32), C.c_uint64, C.POINTER(C.c_int32), C.c_uint64]
+    L.or_world_flat_grid.restype = C.c_int
     assert L.or_params_size() == C.sizeof(Params), "Params layout mismatch"
     _lib = L
     return L
@@ -260,4 +263,18 @@ class World:
 
     def time_steps(self, dt: float, n: int) -> float:
         return lib().or_world_time_steps(self._h, dt, int(n))
+
+    def flat_grid(self):
+        """Grid::to_flat of the current positions (grid.rs:62-175):
+        (grid_min[3], nc[3], cell_start[ncells + 1], sorted[n])."""
+        mn = (C.c_int64 * 3)()
+        nc = (C.c_uint64 * 3)()
+        lib().or_world_flat_grid(self._h, mn, nc, None, 0, None, 0)
+        ncell = nc[0] * nc[1] * nc[2]
+        cs = np.zeros(ncell + 1, dtype=np.int32)
+        so = np.zeros(len(self), dtype=np.int32)
+        rc = lib().or_world_flat_grid(self._h, mn, nc, cs.ctypes.data_as(C.POINTER(C.c_int32)), cs.size,
+                                      so.ctypes.data_as(C.POINTER(C.c_int32)), so.size)
+        assert rc == 0
+        return np.array(list(mn), dtype=np.int64), np.array(list(nc), dtype=np.uint64), cs, so
 

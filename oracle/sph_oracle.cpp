@@ -29,6 +29,7 @@
 // A run-time "mutation" switch plants known mistakes so the tests can show that
 // each check actually fails on a plausible bug.
 
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -538,6 +539,66 @@ void or_world_neighbor_stats(void* w, uint32_t* counts, uint64_t* sum_ids, uint6
         sum_ids[i] = s;
         xor_hash[i] = x;
     }
+}
+
+// Grid::build on the current positions, then Grid::to_flat
+// (crates/phy-fluid/src/sph/grid.rs:62-175): occupied-key box padded to
+// [min - 1, max + 2] per axis, x-fastest flat index, prefix-sum cell starts,
+// particle ids of each cell in ascending order. Fills min3 / nc3 always;
+// fills the arrays and returns 0 when the capacities suffice, else returns 1.
+int or_world_flat_grid(void* wv, int64_t* min3, uint64_t* nc3, int32_t* cell_start, uint64_t cell_cap,
+                       int32_t* sorted, uint64_t n_cap) {
+    World* w = (World*)wv;
+    Grid g(w->params.h);
+    g.build(w->particles);
+    int64_t mn[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, mx[3] = {INT64_MIN, INT64_MIN, INT64_MIN};
+    for (const auto& kv : g.map) {
+        if (kv.second.empty()) continue;
+        const int64_t k[3] = {kv.first.i, kv.first.j, kv.first.k};
+        for (int a = 0; a < 3; ++a) {
+            mn[a] = std::min(mn[a], k[a]);
+            mx[a] = std::max(mx[a], k[a]);
+        }
+    }
+    if (mn[0] > mx[0]) {  // empty grid
+        for (int a = 0; a < 3; ++a) {
+            min3[a] = 0;
+            nc3[a] = 1;
+        }
+        if (cell_cap < 2) return 1;
+        cell_start[0] = 0;
+        cell_start[1] = 0;  // (the reference's vec![0] has one entry; +1 sentinel here)
+        return 0;
+    }
+    uint64_t nc[3];
+    for (int a = 0; a < 3; ++a) {
+        mn[a] = mn[a] - 1;
+        nc[a] = (uint64_t)(mx[a] - mn[a] + 3);
+        min3[a] = mn[a];
+        nc3[a] = nc[a];
+    }
+    const uint64_t ncell = nc[0] * nc[1] * nc[2];
+    const uint64_t n = w->particles.size();
+    if (cell_cap < ncell + 1 || n_cap < n) return 1;
+    auto flat = [&](const Key& k) {
+        return (uint64_t)(k.i - mn[0]) + nc[0] * ((uint64_t)(k.j - mn[1]) + nc[1] * (uint64_t)(k.k - mn[2]));
+    };
+    std::vector<uint32_t> counts(ncell, 0);
+    for (const auto& kv : g.map)
+        if (!kv.second.empty()) counts[flat(kv.first)] += (uint32_t)kv.second.size();
+    int32_t acc = 0;
+    for (uint64_t c = 0; c < ncell; ++c) {
+        cell_start[c] = acc;
+        acc += (int32_t)counts[c];
+    }
+    cell_start[ncell] = acc;
+    std::vector<int32_t> cursor(cell_start, cell_start + ncell);
+    for (const auto& kv : g.map) {
+        if (kv.second.empty()) continue;
+        const uint64_t c = flat(kv.first);
+        for (size_t id : kv.second) sorted[cursor[c]++] = (int32_t)id;
+    }
+    return 0;
 }
 
 // Wall-clock of n steps (seconds), for the CPU baseline.

@@ -17,7 +17,8 @@ COPY_EXE = os.path.join(HERE, "hbm_copy")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["sph_kernels.hip", "sph_adjoint.hip", "fluid_device.cpp", "world.cpp", "phy_ffi.cpp"]
+SOURCES = ["sph_kernels.hip", "sph_adjoint.hip", "fluid_device.cpp", "world.cpp", "archive.cpp",
+           "phy_ffi.cpp"]
 
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 generation.
 COMMON = [

@@ -887,6 +887,15 @@ void DeviceFluid::last_validate_all(int32_t* status) {
     }
 }
 
+void DeviceFluid::download_material(uint32_t* out) {
+    const int n = (int)n_;
+    if (n == 0) return;
+    // a 4-byte gather: the float unsort moves the tags' bits unchanged
+    launch_unsort1(st_, n, id_.get(), reinterpret_cast<const float*>(mat_.get()), stage1_.get());
+    hip_check(hipMemcpyAsync(out, stage1_.get(), n * sizeof(uint32_t), hipMemcpyDeviceToHost, st_), "download");
+    sync();
+}
+
 int DeviceFluid::last_validate() {
     std::vector<int32_t> st(scn_.size());
     last_validate_all(st.data());

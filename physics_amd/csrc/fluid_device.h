@@ -129,6 +129,8 @@ class DeviceFluid {
     // floats per particle for vector fields, 1 otherwise). Auxiliary fields
     // (acc, xsph, mu_eff) hold the last step's values only while keep_aux is on.
     void download(Field f, float* out);
+    // material tags in (concatenated) original index order
+    void download_material(uint32_t* out);
 
     void set_keep_aux(bool on);
     bool keep_aux() const { return keep_aux_; }

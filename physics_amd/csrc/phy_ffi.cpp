@@ -6,9 +6,11 @@
 #include "../../include/phy_ffi.h"
 
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <string>
 
+#include "archive.h"
 #include "world.h"
 
 using phy::FluidBatch;
@@ -288,15 +290,34 @@ int32_t phy_world_rigid_get_angular_velocity(PhyWorldHandle* w, int64_t id, doub
     return -1;
 }
 
+// lib.rs:517-530: the phy-io JSON archive of the world, written to path.
 int32_t phy_world_save(PhyWorldHandle* w, const char* path) {
-    (void)w; (void)path;
-    set_err("phy_world_save: the JSON archive is not provided by this library");
-    return -1;
+    return guard<int32_t>(-1, [&] {
+        if (!w) throw std::invalid_argument("null world handle");
+        if (!path) throw std::invalid_argument("null path");
+        const std::string json = phy::save_world_json(*as_world(w));
+        std::FILE* f = std::fopen(path, "wb");
+        if (!f) throw std::runtime_error(std::string("phy_world_save: cannot open ") + path);
+        const size_t wr = std::fwrite(json.data(), 1, json.size(), f);
+        const int rc = std::fclose(f);
+        if (wr != json.size() || rc != 0) throw std::runtime_error(std::string("phy_world_save: write failed: ") + path);
+        return 0;
+    });
 }
+
+// lib.rs:532-542: a new world from an archive file; NULL on any failure.
 PhyWorldHandle* phy_world_load(const char* path) {
-    (void)path;
-    set_err("phy_world_load: the JSON archive is not provided by this library");
-    return nullptr;
+    return guard<PhyWorldHandle*>(nullptr, [&] {
+        if (!path) throw std::invalid_argument("null path");
+        std::FILE* f = std::fopen(path, "rb");
+        if (!f) throw std::runtime_error(std::string("phy_world_load: cannot open ") + path);
+        std::string json;
+        char buf[1 << 16];
+        size_t r;
+        while ((r = std::fread(buf, 1, sizeof buf, f)) > 0) json.append(buf, r);
+        std::fclose(f);
+        return box_world(phy::load_world_json(json).release());
+    });
 }
 
 void phy_world_destroy(PhyWorldHandle* w) {
@@ -575,6 +596,41 @@ int32_t phy_fluid_batch_set_tuning(PhyFluidBatch* b, const PhyTuning* t) {
 int32_t phy_fluid_batch_autotune(PhyFluidBatch* b, double dt, uint32_t steps, const char* cache_path,
                                  PhyTuning* out, double* ms12) {
     return guard<int32_t>(-1, [&] { return tune(need_batch(b)->device(), dt, steps, cache_path, out, ms12); });
+}
+
+int32_t phy_fluid_flat_export(PhyWorldHandle* w, PhyFlatInfo* info, float* pos4, float* vel4,
+                              float* scalar4, uintptr_t n_cap, int32_t* cell_start, uintptr_t cell_cap,
+                              int32_t* sorted) {
+    return guard<int32_t>(-1, [&] {
+        if (!info) throw std::invalid_argument("null info");
+        FluidWorld& f = need_fluid(w)->world();
+        const phy::SphParams& p = f.params();
+        std::memset(info, 0, sizeof(*info));
+        info->h = (float)p.h;
+        info->rest_density = (float)p.rest_density;
+        info->stiffness = (float)p.stiffness;
+        info->shear_min = (float)p.shear_min;
+        info->gravity[0] = (float)p.gravity.x;
+        info->gravity[1] = (float)p.gravity.y;
+        info->gravity[2] = (float)p.gravity.z;
+        info->n_visc = (uint32_t)std::min<size_t>(p.visc_k.size(), PHY_MAX_MATERIALS);
+        for (uint32_t m = 0; m < info->n_visc; ++m) info->visc_k[m] = (float)p.visc_k[m];
+        for (size_t m = 0; m < p.visc_n.size() && m < PHY_MAX_MATERIALS; ++m) info->visc_n[m] = (float)p.visc_n[m];
+        const bool room = n_cap > 0 || cell_cap > 0;
+        FluidWorld::FlatShape sh;
+        bool filled = false;
+        if (room && pos4 && vel4 && scalar4 && cell_start && sorted)
+            filled = f.flat_export(sh, pos4, vel4, scalar4, n_cap, cell_start, cell_cap, sorted);
+        else
+            f.flat_export(sh, nullptr, nullptr, nullptr, 0, nullptr, 0, nullptr);
+        info->n = sh.n;
+        info->ncells = sh.ncells;
+        for (int a = 0; a < 3; ++a) {
+            info->grid_min[a] = sh.grid_min[a];
+            info->nc[a] = sh.nc[a];
+        }
+        return filled ? 0 : 1;
+    });
 }
 
 int32_t phy_device_mem_info(uint64_t* free_bytes, uint64_t* total_bytes) {

@@ -1,6 +1,11 @@
 // world.cpp — host mirror implementation (see world.h for the reference map).
 #include "world.h"
 
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <stdexcept>
+
 #include <cmath>
 #include <stdexcept>
 
@@ -146,6 +151,14 @@ std::vector<Particle> FluidWorld::particles() {
     put3(&Particle::vel);
     dev_.download(F_MASS, s.data());
     for (size_t i = 0; i < n; ++i) out[i].mass = s[i];
+    {
+        std::vector<uint32_t> m(n);
+        dev_.download_material(m.data());
+        for (size_t i = 0; i < n; ++i) out[i].material = m[i];
+    }
+    if (body_pending_.size() == 3 * n)
+        for (size_t i = 0; i < n; ++i)
+            out[i].body_acc = Vec3{body_pending_[3 * i], body_pending_[3 * i + 1], body_pending_[3 * i + 2]};
     if (dev_.density_valid()) {
         dev_.download(F_RHO, s.data());
         for (size_t i = 0; i < n; ++i) out[i].rho = s[i];
@@ -367,6 +380,82 @@ double FluidSubsystem::kinetic_energy() {
 }
 
 // ---------------------------------------------------------------------------
+
+// to_i64 of the floored key (grid.rs:26-33): NaN / out of range -> 0
+static long long flat_key(double x, double inv) {
+    double f = std::floor(x * inv);
+    if (f >= -9223372036854775808.0 && f < 9223372036854775808.0) return (long long)f;
+    return 0;
+}
+
+bool FluidWorld::flat_export(FlatShape& sh, float* pos4, float* vel4, float* scalar4, uint64_t n_cap,
+                             int32_t* cell_start, uint64_t cell_cap, int32_t* sorted) {
+    const std::vector<Particle> ps = particles();
+    const uint64_t n = ps.size();
+    const double inv = 1.0 / params_.h;  // T::one() / self.cell
+    std::vector<long long> keys(3 * n);
+    long long mn[3] = {LLONG_MAX, LLONG_MAX, LLONG_MAX}, mx[3] = {LLONG_MIN, LLONG_MIN, LLONG_MIN};
+    for (uint64_t i = 0; i < n; ++i) {
+        const double c[3] = {ps[i].pos.x, ps[i].pos.y, ps[i].pos.z};
+        for (int a = 0; a < 3; ++a) {
+            const long long k = flat_key(c[a], inv);
+            keys[3 * i + a] = k;
+            mn[a] = std::min(mn[a], k);
+            mx[a] = std::max(mx[a], k);
+        }
+    }
+    sh.n = n;
+    if (n == 0) {  // empty grid: one cell, origin 0
+        for (int a = 0; a < 3; ++a) {
+            sh.grid_min[a] = 0;
+            sh.nc[a] = 1;
+        }
+        sh.ncells = 1;
+        if (cell_cap < 2) return false;
+        cell_start[0] = cell_start[1] = 0;
+        return true;
+    }
+    // occupied-key box padded to [min - 1, max + 2] (grid.rs:107-114)
+    double vol = 1.0;
+    for (int a = 0; a < 3; ++a) {
+        sh.grid_min[a] = mn[a] - 1;
+        sh.nc[a] = (uint64_t)(mx[a] - sh.grid_min[a] + 3);
+        vol *= (double)sh.nc[a];
+    }
+    if (!(vol < 2147483647.0)) throw std::runtime_error("flat export: more than 2^31 cells in the key box");
+    sh.ncells = sh.nc[0] * sh.nc[1] * sh.nc[2];
+    if (n_cap < n || cell_cap < sh.ncells + 1) return false;
+    // x fastest: c = (i - min_i) + ncx ((j - min_j) + ncy (k - min_k))
+    std::vector<uint32_t> cell(n);
+    std::vector<int32_t> counts(sh.ncells + 1, 0);
+    for (uint64_t i = 0; i < n; ++i) {
+        const uint64_t ci = (uint64_t)(keys[3 * i] - sh.grid_min[0]);
+        const uint64_t cj = (uint64_t)(keys[3 * i + 1] - sh.grid_min[1]);
+        const uint64_t ck = (uint64_t)(keys[3 * i + 2] - sh.grid_min[2]);
+        cell[i] = (uint32_t)(ci + sh.nc[0] * (cj + sh.nc[1] * ck));
+        ++counts[cell[i]];
+    }
+    int32_t acc = 0;
+    for (uint64_t c = 0; c < sh.ncells; ++c) {
+        cell_start[c] = acc;
+        acc += counts[c];
+    }
+    cell_start[sh.ncells] = acc;
+    std::vector<int32_t> cursor(cell_start, cell_start + sh.ncells);
+    for (uint64_t i = 0; i < n; ++i) sorted[cursor[cell[i]]++] = (int32_t)i;  // ascending ids per cell
+    for (uint64_t i = 0; i < n; ++i) {
+        const Particle& q = ps[i];
+        const float p4[4] = {(float)q.pos.x, (float)q.pos.y, (float)q.pos.z, 0.f};
+        const float v4[4] = {(float)q.vel.x, (float)q.vel.y, (float)q.vel.z, 0.f};
+        const float s4[4] = {(float)q.rho, (float)q.p, (float)q.mass, (float)q.material};
+        for (int a = 0; a < 4; ++a) {
+            pos4[4 * i + a] = p4[a];
+            vel4[4 * i + a] = v4[a];
+            scalar4[4 * i + a] = s4[a];
+        }
+    }
+    return true;
+}
 
 // ---------------------------------------------------------------------------
 

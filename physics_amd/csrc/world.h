@@ -117,6 +117,19 @@ class FluidWorld {
     void diagnostics(double out5[5]);
     void neighbor_stats(uint32_t* cnt, uint64_t* sum, uint64_t* xr);
     DeviceFluid& device() { return dev_; }
+
+    // FluidWorld::to_gpu_flat over Grid::to_flat (world.rs:782-859,
+    // grid.rs:62-175), with the grid built from the current positions (as
+    // after build_grid). Fills the shape always; fills the arrays (pos4, vel4,
+    // scalar4 = (rho, p, mass, material): 4 floats per particle; sorted: n;
+    // cell_start: ncells + 1) and returns true when the capacities suffice.
+    struct FlatShape {
+        uint64_t n, ncells;
+        int64_t grid_min[3];
+        uint64_t nc[3];
+    };
+    bool flat_export(FlatShape& shape, float* pos4, float* vel4, float* scalar4, uint64_t n_cap,
+                     int32_t* cell_start, uint64_t cell_cap, int32_t* sorted);
     // push pending host-side particle edits to the device
     void flush() { flush_host(); }
     // the device state was changed outside step() (autotuner trials): the last

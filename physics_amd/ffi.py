@@ -48,7 +48,7 @@ EXPORTS = [
     "phy_fluid_batch_tape_backward", "phy_fluid_batch_tape_end", "phy_fluid_batch_profile_begin",
     "phy_fluid_batch_profile_end", "phy_device_mem_info", "phy_fluid_get_tuning",
     "phy_fluid_set_tuning", "phy_fluid_autotune", "phy_fluid_batch_get_tuning",
-    "phy_fluid_batch_set_tuning", "phy_fluid_batch_autotune",
+    "phy_fluid_batch_set_tuning", "phy_fluid_batch_autotune", "phy_fluid_flat_export",
 ]
 
 
@@ -91,6 +91,17 @@ class PhySphParams(C.Structure):
         q = PhySphParams()
         C.memmove(C.byref(q), C.byref(self), C.sizeof(PhySphParams))
         return q
+
+
+class PhyFlatInfo(C.Structure):
+    """Shape and parameters of a flat export (SphFlatData, sph/gpu_flat.rs:14-30)."""
+
+    _fields_ = [
+        ("n", C.c_uint64), ("ncells", C.c_uint64), ("grid_min", C.c_int64 * 3), ("nc", C.c_uint64 * 3),
+        ("h", C.c_float), ("rest_density", C.c_float), ("stiffness", C.c_float), ("shear_min", C.c_float),
+        ("gravity", C.c_float * 3), ("n_visc", C.c_uint32), ("visc_k", C.c_float * MAX_MAT),
+        ("visc_n", C.c_float * MAX_MAT),
+    ]
 
 
 class PhyTuning(C.Structure):
@@ -204,6 +215,8 @@ def lib():
         "phy_fluid_batch_set_tuning": ([H, C.POINTER(PhyTuning)], C.c_int32),
         "phy_fluid_batch_autotune": ([H, C.c_double, C.c_uint32, C.c_char_p, C.POINTER(PhyTuning), dp],
                                      C.c_int32),
+        "phy_fluid_flat_export": ([H, C.POINTER(PhyFlatInfo), fp, fp, fp, up, C.POINTER(C.c_int32), up,
+                                   C.POINTER(C.c_int32)], C.c_int32),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -237,6 +250,19 @@ class World:
     def create_fluid(cls) -> "World":
         """``phy_world_create_fluid`` (crates/phy-ffi/src/lib.rs:113-126)."""
         return cls(lib().phy_world_create_fluid())
+
+    @classmethod
+    def load(cls, path: str) -> "World":
+        """``phy_world_load``: a world from a phy-io JSON archive (lib.rs:532-542)."""
+        return cls(lib().phy_world_load(os.fsencode(path)))
+
+    def save(self, path: str) -> None:
+        """``phy_world_save``: write the phy-io JSON archive (lib.rs:517-530)."""
+        if lib().phy_world_save(self._h, os.fsencode(path)) != 0:
+            raise PhyError(last_error())
+
+    def sub_count(self) -> int:
+        return int(lib().phy_world_sub_count(self._h))
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -392,6 +418,31 @@ class FluidWorld(World):
     def autotune(self, dt: float, steps: int = 5, cache_path: str | None = None):
         """-> (cached?, (density_list, force_list, split), ms[12] of the timed candidates)."""
         return _autotune(lib().phy_fluid_autotune, self._h, dt, steps, cache_path)
+
+    def flat_export(self):
+        """-> dict of SphFlatData fields (pos, vel, scalar: (n, 4) float32;
+        cell_start, sorted: int32; grid_min, nc; parameters)."""
+        info = PhyFlatInfo()
+        rc = lib().phy_fluid_flat_export(self._h, C.byref(info), None, None, None, 0, None, 0, None)
+        if rc < 0:
+            raise PhyError(last_error())
+        n, nc = int(info.n), int(info.ncells)
+        pos = np.zeros((n, 4), np.float32)
+        vel = np.zeros((n, 4), np.float32)
+        sca = np.zeros((n, 4), np.float32)
+        cs = np.zeros(nc + 1, np.int32)
+        so = np.zeros(max(n, 1), np.int32)
+        i32 = C.POINTER(C.c_int32)
+        rc = lib().phy_fluid_flat_export(self._h, C.byref(info), _fp(pos), _fp(vel), _fp(sca), max(n, 1),
+                                         cs.ctypes.data_as(i32), cs.size, so.ctypes.data_as(i32))
+        if rc != 0:
+            raise PhyError(f"flat export: rc {rc}: {last_error()}")
+        return {"n": n, "pos": pos, "vel": vel, "scalar": sca, "cell_start": cs, "sorted": so[:n],
+                "grid_min": np.array(list(info.grid_min)), "nc": np.array(list(info.nc)),
+                "h": info.h, "rest_density": info.rest_density, "stiffness": info.stiffness,
+                "shear_min": info.shear_min, "gravity": np.array(list(info.gravity)),
+                "visc_k": np.array(list(info.visc_k)[:info.n_visc]),
+                "visc_n": np.array(list(info.visc_n)[:info.n_visc])}
 
     def tape_begin(self, checkpoint_every: int) -> None:
         """Start recording a rollout for reverse-mode gradients."""

@@ -57,8 +57,47 @@ def test_out_of_scope_factories_return_null():
                  "phy_world_create_optics_empty", "phy_world_create_solid_empty"]:
         assert getattr(L, name)() is None
         assert "not provided" in ffi.last_error()
-    assert L.phy_world_save(None, b"/tmp/x.json") == -1
-    assert L.phy_world_load(b"/tmp/x.json") is None
+
+
+def test_archive_error_paths(tmp_path):
+    """phy_world_save / phy_world_load fail with their documented values, and
+    the archive parser rejects what the reference's loader rejects, before any
+    world (GPU) is created."""
+    L = ffi.lib()
+    assert L.phy_world_save(None, str(tmp_path / "x.json").encode()) == -1
+    assert "null world" in ffi.last_error()
+    assert L.phy_world_load(str(tmp_path / "missing.json").encode()) is None
+    assert "cannot open" in ffi.last_error()
+    cases = {
+        "truncated": '{"version": 1, "time": {"__f64": 0}, "subsystems": [',
+        "not json": "hello",
+        "rigid": '{"version": 1, "time": {"__f64": 0}, "subsystems": [{"Rigid": {}}]}',
+        "nan time": '{"version": 1, "time": {"__f64": %d}, "subsystems": []}' % 0x7FF8000000000000,
+        "missing field": '{"version": 1, "subsystems": []}',
+    }
+    for name, text in cases.items():
+        f = tmp_path / f"{name.replace(' ', '_')}.json"
+        f.write_text(text)
+        assert L.phy_world_load(str(f).encode()) is None, name
+        assert "archive" in ffi.last_error(), (name, ffi.last_error())
+
+
+def test_archive_fixture_layout():
+    """The committed fixture follows the reference layout (tests/golden/
+    make_archive_fixture.py): tagged subsystems, {"__f64": bits} scalars, u64
+    bit-pattern vectors, integer material tags."""
+    import json, struct
+    d = json.load(open(os.path.join(ROOT, "tests", "golden", "fluid_archive_small.json")))
+    assert d["version"] == 1 and list(d["subsystems"][0]) == ["Fluid"]
+    fl = d["subsystems"][0]["Fluid"]
+    assert list(fl) == ["world", "thermal_expansion", "heat_gain", "t_ref", "drag", "friction"]
+    p = fl["world"]["params"]
+    assert list(p) == ["rest_density", "stiffness", "viscosity", "visc_k", "visc_n", "shear_min",
+                       "mass", "h", "gravity", "bounds_min", "bounds_max", "boundary_damp", "xsph_eps"]
+    q = fl["world"]["particles"][0]
+    assert list(q) == ["pos", "vel", "acc", "body_acc", "rho", "p", "mass", "material"]
+    h = struct.unpack("<d", struct.pack("<Q", p["h"]["__f64"]))[0]
+    assert h == 0.2 and isinstance(q["material"], int) and len(q["pos"]) == 3
 
 
 def test_params_defaults_match_reference():
