@@ -221,14 +221,25 @@ int32_t phy_world_step_n(PhyWorldHandle *w, double dt, uint32_t n);
  * and writes the cotangents of the initial positions / velocities (3 per
  * particle, may be NULL) and dparams[0] = dL/d stiffness, dparams[1] =
  * dL/d visc_k[0] (FP64 device reductions). Segments are recomputed from their
- * checkpoints; the world state is unchanged. One Newtonian material only
- * (visc_n = [1]); no body_acc. Returns 0 ok, -1 error. Replacing the particles
+ * checkpoints; the world state is unchanged. Any material table and power-law
+ * exponents; no body_acc. Returns 0 ok, -1 error. Replacing the particles
  * while recording invalidates the tape. */
 int32_t phy_fluid_tape_begin(PhyWorldHandle *w, uint32_t checkpoint_every);
 uint32_t phy_fluid_tape_steps(PhyWorldHandle *w);
 int32_t phy_fluid_tape_backward(PhyWorldHandle *w, const double *xbar_final, const double *vbar_final,
                                 const double *rhobar_last, double *xbar0, double *vbar0,
                                 double *dparams);
+
+/* Parameter gradients of the full material table per scene. */
+#define PHY_TAPE_PARAMS (1 + 2 * PHY_MAX_MATERIALS)
+/* As phy_fluid_tape_backward, with dparams holding PHY_TAPE_PARAMS values:
+ * [0] dL/d stiffness, [1 + m] dL/d visc_k[m], [1 + PHY_MAX_MATERIALS + m]
+ * dL/d visc_n[m] for m < PHY_MAX_MATERIALS (zero for m >= n_visc_k). The
+ * visc_n gradient is defined for Newtonian materials too (visc_n = 1):
+ * dmu/dn = mu ln(max(shear, shear_min)). */
+int32_t phy_fluid_tape_backward_full(PhyWorldHandle *w, const double *xbar_final,
+                                     const double *vbar_final, const double *rhobar_last,
+                                     double *xbar0, double *vbar0, double *dparams);
 int32_t phy_fluid_tape_end(PhyWorldHandle *w);
 
 /* Steps are asynchronous on the world's HIP stream; block until the GPU has
@@ -359,6 +370,11 @@ uint32_t phy_fluid_batch_tape_steps(PhyFluidBatch *b);
 int32_t phy_fluid_batch_tape_backward(PhyFluidBatch *b, const float *xbar_final,
                                       const float *vbar_final, const float *rhobar_last,
                                       float *xbar0, float *vbar0, double *dparams);
+/* As phy_fluid_batch_tape_backward with PHY_TAPE_PARAMS values per scene
+ * (layout of phy_fluid_tape_backward_full). */
+int32_t phy_fluid_batch_tape_backward_full(PhyFluidBatch *b, const float *xbar_final,
+                                           const float *vbar_final, const float *rhobar_last,
+                                           float *xbar0, float *vbar0, double *dparams);
 int32_t phy_fluid_batch_tape_end(PhyFluidBatch *b);
 
 /* As phy_fluid_get_tuning / set_tuning / autotune, for a whole batch. */

@@ -461,7 +461,7 @@ void DeviceFluid::step_core(float dt, const StepBufs& s) {
     a.xsph_out = s.xsph;
     a.fvisc_out = s.fvisc;
     if (prof_) ev();
-    launch_force(st_, T, dt, a, any_shear_, tuning_);
+    launch_force(st_, T, dt, a, any_shear_ || taping_, tuning_);
     if (prof_) ev();
     hip_check(hipGetLastError(), "step launch");
 }
@@ -665,12 +665,10 @@ void DeviceFluid::load_state(const Snapshot& c) {
 
 void DeviceFluid::tape_begin(uint32_t K) {
     if (K == 0) throw std::invalid_argument("tape: checkpoint interval must be >= 1");
-    for (const DevParams& P : Ps_)
-        if (!P.newtonian || P.n_mat != 1)
-            throw std::invalid_argument("tape: the adjoint supports one Newtonian material (visc_n = [1])");
     if (has_body_) throw std::invalid_argument("tape: body_acc is not differentiated");
     tape_end();
     taping_ = true;
+    graph_dirty_ = true;  // recorded steps use the shear-rate force variant
     tape_K_ = K;
     tape_n_ = n_;
     tape_dt_.clear();
@@ -695,6 +693,7 @@ void DeviceFluid::tape_end() {
     ckpt_.clear();
     slots_.clear();
     tape_dt_.clear();
+    if (taping_) graph_dirty_ = true;
     taping_ = false;
     tape_broken_ = false;
 }
@@ -720,13 +719,14 @@ DeviceFluid::StepBufs DeviceFluid::slot_bufs(TapeSlot& t) {
 }
 
 void DeviceFluid::tape_backward(const float* xbar_T, const float* vbar_T, const float* rhobar_T,
-                                float* xbar0, float* vbar0, double* dk, double* dmu) {
+                                float* xbar0, float* vbar0, double* dparams) {
     if (!taping_) throw std::logic_error("tape: no recording (call tape_begin first)");
     if (tape_broken_) throw std::logic_error("tape: particles were replaced while recording");
     const size_t T = tape_dt_.size();
     const int n = (int)n_;
     const size_t S = scn_.size();
-    for (size_t s = 0; s < S; ++s) dk[s] = dmu[s] = 0.0;
+    const size_t NP = (size_t)kAdjParams * S;
+    for (size_t k = 0; k < NP; ++k) dparams[k] = 0.0;
     if (T == 0 || n == 0) {
         for (size_t k = 0; k < 3 * n_; ++k) {
             if (xbar0) xbar0[k] = xbar_T ? xbar_T[k] : 0.f;
@@ -738,7 +738,7 @@ void DeviceFluid::tape_backward(const float* xbar_T, const float* vbar_T, const 
     for (int b = 0; b < 2; ++b) { xbar_[b].ensure(n); vbar_[b].ensure(n); }
     Fb_.ensure(n); Sb_.ensure(n); xb_loc_.ensure(n); vb_loc_.ensure(n); xb_acc_.ensure(n);
     rhob_loc_.ensure(n); rhob_.ensure(n); seed_.ensure(n);
-    adj_mu_.ensure(nt); adj_k_.ensure(nt); adj_tot_.ensure(2 * S);
+    mush_.ensure(n); adj_part_.ensure(nt * kAdjParams); adj_tot_.ensure(NP);
     {
         std::vector<float4> xs(n), vs(n);
         for (int i = 0; i < n; ++i) {
@@ -747,12 +747,12 @@ void DeviceFluid::tape_backward(const float* xbar_T, const float* vbar_T, const 
             vs[i] = vbar_T ? make_float4(vbar_T[3 * i], vbar_T[3 * i + 1], vbar_T[3 * i + 2], 0.f)
                            : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        std::vector<double> z(2 * S, 0.0);
+        std::vector<double> z(NP, 0.0);
         hip_check(hipMemcpyAsync(xbar_[0].get(), xs.data(), n * sizeof(float4), hipMemcpyHostToDevice, st_), "adj");
         hip_check(hipMemcpyAsync(vbar_[0].get(), vs.data(), n * sizeof(float4), hipMemcpyHostToDevice, st_), "adj");
         if (rhobar_T)
             hip_check(hipMemcpyAsync(seed_.get(), rhobar_T, n * sizeof(float), hipMemcpyHostToDevice, st_), "adj");
-        hip_check(hipMemcpyAsync(adj_tot_.get(), z.data(), 2 * S * sizeof(double), hipMemcpyHostToDevice, st_), "adj");
+        hip_check(hipMemcpyAsync(adj_tot_.get(), z.data(), NP * sizeof(double), hipMemcpyHostToDevice, st_), "adj");
         sync();
     }
     Snapshot final_state;
@@ -786,34 +786,32 @@ void DeviceFluid::tape_backward(const float* xbar_T, const float* vbar_T, const 
             AdjSlot A;
             A.posm = sl.posm_s.get(); A.velm = sl.velm_s.get(); A.rho = sl.rho.get(); A.prs = sl.prs.get();
             A.cell = sl.cell_s.get(); A.start = sl.start.get(); A.id = sl.id_s.get();
-            A.acc = sl.acc.get(); A.xsph = sl.xsph.get(); A.fvisc = sl.fvisc.get(); A.dt = tape_dt_[t];
+            A.acc = sl.acc.get(); A.xsph = sl.xsph.get(); A.fvisc = sl.fvisc.get();
+            A.mu = sl.mu.get(); A.mat = sl.mat_s.get(); A.dt = tape_dt_[t];
             AdjBuffers B;
             B.xbar_in = xbar_[cur].get(); B.vbar_in = vbar_[cur].get();
             B.Fb = Fb_.get(); B.Sb = Sb_.get(); B.xb_loc = xb_loc_.get(); B.vb_loc = vb_loc_.get();
             B.rhob_loc = rhob_loc_.get(); B.rhob = rhob_.get(); B.xb_acc = xb_acc_.get();
             B.xbar_out = xbar_[1 - cur].get(); B.vbar_out = vbar_[1 - cur].get();
-            B.mu_part = adj_mu_.get(); B.k_part = adj_k_.get(); B.totals = adj_tot_.get();
+            B.mush = mush_.get(); B.part = adj_part_.get(); B.totals = adj_tot_.get();
             launch_adjoint_step(st_, table(sl.dom.get()), A, B,
-                                (t == T - 1 && rhobar_T) ? seed_.get() : nullptr);
+                                (t == T - 1 && rhobar_T) ? seed_.get() : nullptr, any_shear_);
             hip_check(hipGetLastError(), "adjoint launch");
             cur = 1 - cur;
         }
     }
     std::vector<float4> xs(n), vs(n);
-    std::vector<double> tot(2 * S);
+    std::vector<double> tot(NP);
     hip_check(hipMemcpyAsync(xs.data(), xbar_[cur].get(), n * sizeof(float4), hipMemcpyDeviceToHost, st_), "adj out");
     hip_check(hipMemcpyAsync(vs.data(), vbar_[cur].get(), n * sizeof(float4), hipMemcpyDeviceToHost, st_), "adj out");
-    hip_check(hipMemcpyAsync(tot.data(), adj_tot_.get(), 2 * S * sizeof(double), hipMemcpyDeviceToHost, st_), "adj out");
+    hip_check(hipMemcpyAsync(tot.data(), adj_tot_.get(), NP * sizeof(double), hipMemcpyDeviceToHost, st_), "adj out");
     load_state(final_state);
     sync();
     for (int i = 0; i < n; ++i) {
         if (xbar0) { xbar0[3 * i] = xs[i].x; xbar0[3 * i + 1] = xs[i].y; xbar0[3 * i + 2] = xs[i].z; }
         if (vbar0) { vbar0[3 * i] = vs[i].x; vbar0[3 * i + 1] = vs[i].y; vbar0[3 * i + 2] = vs[i].z; }
     }
-    for (size_t s = 0; s < S; ++s) {
-        dk[s] = tot[2 * s];
-        dmu[s] = tot[2 * s + 1];
-    }
+    for (size_t k = 0; k < NP; ++k) dparams[k] = tot[k];
     prof_ = prof;
     // the slot steps wrote only slot buffers and the state, which is restored;
     // rho / p / aux of the last forward step are untouched

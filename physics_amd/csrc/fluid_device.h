@@ -156,13 +156,17 @@ class DeviceFluid {
     // checkpointed every K steps. tape_backward() takes the cotangents of the
     // final x, v (3 per particle) and of the last step's rho (1 per particle),
     // any of them null for zero, all in original index order, and returns the
-    // cotangents of the initial x, v and, per scene, dL/d stiffness and
-    // dL/d visc_k[0]. Each segment is recomputed from its checkpoint into
-    // per-step slots, then the adjoint runs backwards through it. The world
-    // state is left unchanged.
+    // cotangents of the initial x, v and, per scene, kAdjParams parameter
+    // gradients: [0] dL/d stiffness, [1 + m] dL/d visc_k[m], [1 +
+    // kMaxMaterials + m] dL/d visc_n[m] (zero for m >= visc_k.len()). Each
+    // segment is recomputed from its checkpoint into per-step slots, then the
+    // adjoint runs backwards through it. The world state is left unchanged.
+    // While recording, forces are computed by the shear-rate kernel variant
+    // (the visc_n gradient needs the shear sum; for visc_n = 1 the values are
+    // the same), so that recomputed steps match recorded ones bit for bit.
     void tape_begin(uint32_t K);
     void tape_backward(const float* xbar_T, const float* vbar_T, const float* rhobar_T,
-                       float* xbar0, float* vbar0, double* dk, double* dmu);
+                       float* xbar0, float* vbar0, double* dparams);
     void tape_end();
     size_t tape_steps() const { return tape_dt_.size(); }
     bool taping() const { return taping_; }
@@ -288,7 +292,8 @@ class DeviceFluid {
     std::vector<std::unique_ptr<TapeSlot>> slots_;
     DevBuf<float4> xbar_[2], vbar_[2], Fb_, Sb_, xb_loc_, vb_loc_, xb_acc_;
     DevBuf<float> rhob_loc_, rhob_, seed_;
-    DevBuf<double> adj_mu_, adj_k_, adj_tot_;
+    DevBuf<float2> mush_;
+    DevBuf<double> adj_part_, adj_tot_;
 };
 
 }  // namespace phy

@@ -5,6 +5,7 @@
 // The reason of the last failure is kept per thread for phy_last_error().
 #include "../../include/phy_ffi.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -12,6 +13,8 @@
 
 #include "archive.h"
 #include "world.h"
+
+static_assert(PHY_TAPE_PARAMS == phy::kAdjParams, "tape gradient layout");
 
 using phy::FluidBatch;
 using phy::FluidSubsystem;
@@ -476,12 +479,23 @@ int32_t phy_fluid_tape_backward(PhyWorldHandle* w, const double* xbar_T, const d
                                 const double* rhobar_T, double* xbar0, double* vbar0,
                                 double* dparams) {
     return guard<int32_t>(-1, [&] {
-        double dk = 0.0, dmu = 0.0;
-        need_fluid(w)->world().tape_backward(xbar_T, vbar_T, rhobar_T, xbar0, vbar0, &dk, &dmu);
+        double g[phy::kAdjParams];
+        need_fluid(w)->world().tape_backward(xbar_T, vbar_T, rhobar_T, xbar0, vbar0, g);
         if (dparams) {
-            dparams[0] = dk;
-            dparams[1] = dmu;
+            dparams[0] = g[0];
+            dparams[1] = g[1];
         }
+        return 0;
+    });
+}
+
+int32_t phy_fluid_tape_backward_full(PhyWorldHandle* w, const double* xbar_T, const double* vbar_T,
+                                     const double* rhobar_T, double* xbar0, double* vbar0,
+                                     double* dparams) {
+    return guard<int32_t>(-1, [&] {
+        double g[phy::kAdjParams];
+        need_fluid(w)->world().tape_backward(xbar_T, vbar_T, rhobar_T, xbar0, vbar0, g);
+        if (dparams) std::copy(g, g + phy::kAdjParams, dparams);
         return 0;
     });
 }
@@ -796,13 +810,25 @@ int32_t phy_fluid_batch_tape_backward(PhyFluidBatch* b, const float* xbar_final,
     return guard<int32_t>(-1, [&] {
         phy::DeviceFluid& d = need_batch(b)->device();
         const size_t S = d.scenes();
-        std::vector<double> dk(S), dmu(S);
-        d.tape_backward(xbar_final, vbar_final, rhobar_last, xbar0, vbar0, dk.data(), dmu.data());
+        std::vector<double> g(S * phy::kAdjParams);
+        d.tape_backward(xbar_final, vbar_final, rhobar_last, xbar0, vbar0, g.data());
         if (dparams)
             for (size_t s = 0; s < S; ++s) {
-                dparams[2 * s] = dk[s];
-                dparams[2 * s + 1] = dmu[s];
+                dparams[2 * s] = g[s * phy::kAdjParams];
+                dparams[2 * s + 1] = g[s * phy::kAdjParams + 1];
             }
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_tape_backward_full(PhyFluidBatch* b, const float* xbar_final,
+                                           const float* vbar_final, const float* rhobar_last,
+                                           float* xbar0, float* vbar0, double* dparams) {
+    return guard<int32_t>(-1, [&] {
+        phy::DeviceFluid& d = need_batch(b)->device();
+        std::vector<double> g(d.scenes() * phy::kAdjParams);
+        d.tape_backward(xbar_final, vbar_final, rhobar_last, xbar0, vbar0, g.data());
+        if (dparams) std::copy(g.begin(), g.end(), dparams);
         return 0;
     });
 }

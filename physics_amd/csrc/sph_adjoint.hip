@@ -1,17 +1,21 @@
 // sph_adjoint.hip — reverse-mode (vector-Jacobian product) of one SPH step.
 //
 // The forward step (crates/phy-fluid/src/sph/world.rs:183-397) maps
-// (x, v; k = stiffness, mu = visc_k[0]) to (x', v') and the density rho.
+// (x, v; k = stiffness, visc_k[m], visc_n[m]) to (x', v') and the density rho.
 // Given the cotangents of (x', v') (and of rho on the last step), three
-// passes produce the cotangents of (x, v) and accumulate dL/dk and dL/dmu:
+// passes produce the cotangents of (x, v) and accumulate the parameter
+// gradients:
 //
 //   k_adj_local    per particle: wall clamp and semi-implicit Euler backwards;
 //                  F_bar = a_bar / rho (acceleration = force / rho), the XSPH
-//                  cotangent s_bar, and the local rho_bar, mu_bar terms.
+//                  cotangent s_bar, the local rho_bar term, and the power-law
+//                  viscosity backwards (dL/d visc_k, dL/d visc_n per material,
+//                  the shear-rate cotangent).
 //   k_adj_force    gather over the force neighbours: every pair term of the
-//                  pressure, viscosity and XSPH sums, differentiated with
-//                  respect to the particle's own position, velocity, P = p /
-//                  rho^2 and rho, collected from both its own sums and its
+//                  pressure, viscosity (each particle's own mu), XSPH and
+//                  shear-rate sums, differentiated with respect to the
+//                  particle's own position, velocity, P = p / rho^2 and rho,
+//                  collected from both its own sums and its
 //                  neighbours' sums. The neighbour relation is symmetric, so
 //                  the transpose is a gather, not a scatter; then the EOS
 //                  p = k max(0, rho - rho0) backwards (dL/dk).
@@ -19,8 +23,8 @@
 //
 // Neighbour sets and visiting order are the forward's (shared accept tests in
 // sph_device.h) and all sums are gathers, so the adjoint is deterministic.
-// Sums are plain FP32 within a particle; dL/dk and dL/dmu are reduced in FP64
-// over fixed block partials.
+// Sums are plain FP32 within a particle; parameter gradients are reduced in
+// FP64 over fixed per-tile partials.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <math.h>
@@ -42,17 +46,21 @@ __device__ __forceinline__ double block_sum_f64(double v, double* sh) {
     return t;
 }
 
-// Integrate + bounds backwards (world.rs:352-397) and the acc = F / rho split.
+// Integrate + bounds backwards (world.rs:352-397), the acc = F / rho split and
+// the effective viscosity backwards: mu = visc_k[m] max(shear, shear_min)^(n-1)
+// (world.rs:319-327) gives dL/d visc_k[m], dL/d visc_n[m] (FP64 tile partials
+// per material) and the cotangent of the shear-rate sum.
 __global__ __launch_bounds__(256) void k_adj_local(SceneTable T, AdjSlot S,
                                                    const float4* __restrict__ xbar_in,
                                                    const float4* __restrict__ vbar_in, float4* Fb,
                                                    float4* Sb, float4* xb_loc, float4* vb_loc,
-                                                   float* rhob_loc, double* mu_part) {
+                                                   float* rhob_loc, float2* mush, double* part) {
     __shared__ double sh[4];
     const TileLoc TL = tile_loc(T);
     const int i = TL.i;
     const DevParams P = T.params[TL.s];
-    double mu_b = 0.0;
+    double kb = 0.0, nb = 0.0;  // dL/d visc_k[m], dL/d visc_n[m] of this particle's material
+    int m = 0;
     if (TL.act) {
         const uint32_t id = S.id[i];
         const float4 xo = xbar_in[id], vo = vbar_in[id];
@@ -73,12 +81,23 @@ __global__ __launch_bounds__(256) void k_adj_local(SceneTable T, AdjSlot S,
         }
         const float rho = S.rho[i];
         const float4 acc = S.acc[i];
-        float F[3] = {0.f, 0.f, 0.f}, rb = 0.f;
+        const float mu = S.mu[i];
+        float F[3] = {0.f, 0.f, 0.f}, rb = 0.f, shb = 0.f;
+        if (P.n_mat > 1) {
+            const uint32_t mt = S.mat[i];
+            m = mt < (uint32_t)P.n_mat ? (int)mt : 0;
+        }
         if (rho > 0.f) {  // acc = (f_press + f_visc) / rho (world.rs:332-336)
             for (int a = 0; a < 3; ++a) F[a] = ab[a] / rho;
             rb = -(ab[0] * acc.x + ab[1] * acc.y + ab[2] * acc.z) / rho;
-            const float4 fv = S.fvisc[i];
-            mu_b = ((double)F[0] * fv.x + (double)F[1] * fv.y + (double)F[2] * fv.z) / (double)P.visc_k[0];
+            const float4 fv = S.fvisc[i];  // viscous force per unit mu; w: shear sum
+            const double mub = (double)F[0] * fv.x + (double)F[1] * fv.y + (double)F[2] * fv.z;
+            const DevParams* Pm = T.params + TL.s;  // dynamic material index: read through the table
+            const double n = Pm->visc_n[m];
+            const double sreg = fv.w > P.shear_min ? (double)fv.w : (double)P.shear_min;
+            kb = mub * pow(sreg, n - 1.0);                   // d mu / d k = sreg^(n-1)
+            nb = mub * (double)mu * log(sreg);                // d mu / d n = mu ln sreg
+            if (fv.w > P.shear_min) shb = (float)(mub * (double)mu * (n - 1.0) / (double)fv.w);
         }
         const float prs = S.prs[i];
         Fb[i] = make_float4(F[0], F[1], F[2], prs / (rho * rho));
@@ -86,35 +105,52 @@ __global__ __launch_bounds__(256) void k_adj_local(SceneTable T, AdjSlot S,
         xb_loc[i] = make_float4(xb[0], xb[1], xb[2], 0.f);
         vb_loc[i] = make_float4(vt[0], vt[1], vt[2], 0.f);
         rhob_loc[i] = rb;
+        mush[i] = make_float2(mu, shb);
     }
-    mu_b = block_sum_f64(mu_b, sh);
-    if (threadIdx.x == 0) mu_part[TL.t] = mu_b;
+    // per-material tile partials (materials are uniform per scene table, so
+    // every thread runs the same loop)
+    const int nm = P.n_mat;
+    for (int q = 0; q < nm; ++q) {
+        const double tk = block_sum_f64(m == q ? kb : 0.0, sh);
+        const double tn = block_sum_f64(m == q ? nb : 0.0, sh);
+        if (threadIdx.x == 0) {
+            part[(size_t)TL.t * kAdjParams + 1 + q] = tk;
+            part[(size_t)TL.t * kAdjParams + 1 + kMaxMaterials + q] = tn;
+        }
+    }
 }
 
 struct AdjNbr {
     float4 p, v, F, s;
+    float2 ms;  // (mu, shear cotangent)
 };
 
+template <bool SHEAR>
 struct ForceAdjAcc {
     float A1;                     // sum m_b G (F_a - F_b) . D
     float A2[3];                  // sum m_b (P_a + P_b) [G dF + (G'/r)(dF . D) D]
-    float A3[3], A4;              // sum m_b V F_b ; sum (m_b / rho_b) V
-    float A5;                     // sum m_b V (F_b . dv)
-    float A6[3];                  // sum m_b [(F_a . dv) / rho_b - (F_b . dv) / rho_a] D / r
+    float A3[3], A4;              // sum mu_b m_b V F_b ; sum (m_b / rho_b) V
+    float A5;                     // sum mu_b m_b V (F_b . dv)
+    float A6[3];                  // sum m_b [mu_a (F_a . dv) / rho_b - mu_b (F_b . dv) / rho_a] D / r
     float A7, A8[3], A9;          // sum (m_b / rho_b) W ; sum W s_b ; sum W (s_b . dv)
     float A10[3];                 // sum (W'/r) [(m_b/rho_b)(s_a . dv) - (m_a/rho_a)(s_b . dv)] D
+    // shear-rate sum sh_a = sum |dv| (m_b / rho_b) / (r + eps), backwards:
+    float C1[3];                  // sum (shb_a mr_b + shb_b mr_a) / (r + eps) dv / |dv|
+    float C2[3];                  // sum (shb_a mr_b + shb_b mr_a) |dv| D / (r (r + eps)^2)
+    float C3;                     // sum shb_b |dv| / (r + eps)
 };
 
-template <int L>
+template <int L, bool SHEAR>
 __global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
                                                    const float4* __restrict__ Fb,
                                                    const float4* __restrict__ Sb,
                                                    const float4* __restrict__ xb_loc,
                                                    const float4* __restrict__ vb_loc,
                                                    const float* __restrict__ rhob_loc,
+                                                   const float2* __restrict__ mush,
                                                    const float* __restrict__ rhob_seed,
                                                    float* rhob_out, float4* xb_acc, float4* vbar_out,
-                                                   double* k_part) {
+                                                   double* part) {
     __shared__ uint32_t lst[L][256];
     __shared__ double sh[4];
     const int tid = threadIdx.x;
@@ -125,21 +161,26 @@ __global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
     const DevParams P = T.params[TL.s];
     const uint32_t* __restrict__ start = S.start + d.toff;
     float4 pa = make_float4(0.f, 0.f, 0.f, 0.f), va = pa, Fa = pa, sa = pa;
+    float2 msa = make_float2(0.f, 0.f);
     uint32_t ca = d.ncells;
     if (act) {
         pa = S.posm[a];
         va = S.velm[a];
         Fa = Fb[a];
         sa = Sb[a];
+        msa = mush[a];
         ca = S.cell[a];
     }
     const float ma = pa.w, ira = sa.w, mra = va.w;  // m_a, 1/rho_a, m_a/rho_a
+    const float mua = msa.x, shba = msa.y;
     const float h = P.h, h2 = P.h2;
-    ForceAdjAcc A = {};
+    ForceAdjAcc<SHEAR> A = {};
     gather<L, kAllWaves, 2>(
         act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, start, S.posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_maybe(P, pa.x, pa.y, pa.z, pj); },
-        [&](uint32_t j) { return AdjNbr{S.posm[j], S.velm[j], Fb[j], Sb[j]}; },
+        [&](uint32_t j) {
+            return AdjNbr{ldg(S.posm, j), ldg(S.velm, j), ldg(Fb, j), ldg(Sb, j), ldg(mush, j)};
+        },
         [&](uint32_t j, const AdjNbr& nb, bool listed) {
             const float Dx = nb.p.x - pa.x, Dy = nb.p.y - pa.y, Dz = nb.p.z - pa.z;
             float s = Dx * Dx + Dy * Dy + Dz * Dz;
@@ -151,7 +192,7 @@ __global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
             const float ir = __builtin_amdgcn_rsqf(s);
             const float r = s * ir;
             const float hr = h - r;
-            const float mb = nb.p.w, mrb = nb.v.w, irb = nb.s.w;
+            const float mb = nb.p.w, mrb = nb.v.w, irb = nb.s.w, mub = nb.ms.x;
             const float dvx = nb.v.x - va.x, dvy = nb.v.y - va.y, dvz = nb.v.z - va.z;
             const float dFx = nb.F.x - Fa.x, dFy = nb.F.y - Fa.y, dFz = nb.F.z - Fa.z;
             const float dFD = dFx * Dx + dFy * Dy + dFz * Dz;
@@ -176,12 +217,13 @@ __global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
             A.A2[1] += c2a * dFy + c2b * Dy;
             A.A2[2] += c2a * dFz + c2b * Dz;
             const float mV = w * mb * V;
-            A.A3[0] += mV * nb.F.x;
-            A.A3[1] += mV * nb.F.y;
-            A.A3[2] += mV * nb.F.z;
+            const float muV = mub * mV;
+            A.A3[0] += muV * nb.F.x;
+            A.A3[1] += muV * nb.F.y;
+            A.A3[2] += muV * nb.F.z;
             A.A4 += w * mrb * V;
-            A.A5 += mV * Fb_dv;
-            const float c6v = w * mb * (Fa_dv * irb - Fb_dv * ira) * ir;
+            A.A5 += muV * Fb_dv;
+            const float c6v = w * mb * (mua * Fa_dv * irb - mub * Fb_dv * ira) * ir;
             A.A6[0] += c6v * Dx;
             A.A6[1] += c6v * Dy;
             A.A6[2] += c6v * Dz;
@@ -195,22 +237,44 @@ __global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
             A.A10[0] += c10 * Dx;
             A.A10[1] += c10 * Dy;
             A.A10[2] += c10 * Dz;
+            if (SHEAR) {
+                // |dv| is not differentiable at 0: its subgradient 0 there
+                const float dv2 = dvx * dvx + dvy * dvy + dvz * dvz;
+                const float idv = dv2 > 0.f ? __builtin_amdgcn_rsqf(dv2) : 0.f;
+                const float adv = dv2 * idv;
+                const float ire = 1.f / (r + P.r_eps);
+                const float cs1 = w * (shba * mrb + nb.ms.y * mra) * ire;
+                A.C1[0] += cs1 * idv * dvx;
+                A.C1[1] += cs1 * idv * dvy;
+                A.C1[2] += cs1 * idv * dvz;
+                const float cs2 = cs1 * adv * ire * ir;
+                A.C2[0] += cs2 * Dx;
+                A.C2[1] += cs2 * Dy;
+                A.C2[2] += cs2 * Dz;
+                A.C3 += w * nb.ms.y * adv * ire;
+            }
         },
         [&]() {});
     double kb = 0.0;
     if (act) {
-        const float mu = P.visc_k[0];
         const float4 xl = xb_loc[a], vl = vb_loc[a];
         const float rho = S.rho[a];
         float xb[3], vb[3];
-        const float c6m = mu * ma * P.cv;
-        xb[0] = xl.x + ma * A.A2[0] + c6m * A.A6[0] - A.A10[0];
-        xb[1] = xl.y + ma * A.A2[1] + c6m * A.A6[1] - A.A10[1];
-        xb[2] = xl.z + ma * A.A2[2] + c6m * A.A6[2] - A.A10[2];
-        vb[0] = vl.x + mu * ma * (A.A3[0] * ira - Fa.x * A.A4) - sa.x * A.A7 + mra * A.A8[0];
-        vb[1] = vl.y + mu * ma * (A.A3[1] * ira - Fa.y * A.A4) - sa.y * A.A7 + mra * A.A8[1];
-        vb[2] = vl.z + mu * ma * (A.A3[2] * ira - Fa.z * A.A4) - sa.z * A.A7 + mra * A.A8[2];
-        float rb = rhob_loc[a] + (mu * ma * ira * ira) * A.A5 + (ma * ira * ira) * A.A9;
+        const float mcv = ma * P.cv;
+        xb[0] = xl.x + ma * A.A2[0] + mcv * A.A6[0] - A.A10[0];
+        xb[1] = xl.y + ma * A.A2[1] + mcv * A.A6[1] - A.A10[1];
+        xb[2] = xl.z + ma * A.A2[2] + mcv * A.A6[2] - A.A10[2];
+        vb[0] = vl.x + ma * (A.A3[0] * ira - mua * Fa.x * A.A4) - sa.x * A.A7 + mra * A.A8[0];
+        vb[1] = vl.y + ma * (A.A3[1] * ira - mua * Fa.y * A.A4) - sa.y * A.A7 + mra * A.A8[1];
+        vb[2] = vl.z + ma * (A.A3[2] * ira - mua * Fa.z * A.A4) - sa.z * A.A7 + mra * A.A8[2];
+        float rb = rhob_loc[a] + (ma * ira * ira) * A.A5 + (ma * ira * ira) * A.A9;
+        if (SHEAR) {
+            for (int c = 0; c < 3; ++c) {
+                xb[c] += A.C2[c];
+                vb[c] -= A.C1[c];
+            }
+            rb -= (ma * ira * ira) * A.C3;
+        }
         // EOS backwards: P = p / rho^2, p = k max(0, rho - rho0)
         const float Pbar = ma * A.A1;
         const float pbar = Pbar * ira * ira;
@@ -228,7 +292,7 @@ __global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
         vbar_out[id] = make_float4(vb[0], vb[1], vb[2], 0.f);
     }
     kb = block_sum_f64(kb, sh);
-    if (tid == 0) k_part[TL.t] = kb;
+    if (tid == 0) part[(size_t)TL.t * kAdjParams] = kb;
 }
 
 template <int L>
@@ -283,47 +347,47 @@ __global__ __launch_bounds__(256) void k_adj_density(SceneTable T, AdjSlot S,
 }
 
 // Adds one step's tile partials to the running FP64 totals of each scene
-// (one workgroup per scene, fixed summation shape).
-__global__ __launch_bounds__(256) void k_adj_reduce(SceneTable T, const double* __restrict__ mu_part,
-                                                    const double* __restrict__ k_part,
+// (one workgroup per scene, fixed summation shape). Materials beyond the
+// scene's table have no partials and stay zero.
+__global__ __launch_bounds__(256) void k_adj_reduce(SceneTable T, const double* __restrict__ part,
                                                     double* totals) {
-    __shared__ double sh[2][256];
+    __shared__ double sh[256];
     const SceneInfo si = T.info[blockIdx.x];
-    double km = 0.0, kk = 0.0;
-    for (uint32_t t = threadIdx.x; t < si.ntiles; t += 256) {
-        km += mu_part[si.tile0 + t];
-        kk += k_part[si.tile0 + t];
-    }
-    sh[0][threadIdx.x] = kk;
-    sh[1][threadIdx.x] = km;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) {
-            sh[0][threadIdx.x] += sh[0][threadIdx.x + o];
-            sh[1][threadIdx.x] += sh[1][threadIdx.x + o];
-        }
+    const int nm = T.params[blockIdx.x].n_mat;
+    for (int q = 0; q < kAdjParams; ++q) {
+        const int mq = q == 0 ? -1 : (q - 1) % kMaxMaterials;
+        if (mq >= nm) continue;  // uniform over the workgroup
+        double v = 0.0;
+        for (uint32_t t = threadIdx.x; t < si.ntiles; t += 256) v += part[(size_t)(si.tile0 + t) * kAdjParams + q];
+        sh[threadIdx.x] = v;
         __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        totals[2 * blockIdx.x] += sh[0][0];
-        totals[2 * blockIdx.x + 1] += sh[1][0];
+        for (int o = 128; o > 0; o >>= 1) {
+            if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) totals[(size_t)blockIdx.x * kAdjParams + q] += sh[0];
+        __syncthreads();
     }
 }
 
 void launch_adjoint_step(hipStream_t st, const SceneTable& T, const AdjSlot& S, const AdjBuffers& B,
-                         const float* rhob_seed) {
+                         const float* rhob_seed, bool shear) {
     if (T.ntiles > 0) {
         const int nb = T.ntiles;
         hipLaunchKernelGGL(k_adj_local, dim3(nb), dim3(256), 0, st, T, S, B.xbar_in, B.vbar_in, B.Fb,
-                           B.Sb, B.xb_loc, B.vb_loc, B.rhob_loc, B.mu_part);
-        hipLaunchKernelGGL(k_adj_force<kForceList>, dim3(nb), dim3(256), 0, st, T, S, B.Fb, B.Sb,
-                           B.xb_loc, B.vb_loc, B.rhob_loc, rhob_seed, B.rhob, B.xb_acc, B.vbar_out,
-                           B.k_part);
+                           B.Sb, B.xb_loc, B.vb_loc, B.rhob_loc, B.mush, B.part);
+        if (shear)
+            hipLaunchKernelGGL((k_adj_force<kForceList, true>), dim3(nb), dim3(256), 0, st, T, S, B.Fb,
+                               B.Sb, B.xb_loc, B.vb_loc, B.rhob_loc, B.mush, rhob_seed, B.rhob, B.xb_acc,
+                               B.vbar_out, B.part);
+        else
+            hipLaunchKernelGGL((k_adj_force<kForceList, false>), dim3(nb), dim3(256), 0, st, T, S, B.Fb,
+                               B.Sb, B.xb_loc, B.vb_loc, B.rhob_loc, B.mush, rhob_seed, B.rhob, B.xb_acc,
+                               B.vbar_out, B.part);
         hipLaunchKernelGGL(k_adj_density<kDensityList>, dim3(nb), dim3(256), 0, st, T, S, B.rhob,
                            B.xb_acc, B.xbar_out);
     }
-    hipLaunchKernelGGL(k_adj_reduce, dim3(T.nscenes), dim3(256), 0, st, T, B.mu_part, B.k_part,
-                       B.totals);
+    hipLaunchKernelGGL(k_adj_reduce, dim3(T.nscenes), dim3(256), 0, st, T, B.part, B.totals);
 }
 
 }  // namespace phy

@@ -68,7 +68,7 @@ struct ForceArgs {
     float4* acc_out;          // optional (null: not stored)
     float* mu_out;
     float4* xsph_out;         // w: wall-clamp flags (bit per axis)
-    float4* fvisc_out;        // optional viscous force (adjoint tape)
+    float4* fvisc_out;        // optional (adjoint tape): viscous force per unit mu, w: shear sum
 };
 
 // One recorded step for the adjoint, in that step's cell order.
@@ -82,9 +82,15 @@ struct AdjSlot {
     const uint32_t* id;
     const float4* acc;
     const float4* xsph;       // w: wall-clamp flags
-    const float4* fvisc;
+    const float4* fvisc;      // viscous force per unit mu, w: shear-rate sum
+    const float* mu;          // effective viscosity of each particle
+    const uint32_t* mat;      // material tags
     float dt;
 };
+
+// Parameter-gradient layout per scene: [0] dL/d stiffness, [1 + m] dL/d
+// visc_k[m], [1 + kMaxMaterials + m] dL/d visc_n[m].
+constexpr int kAdjParams = 1 + 2 * kMaxMaterials;
 
 struct AdjBuffers {
     const float4* xbar_in;    // cotangents of the step's output x', v' (original order)
@@ -98,14 +104,15 @@ struct AdjBuffers {
     float4* xb_acc;
     float4* xbar_out;         // cotangents of the step's input x, v (original order)
     float4* vbar_out;
-    double* mu_part;          // per-tile FP64 partials
-    double* k_part;
-    double* totals;           // running {dL/dk, dL/dmu} per scene
+    float2* mush;             // per particle: (mu, shear-rate cotangent)
+    double* part;             // per-tile FP64 partials, kAdjParams per tile
+    double* totals;           // running parameter gradients, kAdjParams per scene
 };
 
-// T.dom must be the domain array of the recorded step.
+// T.dom must be the domain array of the recorded step. shear: the force pass
+// has non-Newtonian terms (the power law's shear-rate dependence).
 void launch_adjoint_step(hipStream_t st, const SceneTable& T, const AdjSlot& S, const AdjBuffers& B,
-                         const float* rhob_seed);
+                         const float* rhob_seed, bool shear);
 
 // Every launch covers all scenes of T; T.dom / T.table_total belong to the
 // step's buffer set. Diagnostics write 5 doubles per scene; partial holds

@@ -582,7 +582,12 @@ __global__ __launch_bounds__(256) void k_force(
         mu_out[i] = mu;
         xsph_out[i] = make_float4(xs[0], xs[1], xs[2], __uint_as_float(clamp));
     }
-    if (fvisc_out) fvisc_out[i] = make_float4(fvis[0], fvis[1], fvis[2], 0.f);
+    // adjoint tape: viscous force per unit mu (m cv sum ...) and the raw
+    // shear-rate sum of the power law (0 for a Newtonian pass)
+    if (fvisc_out) {
+        const float u = rho_i > 0.f ? pi.w * P.cv : 0.f;
+        fvisc_out[i] = make_float4(u * A.s[3], u * A.s[4], u * A.s[5], SHEAR ? A.s[9] : 0.f);
+    }
 
     // watchdog: smallest original index with a non-finite position / velocity
     uint32_t id = idv[i];

@@ -305,14 +305,14 @@ void FluidWorld::tape_begin(uint32_t K) {
 }
 
 void FluidWorld::tape_backward(const double* xbar_T, const double* vbar_T, const double* rhobar_T,
-                               double* xbar0, double* vbar0, double* dk, double* dmu) {
+                               double* xbar0, double* vbar0, double* dparams) {
     const size_t n = dev_.size();
     std::vector<float> xb, vb, rb, xo(3 * n), vo(3 * n);
     if (xbar_T) xb.assign(xbar_T, xbar_T + 3 * n);
     if (vbar_T) vb.assign(vbar_T, vbar_T + 3 * n);
     if (rhobar_T) rb.assign(rhobar_T, rhobar_T + n);
     dev_.tape_backward(xbar_T ? xb.data() : nullptr, vbar_T ? vb.data() : nullptr,
-                       rhobar_T ? rb.data() : nullptr, xo.data(), vo.data(), dk, dmu);
+                       rhobar_T ? rb.data() : nullptr, xo.data(), vo.data(), dparams);
     for (size_t k = 0; k < 3 * n; ++k) {
         if (xbar0) xbar0[k] = xo[k];
         if (vbar0) vbar0[k] = vo[k];

@@ -104,10 +104,11 @@ class FluidWorld {
     void set_body_acc(const double* b3);
 
     void set_keep_aux(bool on);
-    // adjoint tape (see DeviceFluid::tape_*); cotangents in original index order
+    // adjoint tape (see DeviceFluid::tape_*); cotangents in original index
+    // order, dparams: kAdjParams parameter gradients
     void tape_begin(uint32_t K);
     void tape_backward(const double* xbar_T, const double* vbar_T, const double* rhobar_T,
-                       double* xbar0, double* vbar0, double* dk, double* dmu);
+                       double* xbar0, double* vbar0, double* dparams);
     void tape_end();
     size_t tape_steps() const { return dev_.tape_steps(); }
     // true while the device state is exactly what the last step wrote, so the

@@ -16,6 +16,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libphy_ffi.so")
 MAX_MAT = 16
+TAPE_PARAMS = 1 + 2 * MAX_MAT  # PHY_TAPE_PARAMS: stiffness, visc_k[16], visc_n[16]
 
 FIELDS = {"pos": 0, "vel": 1, "acc": 2, "xsph": 3, "rho": 4, "p": 5, "mass": 6, "mu_eff": 7}
 VEC_FIELDS = {"pos", "vel", "acc", "xsph"}
@@ -49,6 +50,7 @@ EXPORTS = [
     "phy_fluid_batch_profile_end", "phy_device_mem_info", "phy_fluid_get_tuning",
     "phy_fluid_set_tuning", "phy_fluid_autotune", "phy_fluid_batch_get_tuning",
     "phy_fluid_batch_set_tuning", "phy_fluid_batch_autotune", "phy_fluid_flat_export",
+    "phy_fluid_tape_backward_full", "phy_fluid_batch_tape_backward_full",
 ]
 
 
@@ -183,6 +185,7 @@ def lib():
         "phy_fluid_tape_begin": ([H, C.c_uint32], C.c_int32),
         "phy_fluid_tape_steps": ([H], C.c_uint32),
         "phy_fluid_tape_backward": ([H, dp, dp, dp, dp, dp, dp], C.c_int32),
+        "phy_fluid_tape_backward_full": ([H, dp, dp, dp, dp, dp, dp], C.c_int32),
         "phy_fluid_tape_end": ([H], C.c_int32),
         "phy_fluid_set_graph": ([H, C.c_int32], C.c_int32),
         "phy_fluid_batch_create": ([C.POINTER(PhySphParams), C.c_uint32], H),
@@ -203,6 +206,7 @@ def lib():
         "phy_fluid_batch_tape_begin": ([H, C.c_uint32], C.c_int32),
         "phy_fluid_batch_tape_steps": ([H], C.c_uint32),
         "phy_fluid_batch_tape_backward": ([H, fp, fp, fp, fp, fp, dp], C.c_int32),
+        "phy_fluid_batch_tape_backward_full": ([H, fp, fp, fp, fp, fp, dp], C.c_int32),
         "phy_fluid_batch_tape_end": ([H], C.c_int32),
         "phy_fluid_batch_profile_begin": ([H], C.c_int32),
         "phy_fluid_batch_profile_end": ([H, dp, u32p], C.c_int32),
@@ -467,6 +471,22 @@ class FluidWorld(World):
             raise PhyError(last_error())
         return x0, v0, float(dp[0]), float(dp[1])
 
+    def tape_backward_full(self, xbar=None, vbar=None, rhobar=None):
+        """As tape_backward, with every parameter gradient: -> (x0_bar, v0_bar,
+        dL/d stiffness, dL/d visc_k[0..16], dL/d visc_n[0..16])."""
+        n = len(self)
+        xb = None if xbar is None else np.ascontiguousarray(xbar, dtype=np.float64).reshape(n, 3)
+        vb = None if vbar is None else np.ascontiguousarray(vbar, dtype=np.float64).reshape(n, 3)
+        rb = None if rhobar is None else np.ascontiguousarray(rhobar, dtype=np.float64).reshape(n)
+        x0 = np.zeros((n, 3))
+        v0 = np.zeros((n, 3))
+        dp = np.zeros(TAPE_PARAMS)
+        rc = lib().phy_fluid_tape_backward_full(self._h, _dp(xb), _dp(vb), _dp(rb), _dp(x0), _dp(v0),
+                                                _dp(dp))
+        if rc != 0:
+            raise PhyError(last_error())
+        return x0, v0, float(dp[0]), dp[1:1 + MAX_MAT].copy(), dp[1 + MAX_MAT:].copy()
+
     def tape_end(self) -> None:
         if lib().phy_fluid_tape_end(self._h) != 0:
             raise PhyError(last_error())
@@ -637,6 +657,20 @@ class FluidBatch:
         self._ok(lib().phy_fluid_batch_tape_backward(self._h, _fp(xb), _fp(vb), _fp(rb), _fp(x0),
                                                       _fp(v0), _dp(dp)))
         return x0, v0, dp[:, 0].copy(), dp[:, 1].copy()
+
+    def tape_backward_full(self, xbar=None, vbar=None, rhobar=None):
+        """-> (x0_bar, v0_bar, dL/d stiffness [S], dL/d visc_k [S, 16], dL/d visc_n [S, 16])."""
+        n = len(self)
+        xb = None if xbar is None else np.ascontiguousarray(xbar, dtype=np.float32).reshape(n, 3)
+        vb = None if vbar is None else np.ascontiguousarray(vbar, dtype=np.float32).reshape(n, 3)
+        rb = None if rhobar is None else np.ascontiguousarray(rhobar, dtype=np.float32).reshape(n)
+        x0 = np.zeros((n, 3), dtype=np.float32)
+        v0 = np.zeros((n, 3), dtype=np.float32)
+        dp = np.zeros((self.nscenes, TAPE_PARAMS))
+        self._ok(lib().phy_fluid_batch_tape_backward_full(self._h, _fp(xb), _fp(vb), _fp(rb),
+                                                           _fp(x0), _fp(v0), _dp(dp)))
+        return (x0, v0, dp[:, 0].copy(), dp[:, 1:1 + MAX_MAT].copy(),
+                dp[:, 1 + MAX_MAT:].copy())
 
     def tape_end(self) -> None:
         self._ok(lib().phy_fluid_batch_tape_end(self._h))
