@@ -196,6 +196,71 @@ int32_t phy_fluid_set_keep_aux(PhyWorldHandle *w, int32_t on);
 /* body_acc accumulator (3 per particle, index order), consumed by the next step
  * (world.rs:356, 361). 0 ok, -1 error. */
 int32_t phy_fluid_set_body_acc(PhyWorldHandle *w, const double *b3, uintptr_t len);
+/* The pending body_acc (3 per particle, index order; zero after a step). */
+int32_t phy_fluid_get_body_acc(PhyWorldHandle *w, double *out3, uintptr_t len);
+
+/* ---- Coupling (crates/phy-fluid/src/sph/world.rs:400-773) ----
+ * The reference's FluidWorld::couple_bodies / couple_points / couple_heat,
+ * called on the current state (typically after a step), with the bodies,
+ * points and heat grid handed over as plain arrays: this library hosts no
+ * rigid or heat subsystem, so FluidSubsystem::couple inside a World stays a
+ * no-op. Particles change in place; rho, p and aux keep the last step's
+ * values; a recording tape is invalidated. 0 ok, -1 error. */
+#define PHY_SHAPE_SPHERE 0  /* dims[0] = radius */
+#define PHY_SHAPE_BOX 1     /* dims = half extents */
+#define PHY_SHAPE_CAPSULE 2 /* dims[0] = half height (along local y), dims[1] = radius */
+
+/* Body<f64> as couple_bodies reads it (phy-rigid shape.rs:433-463). */
+typedef struct PhyCoupleBody {
+    double pose[7];   /* position xyz + orientation quaternion w i j k (body -> world),
+                         the layout of phy_world_rigid_add_body's pos7 */
+    double vel[3];    /* linear velocity; updated in place */
+    double inv_mass;  /* <= 0: static (impenetrable, no buoyancy) */
+    int32_t shape;    /* PHY_SHAPE_* */
+    int32_t _pad;
+    double dims[3];
+} PhyCoupleBody;
+
+/* Particles inside a body are pushed to its support point along their local
+ * direction (world.rs:755-773); a dynamic body takes the contact impulses and
+ * rho0 V g dt of buoyancy when any particle entered it (world.rs:414-503). */
+int32_t phy_fluid_couple_bodies(PhyWorldHandle *w, PhyCoupleBody *bodies, uint32_t n, double dt,
+                                double friction);
+
+/* CouplePoint<f64> (sph/couple.rs:10-19). */
+typedef struct PhyCouplePoint {
+    double pos[3];
+    double vel[3];
+    double mass;
+    double force[3];  /* out: buoyancy + drag; untouched outside the fluid bounds */
+} PhyCouplePoint;
+
+/* Points in order: mass-weighted fluid velocity of the 27 cells around each
+ * point, force, and the reaction -force dt spread over those particles by
+ * mass fraction (world.rs:691-752). */
+int32_t phy_fluid_couple_points(PhyWorldHandle *w, PhyCouplePoint *pts, uint32_t n, double dt, double drag,
+                                double soft_density);
+
+/* A temperature grid (phy-field ScalarField: x fastest, idx = x + nx (y + ny z)). */
+typedef struct PhyHeatGrid {
+    uint32_t nx, ny, nz, _pad;
+    double origin[3];
+    double dx;
+    const double *u;  /* temperatures, nx*ny*nz */
+    double *src;      /* heat sources, nx*ny*nz, accumulated in place (NULL when heat_gain <= 0) */
+} PhyHeatGrid;
+
+/* body_acc += -g beta (T(x) - t_ref) with T trilinear (t_ref outside the grid)
+ * for particles inside the fluid bounds; with heat_gain > 0, src[cell] +=
+ * heat_gain |v| dt (world.rs:555-678). */
+int32_t phy_fluid_couple_heat(PhyWorldHandle *w, const PhyHeatGrid *grid, double dt, double t_ref, double beta,
+                              double heat_gain);
+
+/* The velocity sampler couple_heat installs on the heat field (world.rs:614-643):
+ * for each query point the velocity of the nearest particle of the current
+ * state (the first in index order on ties; particle 0 when no distance is
+ * finite; zero without particles). */
+int32_t phy_fluid_sample_velocity(PhyWorldHandle *w, const double *q3, uint32_t n, double *out3);
 
 /* FP64 device reductions over the current state: out5 = total mass, momentum
  * x, y, z (sum m v) and kinetic energy (sum |v|^2 m / 2) (subsystem.rs:149-163).

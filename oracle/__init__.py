@@ -128,6 +128,13 @@ def lib():
     L.or_world_flat_grid.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_uint64),
                                      C.POINTER(C.c_int32), C.c_uint64, C.POINTER(C.c_int32), C.c_uint64]
     L.or_world_flat_grid.restype = C.c_int
+    i32p = C.POINTER(C.c_int32)
+    L.or_world_couple_bodies.argtypes = [vp, sz, dp, dp, dp, i32p, dp, C.c_double]
+    L.or_world_couple_points.argtypes = [vp, sz, dp, dp, dp, dp, C.c_double, C.c_double, C.c_double]
+    L.or_world_couple_heat.argtypes = [vp, sz, sz, sz, dp, C.c_double, dp, dp, C.c_double, C.c_double,
+                                       C.c_double, C.c_double]
+    L.or_world_nearest_velocity.argtypes = [vp, sz, dp, dp]
+    L.or_quat_rotate.argtypes = [dp, dp, dp]
     assert L.or_params_size() == C.sizeof(Params), "Params layout mismatch"
     _lib = L
     return L
@@ -143,6 +150,18 @@ def kernels(h: float, r: float = 0.0):
     v = np.zeros(3)
     lib().or_kernels(h, _dptr(c), r, _dptr(v))
     return c, v
+
+
+def quat_rotate(q_wijk, v) -> np.ndarray:
+    """nalgebra's UnitQuaternion * Vector3 as the oracle restates it."""
+    q = np.ascontiguousarray(q_wijk, dtype=np.float64)
+    a = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.zeros(3)
+    lib().or_quat_rotate(_dptr(q), _dptr(a), _dptr(out))
+    return out
+
+
+SHAPES = {"sphere": 0, "box": 1, "capsule": 2}
 
 
 def set_mutation(name: str) -> None:
@@ -278,3 +297,49 @@ class World:
         assert rc == 0
         return np.array(list(mn), dtype=np.int64), np.array(list(nc), dtype=np.uint64), cs, so
 
+    # ---- coupling (world.rs:400-773) ----
+
+    def couple_bodies(self, pose7, vel3, inv_mass, shape, dims3, dt: float, friction: float = 0.0):
+        """FluidWorld::couple_bodies on bodies given as arrays (pose = pos xyz +
+        quaternion w i j k). Returns the bodies' new velocities; the particles
+        change in place. ``friction`` is unused by the reference."""
+        pose = np.ascontiguousarray(pose7, dtype=np.float64).reshape(-1, 7)
+        nb = pose.shape[0]
+        vel = np.array(vel3, dtype=np.float64).reshape(nb, 3)
+        im = np.ascontiguousarray(inv_mass, dtype=np.float64).reshape(nb)
+        sh = np.ascontiguousarray(shape, dtype=np.int32).reshape(nb)
+        dm = np.ascontiguousarray(dims3, dtype=np.float64).reshape(nb, 3)
+        lib().or_world_couple_bodies(self._h, nb, _dptr(pose), _dptr(vel), _dptr(im),
+                                     sh.ctypes.data_as(C.POINTER(C.c_int32)), _dptr(dm), dt)
+        return vel
+
+    def couple_points(self, pos3, vel3, mass, force3, dt: float, drag: float, soft_density: float):
+        """FluidWorld::couple_points; returns the points' forces (unchanged for
+        points outside the bounds)."""
+        pos = np.ascontiguousarray(pos3, dtype=np.float64).reshape(-1, 3)
+        n = pos.shape[0]
+        vel = np.ascontiguousarray(vel3, dtype=np.float64).reshape(n, 3)
+        m = np.ascontiguousarray(np.broadcast_to(np.asarray(mass, dtype=np.float64), (n,)))
+        f = np.array(force3, dtype=np.float64).reshape(n, 3)
+        lib().or_world_couple_points(self._h, n, _dptr(pos), _dptr(vel), _dptr(m), _dptr(f), dt, drag,
+                                     soft_density)
+        return f
+
+    def couple_heat(self, u, origin, dx: float, src, dt: float, t_ref: float, beta: float,
+                    heat_gain: float):
+        """FluidWorld::couple_heat against a temperature grid u[z, y, x] (x
+        fastest); returns the updated source grid, body_acc changes in place."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        nz, ny, nx = u.shape
+        o = np.ascontiguousarray(origin, dtype=np.float64)
+        out = np.array(src, dtype=np.float64).reshape(nz, ny, nx)
+        lib().or_world_couple_heat(self._h, nx, ny, nz, _dptr(o), dx, _dptr(u), _dptr(out), dt, t_ref, beta,
+                                   heat_gain)
+        return out
+
+    def nearest_velocity(self, q3) -> np.ndarray:
+        """The velocity sampler couple_heat installs (world.rs:614-643)."""
+        q = np.ascontiguousarray(q3, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros_like(q)
+        lib().or_world_nearest_velocity(self._h, q.shape[0], _dptr(q), _dptr(out))
+        return out

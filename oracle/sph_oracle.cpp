@@ -11,6 +11,9 @@
 //   crates/phy-fluid/src/sph/grid.rs    Grid::{key_of, build, for_each_neighbor}
 //   crates/phy-fluid/src/kernels.rs     Kernels::{new, poly6, spiky_grad_mag, visc_lap}
 //   crates/phy-fluid/src/sph/params.rs  SphParams::defaults
+//   crates/phy-fluid/src/sph/world.rs   couple_bodies, push_out, body_volume,
+//                                        couple_heat (+ its velocity sampler),
+//                                        couple_points (world.rs:400-773)
 //   crates/phy-fluid/src/particle.rs    Particle::{new, with_material}
 //   crates/phy-fluid/src/subsystem.rs   FluidSubsystem::{validate, total_momentum,
 //                                        kinetic_energy}
@@ -345,6 +348,304 @@ struct World {
 };
 
 // params.rs:54-85
+// ---------------------------------------------------------------------------
+// Coupling (world.rs:400-773): fluid <-> rigid bodies, fluid <-> heat grid,
+// fluid <-> point masses, and the heat field's nearest-particle velocity
+// sampler. Rigid-body geometry follows phy-rigid shape.rs:198-330 (support,
+// containment) and 433-463 (pose transforms, linear impulse); the heat grid
+// follows phy-field heat.rs:129-180 and grid.rs:80-82, 125-130, 203-208.
+//
+// Quaternion rotation restates nalgebra 0.33.3's UnitQuaternion * Vector3
+// (t = 2 (q_v x v); v' = (t w + q_v x t) + v), a third-party routine that is
+// not vendored in the reference: parity unpinned at the ulp level for rotated
+// bodies (pinned against rotation matrices to 1e-15 by the oracle tests).
+
+struct Quat {
+    double w, i, j, k;
+};
+inline V3 cross(V3 a, V3 b) {
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+inline double dot(V3 a, V3 b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+inline V3 neg(V3 a) { return v3(-a.x, -a.y, -a.z); }
+inline V3 normalize(V3 a) { return a / norm(a); }
+inline V3 qrot(Quat q, V3 v) {
+    V3 u = v3(q.i, q.j, q.k);
+    V3 t = cross(u, v) * 2.0;
+    V3 c = cross(u, t);
+    return (t * q.w + c) + v;
+}
+inline Quat qconj(Quat q) { return Quat{q.w, -q.i, -q.j, -q.k}; }
+
+enum ShapeKind : int32_t { SHAPE_SPHERE = 0, SHAPE_BOX = 1, SHAPE_CAPSULE = 2 };
+
+// Body<f64> restricted to what the coupling reads and writes.
+struct CBody {
+    V3 pos;
+    Quat rot;
+    V3 vel;
+    double inv_mass;
+    int32_t shape;
+    V3 dims;  // sphere: r; box: half extents; capsule: half_height, r
+};
+
+inline V3 body_to_local(const CBody& b, V3 p) { return qrot(qconj(b.rot), p - b.pos); }
+inline V3 body_to_world(const CBody& b, V3 p) { return qrot(b.rot, p) + b.pos; }
+
+// f64::clamp (min <= max assumed; NaN stays NaN)
+inline double clamp_f(double x, double lo, double hi) {
+    if (x < lo) return lo;
+    if (x > hi) return hi;
+    return x;
+}
+
+bool contains_local(const CBody& b, V3 p) {
+    switch (b.shape) {
+        case SHAPE_SPHERE: return norm(p) <= b.dims.x;
+        case SHAPE_BOX:
+            return std::fabs(p.x) <= b.dims.x && std::fabs(p.y) <= b.dims.y && std::fabs(p.z) <= b.dims.z;
+        case SHAPE_CAPSULE: {
+            double hh = b.dims.x, r = b.dims.y;
+            double y = clamp_f(p.y, -hh, hh);
+            double cx = (p.x * p.x + (p.y - y) * (p.y - y)) + p.z * p.z;
+            return cx <= r * r;
+        }
+        default: return false;
+    }
+}
+
+V3 support_local(const CBody& b, V3 dir) {
+    switch (b.shape) {
+        case SHAPE_SPHERE: return normalize(dir) * b.dims.x;
+        case SHAPE_BOX:
+            return v3(dir.x >= 0.0 ? b.dims.x : -b.dims.x, dir.y >= 0.0 ? b.dims.y : -b.dims.y,
+                      dir.z >= 0.0 ? b.dims.z : -b.dims.z);
+        case SHAPE_CAPSULE: {
+            V3 n = normalize(dir);
+            V3 end = v3(0.0, 0.0, 0.0);
+            if (dir.y > 0.0) end = v3(0.0, b.dims.x, 0.0);
+            else if (dir.y < 0.0) end = v3(0.0, -b.dims.x, 0.0);
+            return end + n * b.dims.y;
+        }
+        default: return v3(0.0, 0.0, 0.0);
+    }
+}
+
+// world.rs:507-540 (the three closed forms this library hosts)
+double body_volume(const CBody& b) {
+    const double pi = 3.141592653589793;
+    switch (b.shape) {
+        case SHAPE_SPHERE: { double r = b.dims.x; return 4.0 / 3.0 * pi * r * r * r; }
+        case SHAPE_BOX: return b.dims.x * b.dims.y * b.dims.z * 8.0;
+        case SHAPE_CAPSULE: {
+            double hh = b.dims.x, r = b.dims.y;
+            return pi * r * r * 2.0 * hh + 4.0 / 3.0 * pi * r * r * r;
+        }
+        default: return 0.0;
+    }
+}
+
+// world.rs:755-773
+void push_out(Particle& pt, const CBody& b) {
+    V3 local = body_to_local(b, pt.pos);
+    V3 dir = norm(local) > 1e-9 ? normalize(local) : v3(0.0, 1.0, 0.0);
+    V3 sd = support_local(b, dir);
+    pt.pos = body_to_world(b, sd);
+    V3 v_rel = pt.vel - b.vel;
+    V3 n = normalize(body_to_world(b, normalize(sd)) - b.pos);
+    double vn = dot(v_rel, n);
+    if (vn < 0.0) pt.vel = pt.vel - n * vn;
+}
+
+// world.rs:414-503
+void couple_bodies(World& W, CBody* bodies, size_t nb, double dt) {
+    if (W.particles.empty() || nb == 0) return;
+    const double rho_f = W.params.rest_density;
+    const V3 g = v3(W.params.gravity[0], W.params.gravity[1], W.params.gravity[2]);
+    const double particle_vol = W.params.mass / rho_f;
+    const double restitution = 0.0;
+    std::vector<double> displaced(nb, 0.0);
+    for (size_t pi = 0; pi < W.particles.size(); ++pi) {
+        Particle& pt = W.particles[pi];
+        const V3 p_world = pt.pos;
+        V3 v_p = pt.vel;
+        const double m_p = pt.mass;
+        for (size_t bi = 0; bi < nb; ++bi) {
+            CBody& b = bodies[bi];
+            V3 local = body_to_local(b, p_world);
+            if (!contains_local(b, local)) continue;
+            V3 n = norm(local) > 1e-9 ? normalize(local) : v3(0.0, 1.0, 0.0);
+            V3 v_b = b.vel;
+            if (b.inv_mass <= 0.0) {
+                push_out(pt, b);
+                v_p = pt.vel;
+                continue;
+            }
+            double v_rel_n = dot(v_p - v_b, n);
+            if (v_rel_n < 0.0) {
+                double inv_sum = (1.0 / m_p) + b.inv_mass;
+                double j_mag = -(1.0 + restitution) * v_rel_n / inv_sum;
+                V3 j = n * j_mag;
+                v_p += j / m_p;
+                b.vel += neg(j) * b.inv_mass;  // apply_impulse(-j)
+            }
+            displaced[bi] += particle_vol;
+            push_out(pt, b);
+            v_p = pt.vel;  // the particle keeps push_out's velocity (world.rs:488)
+        }
+        pt.vel = v_p;
+    }
+    for (size_t bi = 0; bi < nb; ++bi) {
+        CBody& b = bodies[bi];
+        if (b.inv_mass <= 0.0) continue;
+        double vol_b = body_volume(b);
+        double v_sub = displaced[bi] > 0.0 ? vol_b : 0.0;
+        V3 buoy = neg(g) * (rho_f * v_sub);
+        b.vel += (buoy * dt) * b.inv_mass;
+    }
+}
+
+// world.rs:691-752
+struct CPoint {
+    V3 pos, vel;
+    double mass;
+    V3 force;
+};
+
+void couple_points(World& W, CPoint* pts, size_t np, double dt, double drag, double soft_density) {
+    if (np == 0 || W.particles.empty()) return;
+    const double rho_f = W.params.rest_density;
+    const V3 g = v3(W.params.gravity[0], W.params.gravity[1], W.params.gravity[2]);
+    const double* lo = W.params.bounds_min;
+    const double* hi = W.params.bounds_max;
+    W.grid.build(W.particles);
+    for (size_t k = 0; k < np; ++k) {
+        CPoint& pt = pts[k];
+        if (pt.pos.x < lo[0] || pt.pos.x > hi[0] || pt.pos.y < lo[1] || pt.pos.y > hi[1] ||
+            pt.pos.z < lo[2] || pt.pos.z > hi[2])
+            continue;
+        V3 v_sum = v3(0.0, 0.0, 0.0);
+        double w_sum = 0.0;
+        W.grid.for_each_neighbor(pt.pos, [&](size_t j) {
+            double w = W.particles[j].mass;
+            v_sum += W.particles[j].vel * w;
+            w_sum += w;
+        });
+        V3 v_f = w_sum > 0.0 ? v_sum / w_sum : v3(0.0, 0.0, 0.0);
+        double vol = soft_density > 0.0 ? pt.mass / soft_density : 0.0;
+        V3 f_b = neg(g) * (rho_f * vol);
+        V3 f_d = (v_f - pt.vel) * (drag * pt.mass);
+        V3 f = f_b + f_d;
+        pt.force = f;
+        double m_sum = 0.0;
+        std::vector<size_t> idxs;
+        W.grid.for_each_neighbor(pt.pos, [&](size_t j) {
+            m_sum += W.particles[j].mass;
+            idxs.push_back(j);
+        });
+        if (m_sum > 0.0) {
+            V3 reaction = neg(f) * dt;
+            for (size_t j : idxs) {
+                double frac = W.particles[j].mass / m_sum;
+                W.particles[j].vel += reaction * frac;
+            }
+        }
+    }
+}
+
+// world.rs:555-678 (heat grid: x fastest, idx = x + nx (y + ny z))
+struct CHeat {
+    size_t nx, ny, nz;
+    V3 origin;
+    double dx;
+    const double* u;
+    double* src;
+    double at(size_t x, size_t y, size_t z) const {
+        size_t cx = std::min(x, nx - 1), cy = std::min(y, ny - 1), cz = std::min(z, nz - 1);
+        return u[cx + nx * (cy + ny * cz)];
+    }
+    double trilinear(size_t cx, size_t cy, size_t cz, double fx, double fy, double fz) const {
+        size_t x1 = std::min(cx + 1, nx - 1), y1 = std::min(cy + 1, ny - 1), z1 = std::min(cz + 1, nz - 1);
+        double c000 = at(cx, cy, cz), c100 = at(x1, cy, cz), c010 = at(cx, y1, cz), c110 = at(x1, y1, cz);
+        double c001 = at(cx, cy, z1), c101 = at(x1, cy, z1), c011 = at(cx, y1, z1), c111 = at(x1, y1, z1);
+        double x00 = c000 + (c100 - c000) * fx;
+        double x10 = c010 + (c110 - c010) * fx;
+        double x01 = c001 + (c101 - c001) * fx;
+        double x11 = c011 + (c111 - c011) * fx;
+        double y0 = x00 + (x10 - x00) * fy;
+        double y1v = x01 + (x11 - x01) * fy;
+        return y0 + (y1v - y0) * fz;
+    }
+};
+
+// usize::to_usize of a floored f64 (None -> unwrap_or(0))
+inline size_t to_usize_or0(double f) {
+    if (f >= 0.0 && f < 18446744073709551616.0) return (size_t)f;
+    return 0;
+}
+
+void couple_heat(World& W, CHeat& H, double dt, double t_ref, double beta, double heat_gain) {
+    if (W.particles.empty()) return;
+    const V3 g = v3(W.params.gravity[0], W.params.gravity[1], W.params.gravity[2]);
+    const double* lo = W.params.bounds_min;
+    const double* hi = W.params.bounds_max;
+    const double dx = H.dx;
+    const double nx_m1 = (double)(H.nx ? H.nx - 1 : 0);
+    const double ny_m1 = (double)(H.ny ? H.ny - 1 : 0);
+    const double nz_m1 = (double)(H.nz ? H.nz - 1 : 0);
+    const V3 hmin = H.origin;
+    auto to_idx = [&](double x, double y, double z, size_t* c, double* t) {
+        if (x < hmin.x || x > hmin.x + dx * nx_m1 || y < hmin.y || y > hmin.y + dx * ny_m1 ||
+            z < hmin.z || z > hmin.z + dx * nz_m1)
+            return false;
+        double fx = std::floor((x - hmin.x) / dx);
+        double fy = std::floor((y - hmin.y) / dx);
+        double fz = std::floor((z - hmin.z) / dx);
+        c[0] = to_usize_or0(fx);
+        c[1] = to_usize_or0(fy);
+        c[2] = to_usize_or0(fz);
+        t[0] = (x - hmin.x) / dx - fx;
+        t[1] = (y - hmin.y) / dx - fy;
+        t[2] = (z - hmin.z) / dx - fz;
+        return true;
+    };
+    for (Particle& pt : W.particles) {
+        if (pt.pos.x < lo[0] || pt.pos.x > hi[0] || pt.pos.y < lo[1] || pt.pos.y > hi[1] ||
+            pt.pos.z < lo[2] || pt.pos.z > hi[2])
+            continue;
+        size_t c[3];
+        double t[3];
+        bool in = to_idx(pt.pos.x, pt.pos.y, pt.pos.z, c, t);
+        double temp = in ? H.trilinear(c[0], c[1], c[2], t[0], t[1], t[2]) : t_ref;
+        V3 buoy_corr = neg(g) * beta * (temp - t_ref);
+        pt.body_acc += buoy_corr;
+        if (heat_gain > 0.0) {
+            double speed = norm(pt.vel);
+            if (in && c[0] < H.nx && c[1] < H.ny && c[2] < H.nz)
+                H.src[c[0] + H.nx * (c[1] + H.ny * c[2])] += heat_gain * speed * dt;
+        }
+    }
+}
+
+// The velocity sampler couple_heat installs (world.rs:614-643): the velocity of
+// the particle nearest to q, the first one in index order on ties; particle 0
+// when no distance compares smaller than infinity; zero without particles.
+V3 nearest_velocity(const World& W, V3 q) {
+    if (W.particles.empty()) return v3(0.0, 0.0, 0.0);
+    size_t bi = 0;
+    double bd = INFINITY;
+    for (size_t i = 0; i < W.particles.size(); ++i) {
+        const V3 s = W.particles[i].pos;
+        double dxq = q.x - s.x, dyq = q.y - s.y, dzq = q.z - s.z;
+        double d = dxq * dxq + dyq * dyq + dzq * dzq;
+        if (d < bd) {
+            bd = d;
+            bi = i;
+        }
+    }
+    return W.particles[bi].vel;
+}
+
 void params_defaults(Params* p) {
     std::memset(p, 0, sizeof(*p));
     p->rest_density = 1000.0;
@@ -602,6 +903,73 @@ int or_world_flat_grid(void* wv, int64_t* min3, uint64_t* nc3, int32_t* cell_sta
 }
 
 // Wall-clock of n steps (seconds), for the CPU baseline.
+// Bodies: pose7 = pos xyz + quaternion w i j k (phy_ffi.h:147), vel3 in/out,
+// inv_mass, shape (0 sphere, 1 box, 2 capsule), dims3.
+void or_world_couple_bodies(void* w, size_t nb, const double* pose7, double* vel3,
+                            const double* inv_mass, const int32_t* shape, const double* dims3,
+                            double dt) {
+    std::vector<CBody> b(nb);
+    for (size_t k = 0; k < nb; ++k) {
+        const double* q = pose7 + 7 * k;
+        b[k].pos = v3(q[0], q[1], q[2]);
+        b[k].rot = Quat{q[3], q[4], q[5], q[6]};
+        b[k].vel = v3(vel3[3 * k], vel3[3 * k + 1], vel3[3 * k + 2]);
+        b[k].inv_mass = inv_mass[k];
+        b[k].shape = shape[k];
+        b[k].dims = v3(dims3[3 * k], dims3[3 * k + 1], dims3[3 * k + 2]);
+    }
+    couple_bodies(*(World*)w, b.data(), nb, dt);
+    for (size_t k = 0; k < nb; ++k) {
+        vel3[3 * k] = b[k].vel.x;
+        vel3[3 * k + 1] = b[k].vel.y;
+        vel3[3 * k + 2] = b[k].vel.z;
+    }
+}
+
+// Points: pos3, vel3, mass in; force3 in/out (untouched outside the bounds).
+void or_world_couple_points(void* w, size_t np, const double* pos3, const double* vel3,
+                            const double* mass, double* force3, double dt, double drag,
+                            double soft_density) {
+    std::vector<CPoint> p(np);
+    for (size_t k = 0; k < np; ++k) {
+        p[k].pos = v3(pos3[3 * k], pos3[3 * k + 1], pos3[3 * k + 2]);
+        p[k].vel = v3(vel3[3 * k], vel3[3 * k + 1], vel3[3 * k + 2]);
+        p[k].mass = mass[k];
+        p[k].force = v3(force3[3 * k], force3[3 * k + 1], force3[3 * k + 2]);
+    }
+    couple_points(*(World*)w, p.data(), np, dt, drag, soft_density);
+    for (size_t k = 0; k < np; ++k) {
+        force3[3 * k] = p[k].force.x;
+        force3[3 * k + 1] = p[k].force.y;
+        force3[3 * k + 2] = p[k].force.z;
+    }
+}
+
+void or_world_couple_heat(void* w, size_t nx, size_t ny, size_t nz, const double* origin3, double dx,
+                          const double* u, double* src, double dt, double t_ref, double beta,
+                          double heat_gain) {
+    CHeat H{nx, ny, nz, v3(origin3[0], origin3[1], origin3[2]), dx, u, src};
+    couple_heat(*(World*)w, H, dt, t_ref, beta, heat_gain);
+}
+
+void or_world_nearest_velocity(void* w, size_t nq, const double* q3, double* out3) {
+    const World& W = *(World*)w;
+    for (size_t k = 0; k < nq; ++k) {
+        V3 v = nearest_velocity(W, v3(q3[3 * k], q3[3 * k + 1], q3[3 * k + 2]));
+        out3[3 * k] = v.x;
+        out3[3 * k + 1] = v.y;
+        out3[3 * k + 2] = v.z;
+    }
+}
+
+// Body-frame helpers for the oracle's own tests.
+void or_quat_rotate(const double* q4, const double* v3in, double* out3) {
+    V3 r = qrot(Quat{q4[0], q4[1], q4[2], q4[3]}, v3(v3in[0], v3in[1], v3in[2]));
+    out3[0] = r.x;
+    out3[1] = r.y;
+    out3[2] = r.z;
+}
+
 double or_world_time_steps(void* w, double dt, int n) {
     auto t0 = std::chrono::steady_clock::now();
     for (int i = 0; i < n; ++i) ((World*)w)->step(dt);

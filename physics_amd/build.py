@@ -17,8 +17,10 @@ COPY_EXE = os.path.join(HERE, "hbm_copy")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["sph_kernels.hip", "sph_adjoint.hip", "fluid_device.cpp", "world.cpp", "archive.cpp",
-           "phy_ffi.cpp"]
+SOURCES = ["sph_kernels.hip", "sph_adjoint.hip", "sph_couple.hip", "fluid_device.cpp", "world.cpp",
+           "archive.cpp", "phy_ffi.cpp"]
+# the coupling's f64 arithmetic rounds operation by operation like the reference
+EXTRA = {"sph_couple.hip": ["-ffp-contract=off"]}
 
 # -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 generation.
 COMMON = [
@@ -40,7 +42,7 @@ def build(verbose: bool = False) -> str:
     for src in SOURCES:
         obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
         objs.append(obj)
-        cmd = [HIPCC, *COMMON, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+        cmd = [HIPCC, *COMMON, *EXTRA.get(src, []), "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

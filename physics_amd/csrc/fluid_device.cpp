@@ -932,4 +932,185 @@ void DeviceFluid::neighbor_stats(uint32_t* cnt, uint64_t* sum, uint64_t* xr) {
     sync();
 }
 
+// ---------------------------------------------------------------------------
+// pending body_acc and coupling
+
+void DeviceFluid::set_body_acc(const float* b3) {
+    if (n_ == 0) return;
+    std::vector<float4> b4(n_);
+    bool any = false;
+    for (size_t i = 0; i < n_; ++i) {
+        b4[i] = make_float4(b3[3 * i], b3[3 * i + 1], b3[3 * i + 2], 0.f);
+        any = any || b4[i].x != 0.f || b4[i].y != 0.f || b4[i].z != 0.f;
+    }
+    has_body_ = any;
+    if (!any) return;
+    stage4_.ensure(n_);
+    hip_check(hipMemcpyAsync(stage4_.get(), b4.data(), n_ * sizeof(float4), hipMemcpyHostToDevice, st_), "body_acc");
+    launch_gather_by_id4(st_, (int)n_, id_.get(), stage4_.get(), body_.get());
+    hip_check(hipGetLastError(), "body_acc");
+    sync();
+}
+
+void DeviceFluid::get_body_acc(float* out3) {
+    if (!has_body_) {
+        for (size_t k = 0; k < 3 * n_; ++k) out3[k] = 0.f;
+        return;
+    }
+    launch_unsort4(st_, (int)n_, id_.get(), body_.get(), stage4_.get());
+    std::vector<float4> h(n_);
+    hip_check(hipMemcpyAsync(h.data(), stage4_.get(), n_ * sizeof(float4), hipMemcpyDeviceToHost, st_), "body_acc");
+    sync();
+    for (size_t i = 0; i < n_; ++i) {
+        out3[3 * i] = h[i].x;
+        out3[3 * i + 1] = h[i].y;
+        out3[3 * i + 2] = h[i].z;
+    }
+}
+
+CoupleParams DeviceFluid::couple_params() const {
+    const HostParams& h = hps_[0];
+    CoupleParams c;
+    for (int a = 0; a < 3; ++a) {
+        c.lo[a] = h.bounds_min[a];
+        c.hi[a] = h.bounds_max[a];
+        c.g[a] = h.gravity[a];
+    }
+    c.rest = h.rest_density;
+    c.mass = h.mass;
+    c.inv_cell = Ps_[0].inv_cell;
+    c.h = h.h;
+    return c;
+}
+
+void DeviceFluid::one_scene(const char* what) const {
+    if (scn_.size() != 1) throw std::logic_error(std::string("phy-fluid: ") + what + " needs a one-scene world");
+}
+
+void DeviceFluid::sort_current() {
+    cdom_.ensure(scn_.size());
+    StepBufs sb = own_bufs();
+    sb.dom = cdom_.get();
+    run_domain_and_sort(sb);
+    inv_.ensure(n_);
+    launch_inverse_perm(st_, (int)n_, id_.get(), inv_.get());
+}
+
+void DeviceFluid::couple_bodies(CoupleBody* bodies, uint32_t nb, double dt) {
+    one_scene("couple_bodies");
+    if (n_ == 0 || nb == 0) return;  // world.rs:415-417
+    tape_invalidate();
+    const int n = (int)n_;
+    cbodies_.ensure(nb);
+    cdisp_.ensure(nb);
+    ccnt_.ensure(n_);
+    coff_.ensure(n_ + 1);
+    cn_.ensure(1);
+    cpartial_.ensure(kScanBlocks);
+    const uint32_t n32 = (uint32_t)n_;
+    std::vector<double> zero(nb, 0.0);
+    hip_check(hipMemcpyAsync(cbodies_.get(), bodies, nb * sizeof(CoupleBody), hipMemcpyHostToDevice, st_), "bodies");
+    hip_check(hipMemcpyAsync(cdisp_.get(), zero.data(), nb * sizeof(double), hipMemcpyHostToDevice, st_), "bodies");
+    hip_check(hipMemcpyAsync(cn_.get(), &n32, sizeof(uint32_t), hipMemcpyHostToDevice, st_), "bodies");
+    launch_contact_count(st_, n, posm_.get(), id_.get(), cbodies_.get(), (int)nb, ccnt_.get());
+    launch_exclusive_scan(st_, ccnt_.get(), cn_.get(), cpartial_.get(), coff_.get());
+    uint32_t K = 0;
+    hip_check(hipMemcpyAsync(&K, coff_.get() + n_, sizeof(uint32_t), hipMemcpyDeviceToHost, st_), "contacts");
+    sync();
+    contacts_.ensure(K ? K : 1);
+    if (K > 0)
+        launch_contact_fill(st_, n, posm_.get(), id_.get(), cbodies_.get(), (int)nb, coff_.get(), contacts_.get());
+    // with no contact only the buoyancy pass runs (every displaced volume is zero)
+    launch_contact_resolve(st_, contacts_.get(), coff_.get() + n_, posm_.get(), vel_.get(), cbodies_.get(), (int)nb,
+                           cdisp_.get(), couple_params(), dt);
+    hip_check(hipGetLastError(), "couple_bodies");
+    hip_check(hipMemcpyAsync(bodies, cbodies_.get(), nb * sizeof(CoupleBody), hipMemcpyDeviceToHost, st_), "bodies");
+    sync();
+}
+
+void DeviceFluid::couple_points(CouplePoint* pts, uint32_t np, double dt, double drag, double soft_density) {
+    one_scene("couple_points");
+    if (np == 0 || n_ == 0) return;  // world.rs:692-694
+    tape_invalidate();
+    const int n = (int)n_;
+    sort_current();
+    vel64_.ensure(3 * n_);
+    cpoints_.ensure(np);
+    launch_widen_vel(st_, n, vel_.get(), vel64_.get());
+    hip_check(hipMemcpyAsync(cpoints_.get(), pts, np * sizeof(CouplePoint), hipMemcpyHostToDevice, st_), "points");
+    launch_couple_points(st_, cpoints_.get(), (int)np, couple_params(), dt, drag, soft_density, posm_s_.get(),
+                         id_s_.get(), start_.get(), cdom_.get(), inv_.get(), vel64_.get());
+    launch_narrow_vel(st_, n, vel64_.get(), vel_.get());
+    hip_check(hipGetLastError(), "couple_points");
+    hip_check(hipMemcpyAsync(pts, cpoints_.get(), np * sizeof(CouplePoint), hipMemcpyDeviceToHost, st_), "points");
+    sync();
+}
+
+void DeviceFluid::couple_heat(const HeatGrid& hgh, double* src, double dt, double t_ref, double beta,
+                              double heat_gain) {
+    one_scene("couple_heat");
+    if (n_ == 0) return;  // world.rs:565-567
+    if (hgh.nx == 0 || hgh.ny == 0 || hgh.nz == 0) throw std::invalid_argument("couple_heat: empty heat grid");
+    const uint64_t ncell = (uint64_t)hgh.nx * hgh.ny * hgh.nz;
+    if (ncell >= 0xffffffffull) throw std::invalid_argument("couple_heat: heat grid too large");
+    if (heat_gain > 0.0 && !src) throw std::invalid_argument("couple_heat: heat_gain > 0 needs a source grid");
+    tape_invalidate();
+    const int n = (int)n_;
+    if (!has_body_) {
+        hip_check(hipMemsetAsync(body_.get(), 0, n_ * sizeof(float4), st_), "body_acc");
+        has_body_ = true;
+    }
+    heat_u_.ensure(ncell);
+    hip_check(hipMemcpyAsync(heat_u_.get(), hgh.u, ncell * sizeof(double), hipMemcpyHostToDevice, st_), "heat");
+    HeatGrid hg = hgh;
+    hg.u = heat_u_.get();
+    HeatScratch hs;
+    heat_q_.ensure(n_);
+    heat_qcell_.ensure(n_);
+    heat_max_.ensure(1);
+    heat_scale_.ensure(1);
+    heat_lo_.ensure(ncell);
+    heat_hi_.ensure(ncell);
+    heat_flags_.ensure(ncell);
+    hs.q = heat_q_.get();
+    hs.qcell = heat_qcell_.get();
+    hs.qmax = heat_max_.get();
+    hs.scale = heat_scale_.get();
+    hs.lo = heat_lo_.get();
+    hs.hi = heat_hi_.get();
+    hs.flags = heat_flags_.get();
+    hip_check(hipMemsetAsync(hs.qmax, 0, sizeof(unsigned long long), st_), "heat");
+    launch_heat_body(st_, n, posm_.get(), vel_.get(), body_.get(), couple_params(), hg, dt, t_ref, beta,
+                     heat_gain, hs);
+    if (heat_gain > 0.0) {
+        heat_src_.ensure(ncell);
+        hip_check(hipMemsetAsync(hs.lo, 0, ncell * sizeof(unsigned long long), st_), "heat");
+        hip_check(hipMemsetAsync(hs.hi, 0, ncell * sizeof(unsigned long long), st_), "heat");
+        hip_check(hipMemsetAsync(hs.flags, 0, ncell * sizeof(uint32_t), st_), "heat");
+        hip_check(hipMemcpyAsync(heat_src_.get(), src, ncell * sizeof(double), hipMemcpyHostToDevice, st_), "heat");
+        launch_heat_source(st_, n, dt, hs, (uint32_t)ncell, heat_src_.get());
+        hip_check(hipMemcpyAsync(src, heat_src_.get(), ncell * sizeof(double), hipMemcpyDeviceToHost, st_), "heat");
+    }
+    hip_check(hipGetLastError(), "couple_heat");
+    sync();
+}
+
+void DeviceFluid::sample_velocity(const double* q3, uint32_t nq, double* out3) {
+    one_scene("sample_velocity");
+    if (nq == 0) return;
+    if (n_ == 0) {
+        for (size_t k = 0; k < 3 * (size_t)nq; ++k) out3[k] = 0.0;
+        return;
+    }
+    sort_current();
+    cq_.ensure(6 * (size_t)nq);
+    hip_check(hipMemcpyAsync(cq_.get(), q3, 3 * (size_t)nq * sizeof(double), hipMemcpyHostToDevice, st_), "sampler");
+    launch_nearest_velocity(st_, (int)nq, cq_.get(), cq_.get() + 3 * (size_t)nq, posm_s_.get(), id_s_.get(),
+                            start_.get(), cdom_.get(), inv_.get(), vel_.get(), couple_params());
+    hip_check(hipGetLastError(), "sampler");
+    hip_check(hipMemcpyAsync(out3, cq_.get() + 3 * (size_t)nq, 3 * (size_t)nq * sizeof(double),
+                             hipMemcpyDeviceToHost, st_), "sampler");
+    sync();
+}
+
 }  // namespace phy

@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "sph_couple.h"
 #include "sph_kernels.h"
 #include "sph_types.h"
 
@@ -197,6 +198,28 @@ class DeviceFluid {
     uint32_t profile_end(double ms4[4]);
     int device() const { return dev_; }
 
+    // Pending body_acc (world.rs:356, 361), consumed by the next step. In
+    // original index order, 3 per particle; all zero clears it.
+    void set_body_acc(const float* b3);
+    void get_body_acc(float* out3);
+    bool has_body_acc() const { return has_body_; }
+
+    // Coupling of a one-scene world (world.rs:400-773, sph_couple.h). They act
+    // on the current state like the reference's calls after a step: particles
+    // change in place (position, velocity, or body_acc), rho / p / aux keep
+    // the last step's values, and a recording tape is invalidated.
+    // bodies: velocities updated in place.
+    void couple_bodies(CoupleBody* bodies, uint32_t nb, double dt);
+    // points: forces written (untouched outside the bounds).
+    void couple_points(CouplePoint* pts, uint32_t np, double dt, double drag, double soft_density);
+    // body_acc += -g beta (T - t_ref); src[nx*ny*nz] += heat_gain |v| dt per
+    // cell (host arrays; src may be null when heat_gain <= 0).
+    void couple_heat(const HeatGrid& host_grid, double* src, double dt, double t_ref, double beta,
+                     double heat_gain);
+    // velocity of the particle nearest to each query (the sampler couple_heat
+    // installs); zero without particles
+    void sample_velocity(const double* q3, uint32_t nq, double* out3);
+
    private:
     struct StepBufs {
         float4 *posm_s, *velm_s, *body_s;
@@ -237,6 +260,11 @@ class DeviceFluid {
     void load_state(const Snapshot& c);
     void tape_record(float dt);
     void tape_invalidate();
+    CoupleParams couple_params() const;
+    void one_scene(const char* what) const;
+    // bins the current positions into the step's sorted buffers with a domain
+    // of its own (the last step's watchdog stays intact) and builds inv
+    void sort_current();
 
     std::vector<HostParams> hps_;
     std::vector<DevParams> Ps_;
@@ -294,6 +322,17 @@ class DeviceFluid {
     DevBuf<float> rhob_loc_, rhob_, seed_;
     DevBuf<float2> mush_;
     DevBuf<double> adj_part_, adj_tot_;
+
+    // coupling scratch
+    DevBuf<Domain> cdom_;
+    DevBuf<uint32_t> inv_, ccnt_, coff_, cn_, cpartial_;
+    DevBuf<Contact> contacts_;
+    DevBuf<CoupleBody> cbodies_;
+    DevBuf<CouplePoint> cpoints_;
+    DevBuf<double> cdisp_, vel64_, heat_u_, heat_src_, heat_q_, cq_;
+    DevBuf<uint32_t> heat_qcell_, heat_flags_;
+    DevBuf<unsigned long long> heat_lo_, heat_hi_, heat_max_;
+    DevBuf<int> heat_scale_;
 };
 
 }  // namespace phy

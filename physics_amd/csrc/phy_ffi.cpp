@@ -446,6 +446,91 @@ int32_t phy_fluid_set_body_acc(PhyWorldHandle* w, const double* b3, uintptr_t le
     });
 }
 
+int32_t phy_fluid_get_body_acc(PhyWorldHandle* w, double* out3, uintptr_t len) {
+    return guard<int32_t>(-1, [&] {
+        FluidWorld& f = need_fluid(w)->world();
+        if (!out3 || len < 3 * f.len()) throw std::invalid_argument("body_acc buffer too short");
+        f.get_body_acc(out3);
+        return 0;
+    });
+}
+
+// ---- coupling (world.rs:400-773) ----
+
+int32_t phy_fluid_couple_bodies(PhyWorldHandle* w, PhyCoupleBody* bodies, uint32_t n, double dt,
+                                double friction) {
+    return guard<int32_t>(-1, [&] {
+        FluidWorld& f = need_fluid(w)->world();
+        if (n && !bodies) throw std::invalid_argument("couple_bodies: null bodies");
+        std::vector<phy::CoupleBody> b(n);
+        for (uint32_t k = 0; k < n; ++k) {
+            const PhyCoupleBody& s = bodies[k];
+            if (s.shape < PHY_SHAPE_SPHERE || s.shape > PHY_SHAPE_CAPSULE)
+                throw std::invalid_argument("couple_bodies: unknown shape kind");
+            for (int a = 0; a < 3; ++a) {
+                b[k].pos[a] = s.pose[a];
+                b[k].vel[a] = s.vel[a];
+                b[k].dims[a] = s.dims[a];
+            }
+            for (int a = 0; a < 4; ++a) b[k].q[a] = s.pose[3 + a];
+            b[k].inv_mass = s.inv_mass;
+            b[k].shape = s.shape;
+            b[k].pad = 0;
+        }
+        f.couple_bodies(b.data(), n, dt, friction);
+        for (uint32_t k = 0; k < n; ++k)
+            for (int a = 0; a < 3; ++a) bodies[k].vel[a] = b[k].vel[a];
+        return 0;
+    });
+}
+
+int32_t phy_fluid_couple_points(PhyWorldHandle* w, PhyCouplePoint* pts, uint32_t n, double dt, double drag,
+                                double soft_density) {
+    return guard<int32_t>(-1, [&] {
+        FluidWorld& f = need_fluid(w)->world();
+        if (n && !pts) throw std::invalid_argument("couple_points: null points");
+        std::vector<phy::CouplePoint> p(n);
+        for (uint32_t k = 0; k < n; ++k) {
+            for (int a = 0; a < 3; ++a) {
+                p[k].pos[a] = pts[k].pos[a];
+                p[k].vel[a] = pts[k].vel[a];
+                p[k].force[a] = pts[k].force[a];
+            }
+            p[k].mass = pts[k].mass;
+        }
+        f.couple_points(p.data(), n, dt, drag, soft_density);
+        for (uint32_t k = 0; k < n; ++k)
+            for (int a = 0; a < 3; ++a) pts[k].force[a] = p[k].force[a];
+        return 0;
+    });
+}
+
+int32_t phy_fluid_couple_heat(PhyWorldHandle* w, const PhyHeatGrid* grid, double dt, double t_ref, double beta,
+                              double heat_gain) {
+    return guard<int32_t>(-1, [&] {
+        FluidWorld& f = need_fluid(w)->world();
+        if (!grid || !grid->u) throw std::invalid_argument("couple_heat: null grid");
+        phy::HeatGrid g;
+        g.nx = grid->nx;
+        g.ny = grid->ny;
+        g.nz = grid->nz;
+        for (int a = 0; a < 3; ++a) g.origin[a] = grid->origin[a];
+        g.dx = grid->dx;
+        g.u = grid->u;
+        f.couple_heat(g, grid->src, dt, t_ref, beta, heat_gain);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_sample_velocity(PhyWorldHandle* w, const double* q3, uint32_t n, double* out3) {
+    return guard<int32_t>(-1, [&] {
+        FluidWorld& f = need_fluid(w)->world();
+        if (n && (!q3 || !out3)) throw std::invalid_argument("sample_velocity: null buffer");
+        f.sample_velocity(q3, n, out3);
+        return 0;
+    });
+}
+
 int32_t phy_fluid_diagnostics_f64(PhyWorldHandle* w, double* out5) {
     return guard<int32_t>(-1, [&] {
         if (!out5) throw std::invalid_argument("null out");

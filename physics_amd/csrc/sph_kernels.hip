@@ -28,6 +28,7 @@
 
 #include "sph_types.h"
 #include "sph_device.h"
+#include "sph_couple.h"
 #include "sph_kernels.h"
 
 namespace phy {
@@ -737,6 +738,13 @@ void launch_sort(hipStream_t st, const SceneTable& T, const SortBuffers& b) {
         hipLaunchKernelGGL(k_reorder, dim3(T.ntiles), dim3(256), 0, st, T, b.tmp, b.idtmp, b.cell,
                            b.start, b.posm_in, b.vel_in, b.mat_in, b.body_in, b.posm_out, b.velm_out,
                            b.id_out, b.mat_out, b.body_out, b.cell_out);
+}
+
+void launch_exclusive_scan(hipStream_t st, uint32_t* counts, const uint32_t* total, uint32_t* partial,
+                           uint32_t* start) {
+    hipLaunchKernelGGL(k_scan_partial, dim3(kScanBlocks), dim3(256), 0, st, counts, total, partial);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, partial, kScanBlocks, total, start);
+    hipLaunchKernelGGL(k_scan_final, dim3(kScanBlocks), dim3(256), 0, st, counts, total, partial, start);
 }
 
 bool tuning_valid(const Tuning& t) {

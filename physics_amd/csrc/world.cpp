@@ -156,9 +156,10 @@ std::vector<Particle> FluidWorld::particles() {
         dev_.download_material(m.data());
         for (size_t i = 0; i < n; ++i) out[i].material = m[i];
     }
-    if (body_pending_.size() == 3 * n)
-        for (size_t i = 0; i < n; ++i)
-            out[i].body_acc = Vec3{body_pending_[3 * i], body_pending_[3 * i + 1], body_pending_[3 * i + 2]};
+    if (dev_.has_body_acc()) {
+        dev_.get_body_acc(a.data());
+        put3(&Particle::body_acc);
+    }
     if (dev_.density_valid()) {
         dev_.download(F_RHO, s.data());
         for (size_t i = 0; i < n; ++i) out[i].rho = s[i];
@@ -228,16 +229,6 @@ double FluidWorld::effective_viscosity(size_t i) {
 
 void FluidWorld::step(double dt) {
     flush_host();
-    if (!body_pending_.empty()) {
-        // re-upload with the body accelerations (rare: coupling callers)
-        std::vector<Particle> ps = particles();
-        for (size_t i = 0; i < ps.size(); ++i)
-            ps[i].body_acc = Vec3{body_pending_[3 * i], body_pending_[3 * i + 1], body_pending_[3 * i + 2]};
-        body_pending_.clear();
-        host_ = std::move(ps);
-        host_dirty_ = true;
-        flush_host();
-    }
     dev_.step((float)dt);
     last_was_step_ = true;
 }
@@ -284,7 +275,6 @@ void FluidWorld::upload_f32(size_t n, const float* pos3, const float* vel3, cons
                             const uint32_t* material) {
     host_.clear();
     host_dirty_ = false;
-    body_pending_.clear();
     dev_.upload(n, pos3, vel3, mass, material, nullptr);
     last_was_step_ = false;
 }
@@ -292,15 +282,51 @@ void FluidWorld::upload_f32(size_t n, const float* pos3, const float* vel3, cons
 void FluidWorld::set_body_acc(const double* b3) {
     flush_host();
     size_t n = dev_.size();
-    body_pending_.assign(3 * n, 0.f);
-    for (size_t i = 0; i < 3 * n; ++i) body_pending_[i] = (float)b3[i];
+    std::vector<float> b(3 * n);
+    for (size_t i = 0; i < 3 * n; ++i) b[i] = (float)b3[i];
+    dev_.set_body_acc(b.data());
+}
+
+void FluidWorld::get_body_acc(double* out3) {
+    flush_host();
+    const size_t n = dev_.size();
+    std::vector<float> b(3 * n);
+    dev_.get_body_acc(b.data());
+    for (size_t i = 0; i < 3 * n; ++i) out3[i] = b[i];
+}
+
+void FluidWorld::couple_bodies(CoupleBody* bodies, size_t n, double dt, double /*friction*/) {
+    flush_host();
+    if (n > 0xffffffffull) throw std::invalid_argument("couple_bodies: too many bodies");
+    dev_.couple_bodies(bodies, (uint32_t)n, dt);
+    last_was_step_ = false;
+}
+
+void FluidWorld::couple_points(CouplePoint* pts, size_t n, double dt, double drag, double soft_density) {
+    flush_host();
+    if (n > 0xffffffffull) throw std::invalid_argument("couple_points: too many points");
+    dev_.couple_points(pts, (uint32_t)n, dt, drag, soft_density);
+    last_was_step_ = false;
+}
+
+void FluidWorld::couple_heat(const HeatGrid& grid, double* src, double dt, double t_ref, double beta,
+                             double heat_gain) {
+    flush_host();
+    // positions and velocities are untouched: the fused watchdog still holds
+    dev_.couple_heat(grid, src, dt, t_ref, beta, heat_gain);
+}
+
+void FluidWorld::sample_velocity(const double* q3, size_t n, double* out3) {
+    flush_host();
+    if (n > 0xffffffffull) throw std::invalid_argument("sample_velocity: too many queries");
+    dev_.sample_velocity(q3, (uint32_t)n, out3);
 }
 
 void FluidWorld::set_keep_aux(bool on) { dev_.set_keep_aux(on); }
 
 void FluidWorld::tape_begin(uint32_t K) {
     flush_host();
-    if (!body_pending_.empty()) throw std::invalid_argument("tape: body_acc is not differentiated");
+    if (dev_.has_body_acc()) throw std::invalid_argument("tape: body_acc is not differentiated");
     dev_.tape_begin(K);
 }
 

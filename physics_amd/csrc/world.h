@@ -102,6 +102,17 @@ class FluidWorld {
                     const uint32_t* material);
     // body_acc accumulator (world.rs:356, 361): added once by the next step
     void set_body_acc(const double* b3);
+    void get_body_acc(double* out3);
+
+    // Coupling (world.rs:400-773) on the current state, as the reference's
+    // FluidWorld::couple_bodies / couple_points / couple_heat (and the
+    // velocity sampler couple_heat installs); see DeviceFluid::couple_*.
+    // friction is accepted and unused, as in the reference (push_out ignores it).
+    void couple_bodies(CoupleBody* bodies, size_t n, double dt, double friction);
+    void couple_points(CouplePoint* pts, size_t n, double dt, double drag, double soft_density);
+    void couple_heat(const HeatGrid& grid, double* src, double dt, double t_ref, double beta,
+                     double heat_gain);
+    void sample_velocity(const double* q3, size_t n, double* out3);
 
     void set_keep_aux(bool on);
     // adjoint tape (see DeviceFluid::tape_*); cotangents in original index
@@ -145,7 +156,6 @@ class FluidWorld {
     bool host_dirty_ = false;
     bool last_was_step_ = false;
     std::vector<Particle> host_;  // pending host-side edits (valid when host_dirty_)
-    std::vector<float> body_pending_;
 };
 
 class FluidSubsystem : public Subsystem {

@@ -50,7 +50,9 @@ EXPORTS = [
     "phy_fluid_batch_profile_end", "phy_device_mem_info", "phy_fluid_get_tuning",
     "phy_fluid_set_tuning", "phy_fluid_autotune", "phy_fluid_batch_get_tuning",
     "phy_fluid_batch_set_tuning", "phy_fluid_batch_autotune", "phy_fluid_flat_export",
-    "phy_fluid_tape_backward_full", "phy_fluid_batch_tape_backward_full",
+    "phy_fluid_tape_backward_full", "phy_fluid_batch_tape_backward_full", "phy_fluid_get_body_acc",
+    "phy_fluid_couple_bodies", "phy_fluid_couple_points", "phy_fluid_couple_heat",
+    "phy_fluid_sample_velocity",
 ]
 
 
@@ -104,6 +106,31 @@ class PhyFlatInfo(C.Structure):
         ("gravity", C.c_float * 3), ("n_visc", C.c_uint32), ("visc_k", C.c_float * MAX_MAT),
         ("visc_n", C.c_float * MAX_MAT),
     ]
+
+
+SHAPES = {"sphere": 0, "box": 1, "capsule": 2}
+
+
+class PhyCoupleBody(C.Structure):
+    """Body<f64> as couple_bodies reads it: pose = position + quaternion w i j k."""
+
+    _fields_ = [("pose", C.c_double * 7), ("vel", C.c_double * 3), ("inv_mass", C.c_double),
+                ("shape", C.c_int32), ("_pad", C.c_int32), ("dims", C.c_double * 3)]
+
+
+class PhyCouplePoint(C.Structure):
+    """CouplePoint<f64> (sph/couple.rs:10-19)."""
+
+    _fields_ = [("pos", C.c_double * 3), ("vel", C.c_double * 3), ("mass", C.c_double),
+                ("force", C.c_double * 3)]
+
+
+class PhyHeatGrid(C.Structure):
+    """Temperature grid (x fastest) and its source accumulator."""
+
+    _fields_ = [("nx", C.c_uint32), ("ny", C.c_uint32), ("nz", C.c_uint32), ("_pad", C.c_uint32),
+                ("origin", C.c_double * 3), ("dx", C.c_double), ("u", C.POINTER(C.c_double)),
+                ("src", C.POINTER(C.c_double))]
 
 
 class PhyTuning(C.Structure):
@@ -177,6 +204,14 @@ def lib():
         "phy_fluid_get_field_f32": ([H, C.c_int32, fp, up], up),
         "phy_fluid_set_keep_aux": ([H, C.c_int32], C.c_int32),
         "phy_fluid_set_body_acc": ([H, dp, up], C.c_int32),
+        "phy_fluid_get_body_acc": ([H, dp, up], C.c_int32),
+        "phy_fluid_couple_bodies": ([H, C.POINTER(PhyCoupleBody), C.c_uint32, C.c_double, C.c_double],
+                                    C.c_int32),
+        "phy_fluid_couple_points": ([H, C.POINTER(PhyCouplePoint), C.c_uint32, C.c_double, C.c_double,
+                                     C.c_double], C.c_int32),
+        "phy_fluid_couple_heat": ([H, C.POINTER(PhyHeatGrid), C.c_double, C.c_double, C.c_double, C.c_double],
+                                  C.c_int32),
+        "phy_fluid_sample_velocity": ([H, dp, C.c_uint32, dp], C.c_int32),
         "phy_fluid_diagnostics_f64": ([H, dp], C.c_int32),
         "phy_fluid_neighbor_stats": ([H, u32p, u64p, u64p, up], C.c_int32),
         "phy_fluid_sync": ([H], C.c_int32),
@@ -384,6 +419,75 @@ class FluidWorld(World):
         b = np.ascontiguousarray(b, dtype=np.float64).reshape(-1)
         if lib().phy_fluid_set_body_acc(self._h, _dp(b), b.size) != 0:
             raise PhyError(last_error())
+
+    def body_acc(self) -> np.ndarray:
+        """The pending body_acc (zero after a step)."""
+        out = np.zeros((len(self), 3))
+        if lib().phy_fluid_get_body_acc(self._h, _dp(out), out.size) != 0:
+            raise PhyError(last_error())
+        return out
+
+    # ---- coupling (world.rs:400-773) ----
+
+    def couple_bodies(self, pose7, vel3, inv_mass, shape, dims3, dt: float, friction: float = 0.0):
+        """FluidWorld::couple_bodies; returns the bodies' new velocities."""
+        pose = np.asarray(pose7, dtype=np.float64).reshape(-1, 7)
+        nb = pose.shape[0]
+        vel = np.asarray(vel3, dtype=np.float64).reshape(nb, 3)
+        im = np.asarray(inv_mass, dtype=np.float64).reshape(nb)
+        sh = np.asarray(shape, dtype=np.int32).reshape(nb)
+        dm = np.asarray(dims3, dtype=np.float64).reshape(nb, 3)
+        arr = (PhyCoupleBody * max(nb, 1))()
+        for k in range(nb):
+            arr[k].pose[:] = list(pose[k])
+            arr[k].vel[:] = list(vel[k])
+            arr[k].inv_mass = float(im[k])
+            arr[k].shape = int(sh[k])
+            arr[k].dims[:] = list(dm[k])
+        if lib().phy_fluid_couple_bodies(self._h, arr, nb, dt, friction) != 0:
+            raise PhyError(last_error())
+        return np.array([list(arr[k].vel) for k in range(nb)]).reshape(nb, 3)
+
+    def couple_points(self, pos3, vel3, mass, force3, dt: float, drag: float, soft_density: float):
+        """FluidWorld::couple_points; returns the points' forces."""
+        pos = np.asarray(pos3, dtype=np.float64).reshape(-1, 3)
+        n = pos.shape[0]
+        vel = np.asarray(vel3, dtype=np.float64).reshape(n, 3)
+        m = np.broadcast_to(np.asarray(mass, dtype=np.float64), (n,))
+        f = np.asarray(force3, dtype=np.float64).reshape(n, 3)
+        arr = (PhyCouplePoint * max(n, 1))()
+        for k in range(n):
+            arr[k].pos[:] = list(pos[k])
+            arr[k].vel[:] = list(vel[k])
+            arr[k].mass = float(m[k])
+            arr[k].force[:] = list(f[k])
+        if lib().phy_fluid_couple_points(self._h, arr, n, dt, drag, soft_density) != 0:
+            raise PhyError(last_error())
+        return np.array([list(arr[k].force) for k in range(n)]).reshape(n, 3)
+
+    def couple_heat(self, u, origin, dx: float, src, dt: float, t_ref: float, beta: float,
+                    heat_gain: float):
+        """FluidWorld::couple_heat against u[z, y, x]; returns the source grid."""
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        nz, ny, nx = u.shape
+        out = np.array(src, dtype=np.float64).reshape(nz, ny, nx)
+        g = PhyHeatGrid()
+        g.nx, g.ny, g.nz = nx, ny, nz
+        g.origin[:] = [float(v) for v in origin]
+        g.dx = dx
+        g.u = _dp(u)
+        g.src = _dp(out)
+        if lib().phy_fluid_couple_heat(self._h, C.byref(g), dt, t_ref, beta, heat_gain) != 0:
+            raise PhyError(last_error())
+        return out
+
+    def sample_velocity(self, q3) -> np.ndarray:
+        """The nearest-particle velocity sampler couple_heat installs."""
+        q = np.ascontiguousarray(q3, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros_like(q)
+        if lib().phy_fluid_sample_velocity(self._h, _dp(q), q.shape[0], _dp(out)) != 0:
+            raise PhyError(last_error())
+        return out
 
     def diagnostics(self) -> np.ndarray:
         out = np.zeros(5)
