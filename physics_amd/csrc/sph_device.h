@@ -290,6 +290,10 @@ struct Cursor {
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
+// true when the condition holds on any active lane; the lane mask stays in
+// scalar registers (no vector-register round trip)
+__device__ __forceinline__ bool wave_any(bool b) { return __builtin_amdgcn_ballot_w64(b) != 0ull; }
+
 // Element idx of an array with a wave-uniform base: a 32-bit byte offset from
 // the base (arrays here hold < 2^27 elements of <= 16 B), so the load can use
 // the scalar-base + 32-bit-offset address form instead of a 64-bit per-lane
@@ -315,7 +319,7 @@ enum GatherMode { kAllWaves = 0, kUniformWaves = 1, kOtherWaves = 2 };
 
 __device__ __forceinline__ bool wave_uniform(bool act, uint32_t ci, const Domain& d) {
     const uint32_t ci0 = uni(ci);
-    return ci0 < d.ncells && __ballot(act && ci != ci0) == 0ull;
+    return ci0 < d.ncells && !wave_any(act && ci != ci0);
 }
 
 // S: list slots drained per round (1 or 2; 2 keeps two neighbour loads in
@@ -330,7 +334,15 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
     static_assert(S == 1 || S == 2, "drain width");
     int cnt = 0;
     auto drain = [&]() {
-        for (int k = 0; __ballot(k < cnt) != 0ull; k += S) {
+        // do-while: the loop test sits at the bottom, so the accumulators
+        // stay in place across iterations
+        if (!wave_any(0 < cnt)) {
+            cnt = 0;
+            flush();
+            return;
+        }
+        int k = 0;
+        do {
             if (S == 2) {
                 const bool v0 = k < cnt, v1 = k + 1 < cnt;
                 const uint32_t j0 = v0 ? lst[k][tid] : self_j;
@@ -345,7 +357,8 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 auto t0 = load(j0);
                 proc(j0, t0, v0);
             }
-        }
+            k += S;
+        } while (wave_any(k < cnt));
         cnt = 0;
         flush();
     };
@@ -381,14 +394,14 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 push(j1, t1);
                 push(j2, t2);
                 push(j3, t3);
-                if (__ballot(cnt > L - 4) != 0ull) drain();
+                if (wave_any(cnt > L - 4)) drain();
             }
         }
         const uint32_t o0 = uni(start[d.ncells]), o1 = uni(start[d.ncells + 1]);
         for (uint32_t j = o0; j < o1; ++j) {
             const float4 pj = posm[j];
             push(j, act && ov_ok(pj) && test(j, pj));
-            if (__ballot(cnt == L) != 0ull) drain();
+            if (wave_any(cnt == L)) drain();
         }
     } else if (MODE != kUniformWaves) {
         Cursor cur;
@@ -396,7 +409,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
         // up to four consecutive candidates of the lane's current run per
         // round: four independent loads in flight instead of one dependent
         // load per candidate
-        while (__ballot(act && cur.j < cur.jend) != 0ull) {
+        while (wave_any(act && cur.j < cur.jend)) {
             if (act && cur.j < cur.jend) {
                 const uint32_t j = cur.j, last = cur.jend - 1;
                 const uint32_t j1 = min(j + 1, last), j2 = min(j + 2, last), j3 = min(j + 3, last);
@@ -415,7 +428,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 cur.j = min(j + 4, cur.jend);
                 if (cur.j >= cur.jend) cur.next_run(d, start);
             }
-            if (__ballot(cnt > L - 4) != 0ull) drain();
+            if (wave_any(cnt > L - 4)) drain();
         }
     }
     drain();
@@ -453,9 +466,9 @@ __device__ __forceinline__ bool density_valid(const DevParams& P, bool listed, f
     const bool fast = s32 < P.s_lo;
     const bool need = listed && !fast;
     bool ex = false;
-    if (__ballot(need) != 0ull) {
-        if (need) ex = exact_s64(xi, yi, zi, pj.x, pj.y, pj.z) < P.s_thresh;
-    }
+    // a divergent branch around the out-of-line f64 call: the wave skips it
+    // when no lane needs it
+    if (need) ex = exact_s64(xi, yi, zi, pj.x, pj.y, pj.z) < P.s_thresh;
     return listed && (fast || ex);
 }
 
@@ -470,12 +483,10 @@ __device__ __forceinline__ bool force_valid(const DevParams& P, bool listed, uin
     const bool fast = s32 < P.s_lo && s32 > 0.f;  // s32 > 0: distinct points, so j != i
     const bool need = listed && j != i && !fast;
     bool ex = false;
-    if (__ballot(need) != 0ull) {
-        if (need) {
-            const double e = exact_s64(xi, yi, zi, pj.x, pj.y, pj.z);
-            const bool nz = xi != pj.x || yi != pj.y || zi != pj.z;
-            ex = (e < P.s_thresh && nz) || e != e;
-        }
+    if (need) {
+        const double e = exact_s64(xi, yi, zi, pj.x, pj.y, pj.z);
+        const bool nz = xi != pj.x || yi != pj.y || zi != pj.z;
+        ex = (e < P.s_thresh && nz) || e != e;
     }
     return listed && (fast || ex);
 }
