@@ -19,9 +19,14 @@ void hip_check(hipError_t e, const char* what) {
 }
 
 // to_i64 of the floored key, as in the kernels (grid.rs:26-33)
+// Half-cell key of the dense table (sph_device.h hkey) on the host; 0 where
+// the key fails (such positions are binned to the overflow segment).
 static long long host_key(float x, double inv) {
-    double f = std::floor((double)x * inv);
-    if (f >= -9223372036854775808.0 && f < 9223372036854775808.0) return (long long)f;
+    const double v = (double)x * inv;
+    const double f = std::floor(v * 2.0);
+    if (f >= -9223372036854775808.0 && f < 9223372036854775808.0 && v >= -9223372036854775808.0 &&
+        v < 9223372036854775808.0)
+        return (long long)f;
     return 0;
 }
 

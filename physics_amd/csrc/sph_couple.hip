@@ -289,43 +289,36 @@ __device__ __forceinline__ long long key_d(double x, double inv) {
     return 0;
 }
 
-// Up to 27 dense cell runs of a point's neighbourhood plus the overflow
-// segment (checked by key adjacency), flattened for the workgroup.
+// The dense runs of a point's 27 reference cells (as half cells, up to 36
+// z-runs) plus the overflow segment (checked by key adjacency), flattened for
+// the workgroup.
 struct Runs {
-    uint32_t a[28], len[28];
+    uint32_t a[37], len[37];
     uint32_t total;
     int nr;
 };
 
 __device__ void point_runs(const D3& q, const CoupleParams& cp, const uint32_t* __restrict__ start,
                            const Domain& d, Runs& R) {
-    const long long k0 = key_d(q.x, cp.inv_cell), k1 = key_d(q.y, cp.inv_cell), k2 = key_d(q.z, cp.inv_cell);
+    long long lo[3], hi[3];
+    ref_block(key_d(q.x, cp.inv_cell), d, 0, lo[0], hi[0]);
+    ref_block(key_d(q.y, cp.inv_cell), d, 1, lo[1], hi[1]);
+    ref_block(key_d(q.z, cp.inv_cell), d, 2, lo[2], hi[2]);
     R.nr = 0;
     R.total = 0;
-    for (int di = -1; di <= 1; ++di)
-        for (int dj = -1; dj <= 1; ++dj)
-            for (int dk = -1; dk <= 1; ++dk) {
-                const unsigned long long x = (unsigned long long)k0 + (unsigned long long)(long long)di -
-                                             (unsigned long long)d.kmin[0];
-                const unsigned long long y = (unsigned long long)k1 + (unsigned long long)(long long)dj -
-                                             (unsigned long long)d.kmin[1];
-                const unsigned long long z = (unsigned long long)k2 + (unsigned long long)(long long)dk -
-                                             (unsigned long long)d.kmin[2];
-                if (x >= (unsigned long long)d.dims[0] || y >= (unsigned long long)d.dims[1] ||
-                    z >= (unsigned long long)d.dims[2])
-                    continue;
-                const uint32_t c = (uint32_t)((x * d.dims[1] + y) * d.dims[2] + z);
-                const uint32_t a = start[c], b = start[c + 1];
-                if (b > a) {
-                    R.a[R.nr] = a;
-                    R.len[R.nr] = b - a;
-                    R.total += b - a;
-                    ++R.nr;
-                }
-            }
+    for (int k = 0; k < 36; ++k) {
+        uint32_t a, b;
+        block_run(k, lo, hi, d, start, a, b);
+        if (b > a) {
+            R.a[R.nr] = a;
+            R.len[R.nr] = b - a;
+            R.total += b - a;
+            ++R.nr;
+        }
+    }
     const uint32_t o0 = start[d.ncells], o1 = start[d.ncells + 1];
-    R.a[27] = o0;
-    R.len[27] = o1 - o0;
+    R.a[36] = o0;
+    R.len[36] = o1 - o0;
 }
 
 __device__ __forceinline__ bool point_adjacent(const D3& q, const float4& p, double inv) {
@@ -347,8 +340,8 @@ __device__ __forceinline__ void for_point_candidates(const Runs& R, const D3& q,
         while (u >= R.len[r]) u -= R.len[r++];
         f(R.a[r] + u);
     }
-    for (uint32_t t = threadIdx.x; t < R.len[27]; t += blockDim.x) {
-        const uint32_t j = R.a[27] + t;
+    for (uint32_t t = threadIdx.x; t < R.len[36]; t += blockDim.x) {
+        const uint32_t j = R.a[36] + t;
         if (point_adjacent(q, posm_s[j], inv)) f(j);
     }
 }
@@ -601,10 +594,10 @@ void launch_heat_source(hipStream_t st, int n, double dt, const HeatScratch& hs,
 
 // ---------------------------------------------------------------------------
 // velocity sampler: exact nearest particle (first in index order on ties) by
-// rings of cells around the query's cell; rings are visited until the best
-// squared distance is below ((r - 1) h)^2, a full cell of margin over the
-// smallest distance an unvisited ring can hold. The overflow segment is
-// always scanned.
+// rings of half cells around the query's half cell; rings are visited until
+// the best squared distance is below ((r - 1) h / 2)^2, a full half cell of
+// margin over the smallest distance an unvisited ring can hold. The overflow
+// segment is always scanned.
 
 __device__ __forceinline__ void nearest_consider(const D3& q, const float4& p, uint32_t o, double& bd,
                                                  uint32_t& bo) {
@@ -630,8 +623,16 @@ __global__ __launch_bounds__(256) void k_nearest_velocity(int nq, const double* 
     start += d.toff;
     double bd = __longlong_as_double(0x7ff0000000000000ll);  // +inf
     uint32_t bo = 0xffffffffu;
-    if (isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) {
-        const long long kq[3] = {key_d(q.x, cp.inv_cell), key_d(q.y, cp.inv_cell), key_d(q.z, cp.inv_cell)};
+    long long kq[3];
+    bool keyed = hkey(q.x * cp.inv_cell, kq[0]) & hkey(q.y * cp.inv_cell, kq[1]) &
+                 hkey(q.z * cp.inv_cell, kq[2]);
+    for (int a = 0; a < 3; ++a)  // far from the table: no ring walk (and no key overflow)
+        keyed = keyed && fabs((double)kq[a] - (double)d.kmin[a]) < 1099511627776.0;
+    if (isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && !keyed) {
+        // a finite query beyond the key range or far from the table: every particle
+        for (uint32_t j = start[0]; j < start[d.ncells + 1]; ++j)
+            nearest_consider(q, posm_s[j], id_s[j], bd, bo);
+    } else if (keyed && isfinite(q.x) && isfinite(q.y) && isfinite(q.z)) {
         // query key relative to the table, and the ring range that meets it
         long long rel[3], rmin = 0, rmax = 0;
         for (int a = 0; a < 3; ++a) {
@@ -667,7 +668,7 @@ __global__ __launch_bounds__(256) void k_nearest_velocity(int nq, const double* 
                     }
                 }
             if (r >= 1) {
-                const double m = (double)(r - 1) * cp.h;
+                const double m = (double)(r - 1) * (0.5 * cp.h);  // half cells
                 if (bd < m * m) break;
             }
         }

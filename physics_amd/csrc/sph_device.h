@@ -23,6 +23,29 @@ __device__ __forceinline__ long long key1(float x, double inv) {
     return 0;
 }
 
+// The dense table bins particles into half cells of size h / 2: the half key
+// floor(2 fl(x (1/h))) is an exact refinement of the reference key (scaling by
+// 2 is exact, so key1 = hkey >> 1). Returns false where the reference key
+// fails (NaN, out of the i64 range): such particles live in the overflow
+// segment, which is matched by reference-key adjacency.
+__device__ __forceinline__ bool hkey(double v, long long& k) {
+    const double f = floor(v * 2.0);
+    if (f >= -9223372036854775808.0 && f < 9223372036854775808.0 && v >= -9223372036854775808.0 &&
+        v < 9223372036854775808.0) {
+        k = (long long)f;
+        return true;
+    }
+    k = 0;
+    return false;
+}
+
+// A dense particle's candidates lie within +-kReach half cells per axis: a
+// pair with r < h has |fl(x_j/h) - fl(x_i/h)| < 1 up to rounding, i.e. half
+// keys at most 2 apart (see the neighbour-search notes of the design).
+constexpr int kReach = 2;
+constexpr int kSpan = 2 * kReach + 1;  // half cells per row of the window
+constexpr int kRows = kSpan * kSpan;   // rows (x, y) of the window
+
 __device__ __forceinline__ bool finite3(float x, float y, float z) {
     return isfinite(x) && isfinite(y) && isfinite(z);
 }
@@ -112,21 +135,50 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 }
 
 __device__ __forceinline__ uint32_t cell_of(float x, float y, float z, double inv, const Domain& d) {
+    long long h0, h1, h2;
+    const bool ok = hkey((double)x * inv, h0) & hkey((double)y * inv, h1) & hkey((double)z * inv, h2);
     // wrapping differences: an out-of-domain key maps to a huge unsigned value
-    unsigned long long k0 = (unsigned long long)key1(x, inv) - (unsigned long long)d.kmin[0];
-    unsigned long long k1 = (unsigned long long)key1(y, inv) - (unsigned long long)d.kmin[1];
-    unsigned long long k2 = (unsigned long long)key1(z, inv) - (unsigned long long)d.kmin[2];
-    if (k0 >= (unsigned long long)d.dims[0] || k1 >= (unsigned long long)d.dims[1] ||
+    const unsigned long long k0 = (unsigned long long)h0 - (unsigned long long)d.kmin[0];
+    const unsigned long long k1 = (unsigned long long)h1 - (unsigned long long)d.kmin[1];
+    const unsigned long long k2 = (unsigned long long)h2 - (unsigned long long)d.kmin[2];
+    if (!ok || k0 >= (unsigned long long)d.dims[0] || k1 >= (unsigned long long)d.dims[1] ||
         k2 >= (unsigned long long)d.dims[2])
         return d.ncells;
     return (uint32_t)((k0 * d.dims[1] + k1) * d.dims[2] + k2);
 }
 
+// Half-key interval (table-relative, per axis) of the 27 reference cells
+// around reference key k: the reference keys k - 1, k, k + 1 (wrapping, like
+// grid.rs:44-48) that fall inside the table, as half cells. Empty: lo > hi.
+__device__ __forceinline__ void ref_block(long long k, const Domain& d, int a, long long& lo,
+                                          long long& hi) {
+    const long long hlo = d.kmin[a] >> 1, hhi = (d.kmin[a] + d.dims[a] - 1) >> 1;
+    const unsigned long long span = (unsigned long long)(hhi - hlo);
+    long long kmn = 0, kmx = -1;
+    bool any = false;
+    for (int dk = -1; dk <= 1; ++dk) {
+        const long long K = (long long)((unsigned long long)k + (unsigned long long)(long long)dk);
+        if ((unsigned long long)K - (unsigned long long)hlo <= span) {
+            kmn = any ? (K < kmn ? K : kmn) : K;
+            kmx = any ? (K > kmx ? K : kmx) : K;
+            any = true;
+        }
+    }
+    if (!any) {
+        lo = 0;
+        hi = -1;
+        return;
+    }
+    const long long c0 = 2 * kmn, c1 = 2 * kmx + 1;
+    lo = (c0 > d.kmin[a] ? c0 : d.kmin[a]) - d.kmin[a];
+    hi = (c1 < d.kmin[a] + d.dims[a] - 1 ? c1 : d.kmin[a] + d.dims[a] - 1) - d.kmin[a];
+}
+
 // ---------------------------------------------------------------------------
 // neighbour visiting
 //
-// Rows (dx, dy) in the reference order with the three z-adjacent cells of each
-// row as one contiguous run [start[c0], start[c2 + 1]).
+// Rows (dx, dy) of the half-cell window with the z-adjacent cells of each
+// row as one contiguous run [start[c0], start[c4 + 1]).
 
 struct CellCoord {
     long long cx, cy, cz;
@@ -142,22 +194,38 @@ __device__ __forceinline__ CellCoord decode(uint32_t c, const Domain& d) {
     return r;
 }
 
-// Run of row k (0..8) of an in-domain particle; empty when the row leaves the
-// domain (those cells hold no dense particles).
+// Run of row k (0..kRows-1) of the window of an in-domain particle: half
+// cells (cx + dx, cy + dy, cz - kReach .. cz + kReach), contiguous in z; empty
+// when the row leaves the domain (those cells hold no dense particles).
 __device__ __forceinline__ void row_run(int k, const CellCoord& cc, const Domain& d,
                                         const uint32_t* __restrict__ start, uint32_t& a,
                                         uint32_t& b) {
-    long long x = cc.cx + (k / 3) - 1;
-    long long y = cc.cy + (k % 3) - 1;
+    long long x = cc.cx + (k / kSpan) - kReach;
+    long long y = cc.cy + (k % kSpan) - kReach;
     if (x < 0 || x >= d.dims[0] || y < 0 || y >= d.dims[1]) {
         a = b = 0;
         return;
     }
-    long long z0 = cc.cz > 0 ? cc.cz - 1 : 0;
-    long long z1 = cc.cz + 1 < d.dims[2] ? cc.cz + 1 : d.dims[2] - 1;
+    long long z0 = cc.cz > kReach ? cc.cz - kReach : 0;
+    long long z1 = cc.cz + kReach < d.dims[2] ? cc.cz + kReach : d.dims[2] - 1;
     long long base = (x * d.dims[1] + y) * d.dims[2];
     a = start[base + z0];
     b = start[base + z1 + 1];
+}
+
+// Run of row k (0..35) of the reference block of an out-of-domain particle
+// (intervals from ref_block); empty rows have a == b.
+__device__ __forceinline__ void block_run(int k, const long long* lo, const long long* hi,
+                                          const Domain& d, const uint32_t* __restrict__ start,
+                                          uint32_t& a, uint32_t& b) {
+    const long long x = lo[0] + k / 6, y = lo[1] + k % 6;
+    if (x > hi[0] || y > hi[1] || lo[2] > hi[2]) {
+        a = b = 0;
+        return;
+    }
+    const long long base = (x * d.dims[1] + y) * d.dims[2];
+    a = start[base + lo[2]];
+    b = start[base + hi[2] + 1];
 }
 
 // Reference cell adjacency on full i64 keys with wrapping offsets (grid.rs:44-48).
@@ -179,29 +247,22 @@ __device__ __forceinline__ void for_each_candidate(uint32_t ci, float xi, float 
                                                    const float4* __restrict__ posm, F&& f) {
     if (ci < d.ncells) {
         CellCoord cc = decode(ci, d);
-        for (int k = 0; k < 9; ++k) {
+        for (int k = 0; k < kRows; ++k) {
             uint32_t a, b;
             row_run(k, cc, d, start, a, b);
             for (uint32_t j = a; j < b; ++j) f(j);
         }
     } else {
-        // out-of-domain particle: visit the in-domain cells among its 27 keys
-        long long k0 = key1(xi, inv), k1 = key1(yi, inv), k2 = key1(zi, inv);
-        for (int di = -1; di <= 1; ++di)
-            for (int dj = -1; dj <= 1; ++dj)
-                for (int dk = -1; dk <= 1; ++dk) {
-                    unsigned long long x = (unsigned long long)k0 + (unsigned long long)(long long)di -
-                                           (unsigned long long)d.kmin[0];
-                    unsigned long long y = (unsigned long long)k1 + (unsigned long long)(long long)dj -
-                                           (unsigned long long)d.kmin[1];
-                    unsigned long long z = (unsigned long long)k2 + (unsigned long long)(long long)dk -
-                                           (unsigned long long)d.kmin[2];
-                    if (x >= (unsigned long long)d.dims[0] || y >= (unsigned long long)d.dims[1] ||
-                        z >= (unsigned long long)d.dims[2])
-                        continue;
-                    uint32_t c = (uint32_t)((x * d.dims[1] + y) * d.dims[2] + z);
-                    for (uint32_t j = start[c]; j < start[c + 1]; ++j) f(j);
-                }
+        // out-of-domain particle: the in-domain half cells of its 27 reference cells
+        long long lo[3], hi[3];
+        ref_block(key1(xi, inv), d, 0, lo[0], hi[0]);
+        ref_block(key1(yi, inv), d, 1, lo[1], hi[1]);
+        ref_block(key1(zi, inv), d, 2, lo[2], hi[2]);
+        for (int k = 0; k < 36; ++k) {
+            uint32_t a, b;
+            block_run(k, lo, hi, d, start, a, b);
+            for (uint32_t j = a; j < b; ++j) f(j);
+        }
     }
     uint32_t o0 = start[d.ncells], o1 = start[d.ncells + 1];
     for (uint32_t j = o0; j < o1; ++j) {
@@ -216,12 +277,13 @@ __device__ __forceinline__ void for_each_candidate(uint32_t ci, float xi, float 
 // Candidates are tested with the cheap FP32 filter (exact f64 fallback in the
 // band) and accepted ones are appended to a per-lane list in LDS; the wave
 // drains the lists whenever one is full, so the heavy per-neighbour work runs
-// on dense lists instead of on every candidate under divergence (about 15 % of
-// the 27-cell candidates are neighbours at rest, 35 % in a collapsed sheet).
+// on dense lists instead of on every candidate under divergence (about 25 % of
+// the half-cell window's candidates are neighbours at rest, ~50 % in a
+// collapsed sheet).
 //
 // Two scans produce the same lists in the same order:
 //  * uniform: every active lane of the wave sits in the same dense cell (the
-//    common case once the column has collapsed, ~400 particles per cell). The
+//    common case once the column has collapsed, ~100 particles per half cell). The
 //    candidate sequence is then wave-uniform: loop control runs on the scalar
 //    unit and candidate loads are uniform.
 //  * per-lane: each lane walks its own rows with a cursor, one candidate per round.
@@ -231,32 +293,15 @@ struct Cursor {
     uint32_t j, jend, o0, o1;
     bool dense;
     CellCoord cc;
-    long long k0, k1, k2;
+    long long lo[3], hi[3];  // out-of-domain particle: ref_block intervals
 
     __device__ __forceinline__ bool in_overflow() const { return row > nrows; }
 
     __device__ __forceinline__ void next_run(const Domain& d, const uint32_t* __restrict__ start) {
         while (j >= jend && row <= nrows) {
             if (row < nrows) {
-                if (dense) {
-                    row_run(row, cc, d, start, j, jend);
-                } else {
-                    int di = row / 9 - 1, dj = (row / 3) % 3 - 1, dk = row % 3 - 1;
-                    unsigned long long x = (unsigned long long)k0 + (unsigned long long)(long long)di -
-                                           (unsigned long long)d.kmin[0];
-                    unsigned long long y = (unsigned long long)k1 + (unsigned long long)(long long)dj -
-                                           (unsigned long long)d.kmin[1];
-                    unsigned long long z = (unsigned long long)k2 + (unsigned long long)(long long)dk -
-                                           (unsigned long long)d.kmin[2];
-                    if (x >= (unsigned long long)d.dims[0] || y >= (unsigned long long)d.dims[1] ||
-                        z >= (unsigned long long)d.dims[2]) {
-                        j = jend = 0;
-                    } else {
-                        uint32_t c = (uint32_t)((x * d.dims[1] + y) * d.dims[2] + z);
-                        j = start[c];
-                        jend = start[c + 1];
-                    }
-                }
+                if (dense) row_run(row, cc, d, start, j, jend);
+                else block_run(row, lo, hi, d, start, j, jend);
             } else {
                 j = o0;
                 jend = o1;
@@ -273,16 +318,19 @@ struct Cursor {
         o0 = start[d.ncells];
         o1 = start[d.ncells + 1];
         dense = ci < d.ncells;
-        nrows = dense ? 9 : 27;
-        k0 = k1 = k2 = 0;
+        nrows = dense ? kRows : 36;
         cc = CellCoord{0, 0, 0};
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = 0;
+            hi[a] = -1;
+        }
         if (!act) return;
         if (dense) {
             cc = decode(ci, d);
         } else {
-            k0 = key1(xi, inv);
-            k1 = key1(yi, inv);
-            k2 = key1(zi, inv);
+            ref_block(key1(xi, inv), d, 0, lo[0], hi[0]);
+            ref_block(key1(yi, inv), d, 1, lo[1], hi[1]);
+            ref_block(key1(zi, inv), d, 2, lo[2], hi[2]);
         }
         next_run(d, start);
     }
@@ -375,7 +423,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                                                : wave_uniform(act, ci, d);
     if (MODE != kOtherWaves && uniform) {
         const CellCoord cc = decode(ci0, d);
-        for (int row = 0; row < 9; ++row) {
+        for (int row = 0; row < kRows; ++row) {
             uint32_t a, b;
             row_run(row, cc, d, start, a, b);
             a = uni(a);

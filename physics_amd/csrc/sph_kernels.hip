@@ -103,8 +103,10 @@ __global__ __launch_bounds__(256) void k_aabb(SceneTable T, const float4* __rest
         const uint32_t local = (uint32_t)(t - (int)si->tile0) * (uint32_t)kTile + threadIdx.x;
         if (local < si->n) {
             const float4 p = posm[si->off + local];
-            if (finite3(p.x, p.y, p.z)) {
-                const long long k[3] = {key1(p.x, inv), key1(p.y, inv), key1(p.z, inv)};
+            long long k[3];
+            // half keys; particles whose reference key fails go to the overflow segment
+            if (hkey((double)p.x * inv, k[0]) & hkey((double)p.y * inv, k[1]) &
+                hkey((double)p.z * inv, k[2])) {
                 for (int a = 0; a < 3; ++a) {
                     mn[a] = k[a] < mn[a] ? k[a] : mn[a];
                     mx[a] = k[a] > mx[a] ? k[a] : mx[a];

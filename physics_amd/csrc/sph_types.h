@@ -54,12 +54,14 @@ struct SceneInfo {
     long long bkey[6];       // cell keys of the scene's bounds box: lo xyz, hi xyz
 };
 
-// Dense grid domain of one scene for one step. Keys are the reference's i64
-// cell keys (grid.rs:26-33); the table covers kmin .. kmin + dims - 1 per axis
-// with z the fastest-varying axis, so the three z-adjacent cells of a neighbour
-// row are one contiguous run of sorted particles. Local cell index ncells is
-// the scene's overflow segment. The scene's cells 0 .. ncells sit at toff in
-// the concatenated cell table of the step.
+// Dense grid domain of one scene for one step. Keys are half-cell keys
+// floor(2 fl(x (1/h))), an exact refinement of the reference's i64 cell keys
+// (grid.rs:26-33: reference key = half key >> 1); the table covers kmin ..
+// kmin + dims - 1 per axis with z the fastest-varying axis, so the
+// z-adjacent half cells of a neighbour row are one contiguous run of sorted
+// particles. Local cell index ncells is the scene's overflow segment. The
+// scene's cells 0 .. ncells sit at toff in the concatenated cell table of the
+// step.
 struct Domain {
     long long kmin[3];
     long long dims[3];
