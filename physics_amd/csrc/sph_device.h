@@ -365,9 +365,56 @@ __device__ __forceinline__ T ldg(const T* __restrict__ base, uint32_t idx) {
 // kernel does not carry the per-lane cursor's registers.
 enum GatherMode { kAllWaves = 0, kUniformWaves = 1, kOtherWaves = 2 };
 
+// The wave's active lanes all sit in dense half cells of one (x, y) column,
+// within kColSpan consecutive z cells [c0, c1]: their candidates are then
+// covered by one shared window (rows of z-runs cz(c0) - kReach .. cz(c1) +
+// kReach), scanned with wave-uniform loads. The window is larger than a
+// lane's own by at most kColSpan - 1 cells per row; every extra candidate is
+// still tested with the exact predicate.
+constexpr uint32_t kColSpan = 3;
+
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+struct WaveCol {
+    bool ok;
+    uint32_t c0, c1;  // lowest and highest cell of the wave's lanes
+};
+
+__device__ __forceinline__ WaveCol wave_column(bool act, uint32_t ci, const Domain& d) {
+    WaveCol w;
+    w.c0 = uni(wave_min_u32(act ? ci : 0xffffffffu));
+    w.c1 = uni(wave_max_u32(act ? ci : 0u));
+    const uint32_t dz = (uint32_t)d.dims[2];
+    w.ok = w.c0 <= w.c1 && w.c1 < d.ncells && w.c1 - w.c0 < kColSpan && w.c0 / dz == w.c1 / dz;
+    return w;
+}
+
 __device__ __forceinline__ bool wave_uniform(bool act, uint32_t ci, const Domain& d) {
-    const uint32_t ci0 = uni(ci);
-    return ci0 < d.ncells && !wave_any(act && ci != ci0);
+    return wave_column(act, ci, d).ok;
+}
+
+// Run of row k of a column wave's shared window: z cells cz0 - kReach ..
+// cz1 + kReach of column (cx + dx, cy + dy).
+__device__ __forceinline__ void col_run(int k, const CellCoord& cc, long long cz1, const Domain& d,
+                                        const uint32_t* __restrict__ start, uint32_t& a,
+                                        uint32_t& b) {
+    long long x = cc.cx + (k / kSpan) - kReach;
+    long long y = cc.cy + (k % kSpan) - kReach;
+    if (x < 0 || x >= d.dims[0] || y < 0 || y >= d.dims[1]) {
+        a = b = 0;
+        return;
+    }
+    long long z0 = cc.cz > kReach ? cc.cz - kReach : 0;
+    long long z1 = cz1 + kReach < d.dims[2] ? cz1 + kReach : d.dims[2] - 1;
+    long long base = (x * d.dims[1] + y) * d.dims[2];
+    a = start[base + z0];
+    b = start[base + z1 + 1];
 }
 
 // S: list slots drained per round (1 or 2; 2 keeps two neighbour loads in
@@ -417,15 +464,14 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
     auto ov_ok = [&](const float4& pj) {
         return keys_adjacent(xi, yi, zi, pj.x, pj.y, pj.z, inv);
     };
-    const uint32_t ci0 = uni(ci);
-    const bool uniform = MODE == kUniformWaves ? true
-                         : MODE == kOtherWaves ? false
-                                               : wave_uniform(act, ci, d);
+    const WaveCol wc = MODE == kOtherWaves ? WaveCol{false, 0u, 0u} : wave_column(act, ci, d);
+    const bool uniform = MODE == kUniformWaves ? true : MODE == kOtherWaves ? false : wc.ok;
     if (MODE != kOtherWaves && uniform) {
-        const CellCoord cc = decode(ci0, d);
+        const CellCoord cc = decode(wc.c0, d);
+        const long long cz1 = cc.cz + (long long)(wc.c1 - wc.c0);
         for (int row = 0; row < kRows; ++row) {
             uint32_t a, b;
-            row_run(row, cc, d, start, a, b);
+            col_run(row, cc, cz1, d, start, a, b);
             a = uni(a);
             b = uni(b);
             for (uint32_t j = a; j < b; j += 4) {
