@@ -265,7 +265,9 @@ void DeviceFluid::ensure_capacity(size_t n) {
     if (n > cap_n_) {
         size_t c = std::max<size_t>(n, 1);
         posm_.ensure(c); vel_.ensure(c); id_.ensure(c); mat_.ensure(c); body_.ensure(c);
-        posm_s_.ensure(c); velm_s_.ensure(c); body_s_.ensure(c);
+        // the sorted positions keep kScanPad spare entries: the uniform scan
+        // loads candidates four at a time
+        posm_s_.ensure(c + kScanPad); velm_s_.ensure(c); body_s_.ensure(c);
         id_s_.ensure(c); mat_s_.ensure(c); cell_s_.ensure(c);
         rho_.ensure(c); prs_.ensure(c); mp_.ensure(c);
         cell_.ensure(c); rank_.ensure(c); tmp_.ensure(c); idtmp_.ensure(c);
@@ -705,7 +707,7 @@ void DeviceFluid::tape_end() {
 
 DeviceFluid::StepBufs DeviceFluid::slot_bufs(TapeSlot& t) {
     const size_t n = n_ ? n_ : 1;
-    t.posm_s.ensure(n); t.velm_s.ensure(n); t.body_s.ensure(1);
+    t.posm_s.ensure(n + kScanPad); t.velm_s.ensure(n); t.body_s.ensure(1);
     t.id_s.ensure(n); t.mat_s.ensure(n); t.cell_s.ensure(n);
     t.rho.ensure(n); t.prs.ensure(n); t.mp.ensure(n);
     t.acc.ensure(n); t.xsph.ensure(n); t.fvisc.ensure(n); t.mu.ensure(n);

@@ -184,6 +184,10 @@ struct CellCoord {
     long long cx, cy, cz;
 };
 
+struct Quad4 {
+    float4 p[4];
+};
+
 __device__ __forceinline__ CellCoord decode(uint32_t c, const Domain& d) {
     CellCoord r;
     long long dz = d.dims[2], dy = d.dims[1];
@@ -476,9 +480,13 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
             b = uni(b);
             for (uint32_t j = a; j < b; j += 4) {
                 const uint32_t last = b - 1;
-                const uint32_t j1 = min(j + 1, last), j2 = min(j + 2, last), j3 = min(j + 3, last);
-                const float4 p0 = posm[j], p1 = posm[j1], p2 = posm[j2], p3 = posm[j3];
-                // non-short-circuit: the four loads issue together, no branches
+                const uint32_t j1 = j + 1, j2 = j + 2, j3 = j + 3;
+                // one 64-byte wave-uniform load for four candidates (the array
+                // has kScanPad spare entries; candidates past the run are
+                // masked below)
+                const Quad4 q = *reinterpret_cast<const Quad4*>(posm + j);
+                const float4 p0 = q.p[0], p1 = q.p[1], p2 = q.p[2], p3 = q.p[3];
+                // non-short-circuit: no branches
                 const bool t0 = test(j, p0) & act;
                 const bool t1 = test(j1, p1) & (act & (j + 1 <= last));
                 const bool t2 = test(j2, p2) & (act & (j + 2 <= last));

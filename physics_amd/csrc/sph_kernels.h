@@ -7,6 +7,9 @@
 
 namespace phy {
 
+// spare entries after the sorted positions (read, never used, by the
+// four-candidate loads of the uniform scan)
+constexpr int kScanPad = 4;
 constexpr int kScanBlocks = 1024;  // fixed grid of the three-phase cell scan
 constexpr int kForceList = 16;     // default per-lane LDS neighbour list length of k_force
 constexpr int kDensityList = 16;   // default per-lane LDS neighbour list length of k_density
