@@ -377,14 +377,14 @@ void launch_adjoint_step(hipStream_t st, const SceneTable& T, const AdjSlot& S, 
         hipLaunchKernelGGL(k_adj_local, dim3(nb), dim3(256), 0, st, T, S, B.xbar_in, B.vbar_in, B.Fb,
                            B.Sb, B.xb_loc, B.vb_loc, B.rhob_loc, B.mush, B.part);
         if (shear)
-            hipLaunchKernelGGL((k_adj_force<kForceList, true>), dim3(nb), dim3(256), 0, st, T, S, B.Fb,
+            hipLaunchKernelGGL((k_adj_force<kAdjList, true>), dim3(nb), dim3(256), 0, st, T, S, B.Fb,
                                B.Sb, B.xb_loc, B.vb_loc, B.rhob_loc, B.mush, rhob_seed, B.rhob, B.xb_acc,
                                B.vbar_out, B.part);
         else
-            hipLaunchKernelGGL((k_adj_force<kForceList, false>), dim3(nb), dim3(256), 0, st, T, S, B.Fb,
+            hipLaunchKernelGGL((k_adj_force<kAdjList, false>), dim3(nb), dim3(256), 0, st, T, S, B.Fb,
                                B.Sb, B.xb_loc, B.vb_loc, B.rhob_loc, B.mush, rhob_seed, B.rhob, B.xb_acc,
                                B.vbar_out, B.part);
-        hipLaunchKernelGGL(k_adj_density<kDensityList>, dim3(nb), dim3(256), 0, st, T, S, B.rhob,
+        hipLaunchKernelGGL(k_adj_density<kAdjList>, dim3(nb), dim3(256), 0, st, T, S, B.rhob,
                            B.xb_acc, B.xbar_out);
     }
     hipLaunchKernelGGL(k_adj_reduce, dim3(T.nscenes), dim3(256), 0, st, T, B.part, B.totals);

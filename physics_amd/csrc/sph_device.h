@@ -478,25 +478,22 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
             col_run(row, cc, cz1, d, start, a, b);
             a = uni(a);
             b = uni(b);
-            for (uint32_t j = a; j < b; j += 4) {
+            for (uint32_t j = a; j < b; j += 8) {
                 const uint32_t last = b - 1;
-                const uint32_t j1 = j + 1, j2 = j + 2, j3 = j + 3;
-                // one 64-byte wave-uniform load for four candidates (the array
-                // has kScanPad spare entries; candidates past the run are
-                // masked below)
-                const Quad4 q = *reinterpret_cast<const Quad4*>(posm + j);
-                const float4 p0 = q.p[0], p1 = q.p[1], p2 = q.p[2], p3 = q.p[3];
-                // non-short-circuit: no branches
-                const bool t0 = test(j, p0) & act;
-                const bool t1 = test(j1, p1) & (act & (j + 1 <= last));
-                const bool t2 = test(j2, p2) & (act & (j + 2 <= last));
-                const bool t3 = test(j3, p3) & (act & (j + 3 <= last));
-                // cnt <= L - 4 here, so the four writes stay inside the list
-                push(j, t0);
-                push(j1, t1);
-                push(j2, t2);
-                push(j3, t3);
-                if (wave_any(cnt > L - 4)) drain();
+                // two 64-byte wave-uniform loads for eight candidates (the
+                // array has kScanPad spare entries; candidates past the run
+                // are masked below)
+                const Quad4 qa = *reinterpret_cast<const Quad4*>(posm + j);
+                const Quad4 qb = *reinterpret_cast<const Quad4*>(posm + j + 4);
+                // non-short-circuit: no branches; cnt <= L - 8 here, so the
+                // eight writes stay inside the list
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u)
+                    push(j + u, test(j + u, qa.p[u]) & (act & (j + u <= last)));
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u)
+                    push(j + 4 + u, test(j + 4 + u, qb.p[u]) & (act & (j + 4 + u <= last)));
+                if (wave_any(cnt > L - 8)) drain();
             }
         }
         const uint32_t o0 = uni(start[d.ncells]), o1 = uni(start[d.ncells + 1]);

@@ -8,11 +8,12 @@
 namespace phy {
 
 // spare entries after the sorted positions (read, never used, by the
-// four-candidate loads of the uniform scan)
-constexpr int kScanPad = 4;
+// eight-candidate loads of the uniform scan)
+constexpr int kScanPad = 8;
 constexpr int kScanBlocks = 1024;  // fixed grid of the three-phase cell scan
-constexpr int kForceList = 16;     // default per-lane LDS neighbour list length of k_force
+constexpr int kForceList = 32;     // default per-lane LDS neighbour list length of k_force
 constexpr int kDensityList = 16;   // default per-lane LDS neighbour list length of k_density
+constexpr int kAdjList = 16;       // per-lane list length of the adjoint gathers
 
 // Launch choices of the gather kernels, picked by the autotuner. The list
 // length sets the LDS per workgroup (L x 1 KiB, hence occupancy) against the
