@@ -34,15 +34,21 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_force_1M_dambreak_hbm_traffic.md")
 
 
-def measured_traffic(kernel: str):
-    """Bytes per launch of `kernel` from the committed counter summary, or None."""
+def measured_counter(key: str, kernel: str):
+    """A per-launch figure of `kernel` from the committed counter summary, or None."""
     try:
         for line in open(TRAFFIC_PROFILE):
-            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
-                return int(line.split(":")[1])
+            if line.startswith(f"{key} {kernel}:"):
+                return float(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def measured_traffic(kernel: str):
+    """Bytes per launch of `kernel` from the committed counter summary, or None."""
+    v = measured_counter("traffic_bytes_per_launch", kernel)
+    return None if v is None else int(v)
 
 
 def parse():
@@ -167,6 +173,9 @@ def main():
             "traffic": traffic,
             "traffic_source": traffic_src,
             "alg_bytes_per_particle": ALG_BYTES[dom],
+            # the kernel is bound by vector-ALU issue, not by HBM: its VALUBusy
+            # from the same counter summary
+            "valu_busy_pct": measured_counter("valu_busy_pct", f"k_{dom}") if traffic else None,
         },
     }
 
