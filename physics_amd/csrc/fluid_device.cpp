@@ -53,6 +53,20 @@ DevParams make_dev_params(const HostParams& hp) {
         P.s_lo = -INFINITY;
         P.s_hi = INFINITY;
     }
+    // z half-widths of the trimmed scan rows. A neighbour has real
+    // dx^2 + dy^2 + dz^2 < h^2 (1 + 2^-50) (s64 < s_thresh plus the rounding of
+    // s64). Two half-cell columns 2 apart hold points at least h/2 apart, less
+    // the key rounding (|x| 2^-52 per point; the scan trims only domains with
+    // keys below 2^40 in magnitude, so less than h 2^-11): (h/2)(1 - 2^-8) is a
+    // safe gap. Degenerate h: no trimming (infinite widths).
+    {
+        const double gap2 = 0.25 * (1.0 - std::ldexp(1.0, -8)) * (1.0 - std::ldexp(1.0, -8));
+        const double top = 1.0 + std::ldexp(1.0, -40), up = 1.0 + std::ldexp(1.0, -30);
+        const bool ok = std::isnormal(h) && h > 0.0 && std::isfinite(h * h * 4.0);
+        P.zw[0] = ok ? h * std::sqrt(top) * up : INFINITY;
+        P.zw[1] = ok ? h * std::sqrt(top - gap2) * up : INFINITY;
+        P.zw[2] = ok ? h * std::sqrt(top - 2.0 * gap2) * up : INFINITY;
+    }
     // kernels.rs:22-35 in f64
     const double pi = 3.141592653589793;
     double h2 = h * h, h3 = h2 * h, h6 = h3 * h3, h9 = h6 * h3;
@@ -270,7 +284,7 @@ void DeviceFluid::ensure_capacity(size_t n) {
         posm_s_.ensure(c + kScanPad); velm_s_.ensure(c); body_s_.ensure(c);
         id_s_.ensure(c); mat_s_.ensure(c); cell_s_.ensure(c);
         rho_.ensure(c); prs_.ensure(c); mp_.ensure(c);
-        cell_.ensure(c); rank_.ensure(c); tmp_.ensure(c); idtmp_.ensure(c);
+        cell_.ensure(c); rank_.ensure(c); tmp_.ensure(c); idtmp_.ensure(c); ztmp_.ensure(c);
         stage4_.ensure(c); stage1_.ensure(c);
         if (keep_aux_) { acc_.ensure(c); xsph_.ensure(c); mu_.ensure(c); }
         cap_n_ = c;
@@ -395,6 +409,7 @@ void DeviceFluid::run_domain_and_sort(const StepBufs& s) {
     b.start = s.start;
     b.tmp = tmp_.get();
     b.idtmp = idtmp_.get();
+    b.ztmp = ztmp_.get();
     b.posm_out = s.posm_s;
     b.velm_out = s.velm_s;
     b.id_out = s.id_s;

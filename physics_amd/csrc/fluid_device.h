@@ -293,6 +293,7 @@ class DeviceFluid {
     DevBuf<float> rho_, prs_, mp_;
     // sort scratch
     DevBuf<uint32_t> cell_, rank_, counts_, partial_, start_, tmp_, idtmp_;
+    DevBuf<float> ztmp_;
     // aux outputs
     DevBuf<float4> acc_, xsph_;
     DevBuf<float> mu_;

@@ -176,7 +176,7 @@ __global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
     const float h = P.h, h2 = P.h2;
     ForceAdjAcc<SHEAR> A = {};
     gather<L, kAllWaves, 2>(
-        act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, start, S.posm, lst, tid,
+        act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, zwin(P, d), start, S.posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_maybe(P, pa.x, pa.y, pa.z, pj); },
         [&](uint32_t j) {
             return AdjNbr{ldg(S.posm, j), ldg(S.velm, j), ldg(Fb, j), ldg(Sb, j), ldg(mush, j)};
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(256) void k_adj_density(SceneTable T, AdjSlot S,
         float rb;
     };
     gather<L, kAllWaves, 2>(
-        act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, start, S.posm, lst, tid,
+        act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, zwin(P, d), start, S.posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return density_maybe(P, pa.x, pa.y, pa.z, pj); },
         [&](uint32_t j) { return DN{S.posm[j], rhob[j]}; },
         [&](uint32_t j, const DN& nb, bool listed) {

@@ -403,6 +403,72 @@ __device__ __forceinline__ bool wave_uniform(bool act, uint32_t ci, const Domain
     return wave_column(act, ci, d).ok;
 }
 
+// z half-widths of the trimmed rows of a uniform scan (DevParams::zw). Off
+// for domains with keys of magnitude 2^40 and more: the column gap that zw[1]
+// and zw[2] assume holds for smaller coordinates.
+struct ZWin {
+    double w0, w1, w2;
+    bool on;
+};
+
+__device__ __forceinline__ ZWin zwin(const DevParams& P, const Domain& d) {
+    ZWin z{P.zw[0], P.zw[1], P.zw[2], true};
+    for (int a = 0; a < 3; ++a)
+        z.on = z.on && d.kmin[a] > -(1ll << 40) && d.kmin[a] < (1ll << 40) &&
+               d.dims[a] < (1ll << 40) && d.kmin[a] + d.dims[a] < (1ll << 40);
+    return z;
+}
+
+__device__ __forceinline__ float wave_min_f(float v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const float w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ float wave_max_f(float v) {
+    for (int o = 32; o > 0; o >>= 1) {
+        const float w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// FP32 bounds that contain [lo64, hi64] (rounded outward)
+__device__ __forceinline__ float f32_down(double v) {
+    float f = (float)v;
+    return (double)f > v ? nextafterf(f, -INFINITY) : f;
+}
+__device__ __forceinline__ float f32_up(double v) {
+    float f = (float)v;
+    return (double)f < v ? nextafterf(f, INFINITY) : f;
+}
+
+// Trims the z-run [a, b) of a uniform scan to the candidates with z in
+// [zlo, zhi]. Inside a dense cell the members are ordered by z, and the cells
+// of a run follow in z order (keys are monotone in z), so the run is sorted by
+// z: the candidates below zlo form a prefix and those above zhi a suffix,
+// found 64 at a time with one coalesced load and a ballot.
+__device__ __forceinline__ void trim_run(const float4* __restrict__ posm, float zlo, float zhi,
+                                         uint32_t& a, uint32_t& b) {
+    const uint32_t lane = threadIdx.x & 63u;
+    for (;;) {
+        const uint32_t j = a + lane;
+        const bool below = j < b && posm[j].z < zlo;
+        const uint32_t nb = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(below));
+        a += nb;
+        if (nb < 64u) break;
+    }
+    for (;;) {
+        const uint32_t j = b - 1u - lane;
+        const bool above = lane < b - a && posm[j].z > zhi;
+        const uint32_t na = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(above));
+        b -= na;
+        if (na < 64u) break;
+    }
+}
+
 // Run of row k of a column wave's shared window: z cells cz0 - kReach ..
 // cz1 + kReach of column (cx + dx, cy + dy).
 __device__ __forceinline__ void col_run(int k, const CellCoord& cc, long long cz1, const Domain& d,
@@ -423,10 +489,11 @@ __device__ __forceinline__ void col_run(int k, const CellCoord& cc, long long cz
 
 // S: list slots drained per round (1 or 2; 2 keeps two neighbour loads in
 // flight, 1 needs fewer registers).
+// zt: the z half-widths of the uniform scan's trimmed rows.
 template <int L, int MODE, int S, class Test, class Load, class Proc, class Flush>
 __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi, float zi,
                                        uint32_t self_j, const Domain& d, double inv,
-                                       const uint32_t* __restrict__ start,
+                                       const ZWin& zt, const uint32_t* __restrict__ start,
                                        const float4* __restrict__ posm, uint32_t (*lst)[256],
                                        int tid, Test&& test, Load&& load, Proc&& proc,
                                        Flush&& flush) {
@@ -473,11 +540,23 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
     if (MODE != kOtherWaves && uniform) {
         const CellCoord cc = decode(wc.c0, d);
         const long long cz1 = cc.cz + (long long)(wc.c1 - wc.c0);
+        // the lanes' z-range: a candidate farther than the row's half-width
+        // below or above it is no lane's neighbour
+        const double zmn = (double)wave_min_f(act ? zi : INFINITY);
+        const double zmx = (double)wave_max_f(act ? zi : -INFINITY);
         for (int row = 0; row < kRows; ++row) {
             uint32_t a, b;
             col_run(row, cc, cz1, d, start, a, b);
             a = uni(a);
             b = uni(b);
+            if (zt.on && a < b) {
+                const int ex = row / kSpan - kReach, ey = row % kSpan - kReach;
+                const int cls = (ex == kReach || ex == -kReach) + (ey == kReach || ey == -kReach);
+                const double w = cls == 0 ? zt.w0 : (cls == 1 ? zt.w1 : zt.w2);
+                trim_run(posm, f32_down(zmn - w), f32_up(zmx + w), a, b);
+                a = uni(a);
+                b = uni(b);
+            }
             for (uint32_t j = a; j < b; j += 8) {
                 const uint32_t last = b - 1;
                 // two 64-byte wave-uniform loads for eight candidates (the

@@ -45,6 +45,7 @@ struct SortBuffers {
     uint32_t* start;
     uint32_t* tmp;
     uint32_t* idtmp;
+    float* ztmp;              // z of each scatter slot (within-cell order)
     // outputs (cell order)
     float4* posm_out;
     float4* velm_out;

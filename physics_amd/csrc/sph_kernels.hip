@@ -320,36 +320,52 @@ __global__ __launch_bounds__(256) void k_scan_final(uint32_t* counts,
 __global__ __launch_bounds__(256) void k_scatter(int n, const uint32_t* __restrict__ cell,
                                                  const uint32_t* __restrict__ rank,
                                                  const uint32_t* __restrict__ start,
-                                                 const uint32_t* __restrict__ id, uint32_t* tmp,
-                                                 uint32_t* idtmp) {
+                                                 const uint32_t* __restrict__ id,
+                                                 const float4* __restrict__ posm, uint32_t* tmp,
+                                                 uint32_t* idtmp, float* ztmp) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     uint32_t q = start[cell[s]] + rank[s];
     tmp[q] = (uint32_t)s;
     idtmp[q] = id[s];
+    ztmp[q] = posm[s].z;
 }
 
 // Slot q of the scatter holds some member of its cell in arrival order; its
-// final slot is the cell start plus the number of members with a smaller
-// original index (ascending index inside a cell, grid.rs:37-38). Slots of a
-// scene stay inside the scene's particle range, so the tile names the scene;
-// the stored cell index is the scene-local one.
+// final slot is the cell start plus the number of members ordered before it.
+// Dense cells order their members by z, ties by original index, so each z-run
+// of cells is sorted by z (the uniform scan trims runs by z); the overflow
+// segment keeps ascending original index (grid.rs:37-38). Either order is a
+// function of the state alone. Slots of a scene stay inside the scene's
+// particle range, so the tile names the scene; the stored cell index is the
+// scene-local one.
 __global__ __launch_bounds__(256) void k_reorder(
     SceneTable T, const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ idtmp,
-    const uint32_t* __restrict__ cell, const uint32_t* __restrict__ start,
-    const float4* __restrict__ posm_in, const float4* __restrict__ vel_in,
-    const uint32_t* __restrict__ mat_in, const float4* __restrict__ body_in, float4* posm_out,
-    float4* velm_out, uint32_t* id_out, uint32_t* mat_out, float4* body_out, uint32_t* cell_out) {
+    const float* __restrict__ ztmp, const uint32_t* __restrict__ cell,
+    const uint32_t* __restrict__ start, const float4* __restrict__ posm_in,
+    const float4* __restrict__ vel_in, const uint32_t* __restrict__ mat_in,
+    const float4* __restrict__ body_in, float4* posm_out, float4* velm_out, uint32_t* id_out,
+    uint32_t* mat_out, float4* body_out, uint32_t* cell_out) {
     const TileLoc L = tile_loc(T);
     if (!L.act) return;
     const int q = L.i;
     const uint32_t toff = T.dom[L.s].toff;
+    const uint32_t ovf = toff + T.dom[L.s].ncells;
     uint32_t s = tmp[q];
     uint32_t my = idtmp[q];
     uint32_t c = cell[s];
     uint32_t lo = start[c], hi = start[c + 1];
     uint32_t r = 0;
-    for (uint32_t t = lo; t < hi; ++t) r += idtmp[t] < my ? 1u : 0u;
+    if (c == ovf) {
+        for (uint32_t t = lo; t < hi; ++t) r += idtmp[t] < my ? 1u : 0u;
+    } else {
+        // dense cells hold finite positions only
+        const float mz = ztmp[q];
+        for (uint32_t t = lo; t < hi; ++t) {
+            const float zt = ztmp[t];
+            r += (zt < mz || (zt == mz && idtmp[t] < my)) ? 1u : 0u;
+        }
+    }
     uint32_t dst = lo + r;
     posm_out[dst] = posm_in[s];
     float4 v = vel_in[s];
@@ -390,7 +406,7 @@ __global__ __launch_bounds__(256) void k_density(SceneTable T, const float4* __r
     const double xi = pi.x, yi = pi.y, zi = pi.z;
     double sum = 0.0;
     gather<L, MODE, (MODE == kUniformWaves ? 1 : 2)>(
-        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, start, posm, lst, tid,
+        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), start, posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return density_maybe(P, pi.x, pi.y, pi.z, pj); },
         [&](uint32_t j) { return ldg(posm, j); },
         [&](uint32_t j, const float4& pj, bool listed) {
@@ -515,7 +531,7 @@ __global__ __launch_bounds__(256) void k_force(
     ForceAcc<SHEAR> A;
     A.zero();
     gather<L, MODE, (MODE == kUniformWaves ? 1 : 2)>(
-        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, start, posm, lst, tid,
+        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), start, posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_maybe(P, pi.x, pi.y, pi.z, pj); },
         [&](uint32_t j) { return NbrData{ldg(posm, j), ldg(velm, j), ldg(mp, j)}; },
         [&](uint32_t j, const NbrData& nb, bool listed) {
@@ -735,10 +751,10 @@ void launch_sort(hipStream_t st, const SceneTable& T, const SortBuffers& b) {
                        b.partial, b.start);
     if (T.n > 0)
         hipLaunchKernelGGL(k_scatter, dim3(nblk(T.n, 256)), dim3(256), 0, st, T.n, b.cell, b.rank,
-                           b.start, b.id_in, b.tmp, b.idtmp);
+                           b.start, b.id_in, b.posm_in, b.tmp, b.idtmp, b.ztmp);
     if (T.ntiles > 0)
-        hipLaunchKernelGGL(k_reorder, dim3(T.ntiles), dim3(256), 0, st, T, b.tmp, b.idtmp, b.cell,
-                           b.start, b.posm_in, b.vel_in, b.mat_in, b.body_in, b.posm_out, b.velm_out,
+        hipLaunchKernelGGL(k_reorder, dim3(T.ntiles), dim3(256), 0, st, T, b.tmp, b.idtmp, b.ztmp,
+                           b.cell, b.start, b.posm_in, b.vel_in, b.mat_in, b.body_in, b.posm_out, b.velm_out,
                            b.id_out, b.mat_out, b.body_out, b.cell_out);
 }
 

@@ -32,6 +32,11 @@ struct DevParams {
     double s_thresh;     // smallest f64 s with sqrt(s) >= h: r < h  <=>  s < s_thresh
     double h2_64, c6_64;  // f64 h*h and poly6 coefficient (density pass)
     double rest64, stiff64;
+    // Largest |dz| of a pair with r < h whose x / y half-cell columns are 0, 1
+    // or 2 apart in both (zw[0]), apart by 2 in one (zw[1]) or in both axes
+    // (zw[2]); rounded up. The uniform scan trims each z-run to the lanes'
+    // z-range widened by these.
+    double zw[3];
     float s_lo, s_hi;    // FP32 filter: s32 < s_lo => inside; s32 > s_hi => outside
     float h, h2;         // FP32 support radius and its square
     float c6, cs, cv;    // poly6, spiky-gradient, viscosity-laplacian coefficients
