@@ -30,6 +30,12 @@ static long long host_key(float x, double inv) {
     return 0;
 }
 
+// The gathers address neighbour records with 32-bit byte offsets (16 bytes
+// per particle) from a wave-uniform base: at most 2^27 particles per world or
+// batch.
+static constexpr uint64_t kMaxParticles = 1ull << 27;
+static const char* const kTooMany = "phy-fluid: at most 2^27 particles per world or batch";
+
 DevParams make_dev_params(const HostParams& hp) {
     DevParams P;
     memset(&P, 0, sizeof(P));
@@ -49,6 +55,12 @@ DevParams make_dev_params(const HostParams& hp) {
     P.s_hi = (float)(s * (1.0 + marg));
     if ((double)P.s_lo >= s * (1.0 - marg)) P.s_lo = std::nextafter(P.s_lo, 0.0f);
     if ((double)P.s_hi <= s * (1.0 + marg)) P.s_hi = std::nextafter(P.s_hi, INFINITY);
+    // Column-local filter of the uniform scan: its FP32 terms stay finite and
+    // its rounding below 1e-5 h^2 for h in [1e-15, 1e15]; elsewhere NaN, which
+    // lets every candidate through.
+    P.s_hq = (float)(s * (1.0 + std::ldexp(1.0, -16)));
+    if ((double)P.s_hq <= s * (1.0 + std::ldexp(1.0, -16))) P.s_hq = std::nextafter(P.s_hq, INFINITY);
+    if (!(h >= 1e-15 && h <= 1e15)) P.s_hq = NAN;
     if (!(h > 0.0)) {  // degenerate support radius: only the exact path decides
         P.s_lo = -INFINITY;
         P.s_hi = INFINITY;
@@ -178,7 +190,7 @@ void DeviceFluid::set_layout(const uint32_t* counts) {
         tile_scene_.insert(tile_scene_.end(), nt, (uint32_t)s);
         off += n;
         t += nt;
-        if (off >= 0x7fffffffull) throw std::invalid_argument("phy-fluid: at most 2^31-1 particles");
+        if (off > kMaxParticles) throw std::invalid_argument(kTooMany);
     }
     n_ = (size_t)off;
     ntiles_ = (int)t;
@@ -279,9 +291,10 @@ void DeviceFluid::ensure_capacity(size_t n) {
     if (n > cap_n_) {
         size_t c = std::max<size_t>(n, 1);
         posm_.ensure(c); vel_.ensure(c); id_.ensure(c); mat_.ensure(c); body_.ensure(c);
-        // the sorted positions keep kScanPad spare entries: the uniform scan
-        // loads candidates four at a time
+        // the sorted and column-local positions keep kScanPad spare entries:
+        // the uniform scan loads candidates eight at a time
         posm_s_.ensure(c + kScanPad); velm_s_.ensure(c); body_s_.ensure(c);
+        posl_.ensure(c + kScanPad);
         id_s_.ensure(c); mat_s_.ensure(c); cell_s_.ensure(c);
         rho_.ensure(c); prs_.ensure(c); mp_.ensure(c);
         cell_.ensure(c); rank_.ensure(c); tmp_.ensure(c); idtmp_.ensure(c); ztmp_.ensure(c);
@@ -296,7 +309,7 @@ void DeviceFluid::ensure_capacity(size_t n) {
 void DeviceFluid::upload(size_t n, const float* pos3, const float* vel3, const float* mass,
                          const uint32_t* material, const float* body3) {
     if (scn_.size() != 1) throw std::logic_error("phy-fluid: upload() needs a one-scene world");
-    if (n >= 0x7fffffffull) throw std::invalid_argument("phy-fluid: at most 2^31-1 particles");
+    if (n > kMaxParticles) throw std::invalid_argument(kTooMany);
     const uint32_t c = (uint32_t)n;
     upload_scenes(&c, pos3, vel3, mass, material);
     has_body_ = false;
@@ -317,7 +330,7 @@ void DeviceFluid::upload_scenes(const uint32_t* counts, const float* pos3, const
                                 const float* mass, const uint32_t* material) {
     uint64_t total = 0;
     for (size_t s = 0; s < scn_.size(); ++s) total += counts[s];
-    if (total >= 0x7fffffffull) throw std::invalid_argument("phy-fluid: at most 2^31-1 particles");
+    if (total > kMaxParticles) throw std::invalid_argument(kTooMany);
     set_layout(counts);
     ensure_capacity(n_);
     size_tables(pos3);
@@ -378,6 +391,7 @@ DeviceFluid::StepBufs DeviceFluid::own_bufs() {
     s.posm_s = posm_s_.get();
     s.velm_s = velm_s_.get();
     s.body_s = body_s_.get();
+    s.posl = posl_.get();
     s.id_s = id_s_.get();
     s.mat_s = mat_s_.get();
     s.cell_s = cell_s_.get();
@@ -416,6 +430,7 @@ void DeviceFluid::run_domain_and_sort(const StepBufs& s) {
     b.mat_out = s.mat_s;
     b.body_out = s.body_s;
     b.cell_out = s.cell_s;
+    b.posl_out = s.posl;
     launch_sort(st_, T, b);
 }
 
@@ -462,9 +477,11 @@ void DeviceFluid::step_core(float dt, const StepBufs& s) {
     run_domain_and_sort(s);
     if (prof_) ev();
     const SceneTable T = table(s.dom);
-    launch_density(st_, T, s.posm_s, s.cell_s, s.start, s.rho, s.prs, s.velm_s, s.mp, tuning_);
+    launch_density(st_, T, s.posm_s, s.posl, s.cell_s, s.start, s.rho, s.prs, s.velm_s, s.mp,
+                   tuning_);
     ForceArgs a;
     a.posm = s.posm_s;
+    a.posl = s.posl;
     a.velm = s.velm_s;
     a.rho = s.rho;
     a.prs = s.prs;
@@ -730,6 +747,7 @@ DeviceFluid::StepBufs DeviceFluid::slot_bufs(TapeSlot& t) {
     t.dom.ensure(scn_.size());
     StepBufs s;
     s.posm_s = t.posm_s.get(); s.velm_s = t.velm_s.get(); s.body_s = t.body_s.get();
+    s.posl = posl_.get();
     s.id_s = t.id_s.get(); s.mat_s = t.mat_s.get(); s.cell_s = t.cell_s.get();
     s.rho = t.rho.get(); s.prs = t.prs.get(); s.mp = t.mp.get();
     // with compact slots the sort uses the world's own full-size table and

@@ -223,6 +223,7 @@ class DeviceFluid {
    private:
     struct StepBufs {
         float4 *posm_s, *velm_s, *body_s;
+        float4* posl;  // consumed within the step: one buffer serves every slot
         uint32_t *id_s, *mat_s, *cell_s;
         float *rho, *prs, *mp;
         uint32_t* start;
@@ -289,6 +290,7 @@ class DeviceFluid {
     DevBuf<float4> body_;
     // sorted copies for the step
     DevBuf<float4> posm_s_, velm_s_, body_s_;
+    DevBuf<float4> posl_;
     DevBuf<uint32_t> id_s_, mat_s_, cell_s_;
     DevBuf<float> rho_, prs_, mp_;
     // sort scratch

@@ -175,17 +175,17 @@ __global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
     const float mua = msa.x, shba = msa.y;
     const float h = P.h, h2 = P.h2;
     ForceAdjAcc<SHEAR> A = {};
-    gather<L, kAllWaves, 2>(
-        act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, zwin(P, d), start, S.posm, lst, tid,
+    gather<L, kAllWaves, 2, false, true>(
+        act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, zwin(P, d), ColLocal{}, start, S.posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_maybe(P, pa.x, pa.y, pa.z, pj); },
-        [&](uint32_t j) {
-            return AdjNbr{ldg(S.posm, j), ldg(S.velm, j), ldg(Fb, j), ldg(Sb, j), ldg(mush, j)};
+        [&](uint32_t o) {
+            return AdjNbr{at(S.posm, o), at(S.velm, o), at(Fb, o), at(Sb, o), at(mush, o)};
         },
-        [&](uint32_t j, const AdjNbr& nb, bool listed) {
+        [&](uint32_t o, const AdjNbr& nb, bool listed) {
             const float Dx = nb.p.x - pa.x, Dy = nb.p.y - pa.y, Dz = nb.p.z - pa.z;
             float s = Dx * Dx + Dy * Dy + Dz * Dz;
-            const bool valid = force_valid(P, listed, (uint32_t)a, j, pa.x, pa.y, pa.z, nb.p, s);
-            s = s < 1.17549435e-38f ? 1.17549435e-38f : s;
+            const bool valid = force_valid(P, listed, (uint32_t)a << 4, o, pa.x, pa.y, pa.z, nb.p, s);
+            s = __builtin_elementwise_maximum(s, 1.17549435e-38f);
             // padding slot (the particle itself, D = 0): evaluate the kernels at
             // r = h/2 so every term stays finite before its zero weight applies
             s = valid ? s : 0.25f * h2;
@@ -319,11 +319,11 @@ __global__ __launch_bounds__(256) void k_adj_density(SceneTable T, AdjSlot S,
         float4 p;
         float rb;
     };
-    gather<L, kAllWaves, 2>(
-        act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, zwin(P, d), start, S.posm, lst, tid,
+    gather<L, kAllWaves, 2, false, true>(
+        act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, zwin(P, d), ColLocal{}, start, S.posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return density_maybe(P, pa.x, pa.y, pa.z, pj); },
-        [&](uint32_t j) { return DN{S.posm[j], rhob[j]}; },
-        [&](uint32_t j, const DN& nb, bool listed) {
+        [&](uint32_t o) { return DN{at(S.posm, o), at(rhob, o)}; },
+        [&](uint32_t o, const DN& nb, bool listed) {
             const float Dx = nb.p.x - pa.x, Dy = nb.p.y - pa.y, Dz = nb.p.z - pa.z;
             const float s = Dx * Dx + Dy * Dy + Dz * Dz;
             const bool valid = density_valid(P, listed, pa.x, pa.y, pa.z, nb.p, s);

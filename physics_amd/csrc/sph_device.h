@@ -355,11 +355,25 @@ __device__ __forceinline__ T ldg(const T* __restrict__ base, uint32_t idx) {
     return *(const T*)((const char*)base + (idx * (uint32_t)sizeof(T)));
 }
 
+// Element o / 16 of an array, o the byte offset of a 16-byte record (the
+// neighbour lists hold these offsets, so 16-byte loads need no shift).
+template <class T>
+__device__ __forceinline__ T at(const T* __restrict__ base, uint32_t o) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "4, 8 or 16-byte elements");
+    constexpr uint32_t sh = sizeof(T) == 16 ? 0u : (sizeof(T) == 8 ? 1u : 2u);
+    return *(const T*)((const char*)base + (o >> sh));
+}
+
 // test(j, pj) -> bool, a conservative FP32 filter (every true neighbour
-// passes; the exact decision is proc's); load(j) -> T; proc(j, T, listed)
-// accumulates (a slot that is not listed, or that proc's exact test rejects,
-// must contribute exactly nothing); flush() after every drain. self_j is a
-// valid index used to pad drain slots. Overflow-segment candidates
+// passes; the exact decision is proc's); load(o) -> T and proc(o, T, listed)
+// take the byte offset o = 16 j of candidate j (proc accumulates; a slot that
+// is not listed, or that proc's exact test rejects, must contribute exactly
+// nothing); flush() after every drain. self_j is a valid index: every list
+// slot starts as its offset, and every slot written later holds a valid
+// candidate, so drain slots beyond a lane's count read valid records. With
+// PAD those slots are replaced by self_j itself: a proc that multiplies its
+// zero weights into the neighbour's data needs finite data there (a stale
+// slot may name a NaN particle); a proc that selects its terms does not. Overflow-segment candidates
 // additionally need reference key adjacency. Appends are branch-free: every
 // candidate is written at the list head and the head advances only when the
 // filter passes.
@@ -449,25 +463,52 @@ __device__ __forceinline__ float f32_up(double v) {
 // [zlo, zhi]. Inside a dense cell the members are ordered by z, and the cells
 // of a run follow in z order (keys are monotone in z), so the run is sorted by
 // z: the candidates below zlo form a prefix and those above zhi a suffix,
-// found 64 at a time with one coalesced load and a ballot.
-__device__ __forceinline__ void trim_run(const float4* __restrict__ posm, float zlo, float zhi,
+// found 64 at a time with one coalesced load and a ballot. ZW: z is the w
+// component of src (column-local coordinates), else its z component.
+template <bool ZW>
+__device__ __forceinline__ void trim_run(const float4* __restrict__ src, float zlo, float zhi,
                                          uint32_t& a, uint32_t& b) {
     const uint32_t lane = threadIdx.x & 63u;
+    auto zof = [&](uint32_t j) { return ZW ? src[j].w : src[j].z; };
     for (;;) {
         const uint32_t j = a + lane;
-        const bool below = j < b && posm[j].z < zlo;
+        const bool below = j < b && zof(j) < zlo;
         const uint32_t nb = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(below));
         a += nb;
         if (nb < 64u) break;
     }
     for (;;) {
         const uint32_t j = b - 1u - lane;
-        const bool above = lane < b - a && posm[j].z > zhi;
+        const bool above = lane < b - a && zof(j) > zhi;
         const uint32_t na = (uint32_t)__popcll(__builtin_amdgcn_ballot_w64(above));
         b -= na;
         if (na < 64u) break;
     }
 }
+
+// Column-local coordinates of a particle in dense cell c: (x - X, y - Y,
+// (x - X)^2 + (y - Y)^2, z), X and Y the lower corner of its half-cell column.
+// The uniform scan filters with them: for a candidate of row (ex, ey) and a
+// lane with local (u, v), dx = x_j - (u - ex h/2) and likewise dy, so
+// dx^2 + dy^2 = n_j - 2 xr x_j - 2 yr y_j + (xr^2 + yr^2): two FMAs per
+// candidate against per-row lane constants, all terms of size h^2 (no
+// cancellation of large coordinates).
+__device__ __forceinline__ float4 col_local(const float4& p, uint32_t c, const Domain& d, double half) {
+    const CellCoord cc = decode(c, d);
+    const double X = (double)(d.kmin[0] + cc.cx) * half, Y = (double)(d.kmin[1] + cc.cy) * half;
+    const float xl = (float)((double)p.x - X), yl = (float)((double)p.y - Y);
+    return make_float4(xl, yl, fmaf(yl, yl, xl * xl), p.z);
+}
+
+// The column-local filter of a uniform scan (posl: column-local coordinates
+// of the sorted particles, kScanPad spare entries; li: the lane's own; hh:
+// h / 2; s_hq: the bound, NaN to pass everything). Conservative: its FP32
+// rounding is below 1e-5 h^2 while s_hq sits s_thresh 2^-16 above s_thresh.
+struct ColLocal {
+    const float4* __restrict__ posl;
+    float4 li;
+    float hh, s_hq;
+};
 
 // Run of row k of a column wave's shared window: z cells cz0 - kReach ..
 // cz1 + kReach of column (cx + dx, cy + dy).
@@ -489,16 +530,23 @@ __device__ __forceinline__ void col_run(int k, const CellCoord& cc, long long cz
 
 // S: list slots drained per round (1 or 2; 2 keeps two neighbour loads in
 // flight, 1 needs fewer registers).
-// zt: the z half-widths of the uniform scan's trimmed rows.
-template <int L, int MODE, int S, class Test, class Load, class Proc, class Flush>
+// zt: the z half-widths of the uniform scan's trimmed rows. With LOC the
+// uniform scan filters with the column-local coordinates of cl instead of
+// test (which still filters the overflow segment and the per-lane scan).
+template <int L, int MODE, int S, bool LOC, bool PAD, class Test, class Load, class Proc,
+          class Flush>
 __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi, float zi,
                                        uint32_t self_j, const Domain& d, double inv,
-                                       const ZWin& zt, const uint32_t* __restrict__ start,
+                                       const ZWin& zt, const ColLocal& cl,
+                                       const uint32_t* __restrict__ start,
                                        const float4* __restrict__ posm, uint32_t (*lst)[256],
                                        int tid, Test&& test, Load&& load, Proc&& proc,
                                        Flush&& flush) {
     static_assert(S == 1 || S == 2, "drain width");
+    static_assert(L % 2 == 0, "even list length (two slots per drain round)");
     int cnt = 0;
+    const uint32_t self_o = self_j << 4;
+    for (int k = 0; k < L; ++k) lst[k][tid] = self_o;
     auto drain = [&]() {
         // do-while: the loop test sits at the bottom, so the accumulators
         // stay in place across iterations
@@ -511,25 +559,31 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
         do {
             if (S == 2) {
                 const bool v0 = k < cnt, v1 = k + 1 < cnt;
-                const uint32_t j0 = v0 ? lst[k][tid] : self_j;
-                const uint32_t j1 = v1 ? lst[k + 1][tid] : self_j;
-                auto t0 = load(j0);
-                auto t1 = load(j1);
-                proc(j0, t0, v0);
-                proc(j1, t1, v1);
+                uint32_t o0 = lst[k][tid], o1 = lst[k + 1][tid];
+                if (PAD) {
+                    o0 = v0 ? o0 : self_o;
+                    o1 = v1 ? o1 : self_o;
+                }
+                auto t0 = load(o0);
+                auto t1 = load(o1);
+                proc(o0, t0, v0);
+                proc(o1, t1, v1);
             } else {
                 const bool v0 = k < cnt;
-                const uint32_t j0 = v0 ? lst[k][tid] : self_j;
-                auto t0 = load(j0);
-                proc(j0, t0, v0);
+                uint32_t o0 = lst[k][tid];
+                if (PAD) o0 = v0 ? o0 : self_o;
+                auto t0 = load(o0);
+                proc(o0, t0, v0);
             }
             k += S;
         } while (wave_any(k < cnt));
         cnt = 0;
         flush();
     };
+    // j must be a valid index even when ok is false (it stays in the list
+    // until overwritten, and drains read every slot)
     auto push = [&](uint32_t j, bool ok) {
-        lst[cnt][tid] = j;
+        lst[cnt][tid] = j << 4;
         cnt += ok ? 1 : 0;
     };
     auto ov_ok = [&](const float4& pj) {
@@ -549,29 +603,44 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
             col_run(row, cc, cz1, d, start, a, b);
             a = uni(a);
             b = uni(b);
+            const int ex = row / kSpan - kReach, ey = row % kSpan - kReach;
             if (zt.on && a < b) {
-                const int ex = row / kSpan - kReach, ey = row % kSpan - kReach;
                 const int cls = (ex == kReach || ex == -kReach) + (ey == kReach || ey == -kReach);
                 const double w = cls == 0 ? zt.w0 : (cls == 1 ? zt.w1 : zt.w2);
-                trim_run(posm, f32_down(zmn - w), f32_up(zmx + w), a, b);
+                trim_run<LOC>(LOC ? cl.posl : posm, f32_down(zmn - w), f32_up(zmx + w), a, b);
                 a = uni(a);
                 b = uni(b);
             }
+            // the lane's position relative to the row's column corner
+            const float xr = cl.li.x - (float)ex * cl.hh, yr = cl.li.y - (float)ey * cl.hh;
+            const float mx = -2.f * xr, my = -2.f * yr;
+            const float thr = cl.s_hq - fmaf(yr, yr, xr * xr);
+            auto loc_maybe = [&](const float4& q) {
+                const float dz = q.w - zi;
+                const float t = fmaf(q.y, my, fmaf(q.x, mx, fmaf(dz, dz, q.z)));
+                return !(t >= thr);  // a NaN bound passes everything
+            };
             for (uint32_t j = a; j < b; j += 8) {
                 const uint32_t last = b - 1;
                 // two 64-byte wave-uniform loads for eight candidates (the
                 // array has kScanPad spare entries; candidates past the run
                 // are masked below)
-                const Quad4 qa = *reinterpret_cast<const Quad4*>(posm + j);
-                const Quad4 qb = *reinterpret_cast<const Quad4*>(posm + j + 4);
+                const float4* __restrict__ src = LOC ? cl.posl : posm;
+                const Quad4 qa = *reinterpret_cast<const Quad4*>(src + j);
+                const Quad4 qb = *reinterpret_cast<const Quad4*>(src + j + 4);
                 // non-short-circuit: no branches; cnt <= L - 8 here, so the
                 // eight writes stay inside the list
+                // a candidate past the run is written as the run's last one
+                // (a valid index; its flag is false)
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u)
-                    push(j + u, test(j + u, qa.p[u]) & (act & (j + u <= last)));
+                    push(min(j + u, last), (LOC ? loc_maybe(qa.p[u]) : test(j + u, qa.p[u])) &
+                                               (act & (j + u <= last)));
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u)
-                    push(j + 4 + u, test(j + 4 + u, qb.p[u]) & (act & (j + 4 + u <= last)));
+                    push(min(j + 4 + u, last),
+                         (LOC ? loc_maybe(qb.p[u]) : test(j + 4 + u, qb.p[u])) &
+                             (act & (j + 4 + u <= last)));
                 if (wave_any(cnt > L - 8)) drain();
             }
         }
@@ -651,7 +720,8 @@ __device__ __forceinline__ bool density_valid(const DevParams& P, bool listed, f
 }
 
 // Exact force-pass decision for a listed slot j of particle i (world.rs:
-// 292-299): skip j == i, r >= h and r <= 0; a NaN distance is kept. s32 is the
+// 292-299): skip j == i, r >= h and r <= 0; a NaN distance is kept. i and j
+// are compared for equality only (indices or list offsets alike). s32 is the
 // slot's FP32 squared distance. Clear cases are decided in FP32; the f64 band,
 // NaN and coincident distinct points take the exact f64 predicate, entered only
 // when some lane of the wave needs it.
@@ -665,6 +735,27 @@ __device__ __forceinline__ bool force_valid(const DevParams& P, bool listed, uin
         const double e = exact_s64(xi, yi, zi, pj.x, pj.y, pj.z);
         const bool nz = xi != pj.x || yi != pj.y || zi != pj.z;
         ex = (e < P.s_thresh && nz) || e != e;
+    }
+    return listed && (fast || ex);
+}
+
+// force_valid for a proc that multiplies its zero weights into the
+// neighbour's velocity: a listed pair the exact test rejects (a band pair
+// beyond h, a coincident point) gets the particle's own velocity vi in vj, so
+// a NaN velocity there cannot reach the sums. The replacement sits inside the
+// rarely taken exact branch.
+__device__ __forceinline__ bool force_valid_v(const DevParams& P, bool listed, uint32_t i,
+                                              uint32_t j, float xi, float yi, float zi,
+                                              const float4& pj, float s32, const float4& vi,
+                                              float4& vj) {
+    const bool fast = s32 < P.s_lo && s32 > 0.f;
+    const bool need = listed && j != i && !fast;
+    bool ex = false;
+    if (need) {
+        const double e = exact_s64(xi, yi, zi, pj.x, pj.y, pj.z);
+        const bool nz = xi != pj.x || yi != pj.y || zi != pj.z;
+        ex = (e < P.s_thresh && nz) || e != e;
+        if (!ex) vj = vi;
     }
     return listed && (fast || ex);
 }

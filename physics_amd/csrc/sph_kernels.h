@@ -53,10 +53,12 @@ struct SortBuffers {
     uint32_t* mat_out;
     float4* body_out;
     uint32_t* cell_out;
+    float4* posl_out;         // column-local coordinates of the uniform scan (may be null)
 };
 
 struct ForceArgs {
     const float4* posm;
+    const float4* posl;       // column-local coordinates (k_reorder), kScanPad spare entries
     const float4* velm;
     const float* rho;
     const float* prs;
@@ -124,9 +126,9 @@ void launch_adjoint_step(hipStream_t st, const SceneTable& T, const AdjSlot& S, 
 // 5 per tile.
 void launch_domain(hipStream_t st, const SceneTable& T, const float4* posm);
 void launch_sort(hipStream_t st, const SceneTable& T, const SortBuffers& b);
-void launch_density(hipStream_t st, const SceneTable& T, const float4* posm, const uint32_t* cellv,
-                    const uint32_t* start, float* rho, float* prs, float4* velm, float* mp,
-                    const Tuning& tu);
+void launch_density(hipStream_t st, const SceneTable& T, const float4* posm, const float4* posl,
+                    const uint32_t* cellv, const uint32_t* start, float* rho, float* prs,
+                    float4* velm, float* mp, const Tuning& tu);
 void launch_force(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a, bool shear,
                   const Tuning& tu);
 void launch_diag(hipStream_t st, const SceneTable& T, const float4* posm, const float4* vel,

@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void k_reorder(
     const uint32_t* __restrict__ start, const float4* __restrict__ posm_in,
     const float4* __restrict__ vel_in, const uint32_t* __restrict__ mat_in,
     const float4* __restrict__ body_in, float4* posm_out, float4* velm_out, uint32_t* id_out,
-    uint32_t* mat_out, float4* body_out, uint32_t* cell_out) {
+    uint32_t* mat_out, float4* body_out, uint32_t* cell_out, float4* posl_out) {
     const TileLoc L = tile_loc(T);
     if (!L.act) return;
     const int q = L.i;
@@ -367,7 +367,11 @@ __global__ __launch_bounds__(256) void k_reorder(
         }
     }
     uint32_t dst = lo + r;
-    posm_out[dst] = posm_in[s];
+    const float4 p = posm_in[s];
+    posm_out[dst] = p;
+    if (posl_out)
+        posl_out[dst] = c == ovf ? make_float4(0.f, 0.f, 0.f, 0.f)
+                                 : col_local(p, c - toff, T.dom[L.s], 0.5 * T.params[L.s].h64);
     float4 v = vel_in[s];
     v.w = 0.f;
     velm_out[dst] = v;
@@ -387,6 +391,7 @@ __global__ __launch_bounds__(256) void k_reorder(
 
 template <int L, int MODE>
 __global__ __launch_bounds__(256) void k_density(SceneTable T, const float4* __restrict__ posm,
+                                                 const float4* __restrict__ posl,
                                                  const uint32_t* __restrict__ cellv,
                                                  const uint32_t* __restrict__ start_all,
                                                  float* rho_out, float* p_out, float4* velm,
@@ -402,13 +407,18 @@ __global__ __launch_bounds__(256) void k_density(SceneTable T, const float4* __r
     const DevParams P = T.params[TL.s];
     const uint32_t* __restrict__ start = start_all + d.toff;
     float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (act) pi = posm[i];
+    constexpr bool LOC = MODE != kOtherWaves;
+    ColLocal cl{posl, make_float4(0.f, 0.f, 0.f, 0.f), (float)(0.5 * P.h64), P.s_hq};
+    if (act) {
+        pi = posm[i];
+        if (LOC) cl.li = posl[i];
+    }
     const double xi = pi.x, yi = pi.y, zi = pi.z;
     double sum = 0.0;
-    gather<L, MODE, (MODE == kUniformWaves ? 1 : 2)>(
-        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), start, posm, lst, tid,
+    gather<L, MODE, (MODE == kUniformWaves ? 1 : 2), LOC, false>(
+        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), cl, start, posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return density_maybe(P, pi.x, pi.y, pi.z, pj); },
-        [&](uint32_t j) { return ldg(posm, j); },
+        [&](uint32_t o) { return at(posm, o); },
         [&](uint32_t j, const float4& pj, bool listed) {
             // the reference's own f64 distance decides r < h exactly (NaN fails)
             const double dx = xi - (double)pj.x, dy = yi - (double)pj.y, dz = zi - (double)pj.z;
@@ -460,13 +470,14 @@ template <bool SHEAR>
 __device__ __forceinline__ void pair_force(ForceAcc<SHEAR>& A, const DevParams& P, uint32_t i,
                                            uint32_t j, float4 pi, float4 vi, float Pi,
                                            const NbrData& nb, bool listed) {
-    const float4 pj = nb.p, vj = nb.v;
+    const float4 pj = nb.p;
+    float4 vj = nb.v;
     float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
     float s = dx * dx + dy * dy + dz * dz;
-    const bool valid = force_valid(P, listed, i, j, pi.x, pi.y, pi.z, pj, s);
+    const bool valid = force_valid_v(P, listed, i, j, pi.x, pi.y, pi.z, pj, s, vi, vj);
     // distinct FP32 points closer than ~1e-19 underflow s; keep r tiny but finite.
-    // (not fmaxf: a NaN distance must stay NaN, as in the reference)
-    s = s < 1.17549435e-38f ? 1.17549435e-38f : s;
+    // (the IEEE maximum: a NaN distance stays NaN, as in the reference)
+    s = __builtin_elementwise_maximum(s, 1.17549435e-38f);
     float inv_r = __builtin_amdgcn_rsqf(s);
     float r = s * inv_r;
     float hr = P.h - r;
@@ -504,7 +515,8 @@ __device__ __forceinline__ void pair_force(ForceAcc<SHEAR>& A, const DevParams& 
 
 template <int L, bool SHEAR, int MODE>
 __global__ __launch_bounds__(256) void k_force(
-    SceneTable T, float dt, const float4* __restrict__ posm, const float4* __restrict__ velm,
+    SceneTable T, float dt, const float4* __restrict__ posm, const float4* __restrict__ posl,
+    const float4* __restrict__ velm,
     const float* __restrict__ rho, const float* __restrict__ prs, const float* __restrict__ mp,
     const uint32_t* __restrict__ cellv, const uint32_t* __restrict__ start_all,
     const uint32_t* __restrict__ idv, const uint32_t* __restrict__ matv,
@@ -528,14 +540,17 @@ __global__ __launch_bounds__(256) void k_force(
         rho_i = rho[i];
         Pi = prs[i] / (rho_i * rho_i);
     }
+    constexpr bool LOC = MODE != kOtherWaves;
+    ColLocal cl{posl, make_float4(0.f, 0.f, 0.f, 0.f), (float)(0.5 * P.h64), P.s_hq};
+    if (LOC && act) cl.li = posl[i];
     ForceAcc<SHEAR> A;
     A.zero();
-    gather<L, MODE, (MODE == kUniformWaves ? 1 : 2)>(
-        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), start, posm, lst, tid,
+    gather<L, MODE, (MODE == kUniformWaves ? 1 : 2), LOC, true>(
+        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), cl, start, posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_maybe(P, pi.x, pi.y, pi.z, pj); },
-        [&](uint32_t j) { return NbrData{ldg(posm, j), ldg(velm, j), ldg(mp, j)}; },
-        [&](uint32_t j, const NbrData& nb, bool listed) {
-            pair_force<SHEAR>(A, P, (uint32_t)i, j, pi, vi, Pi, nb, listed);
+        [&](uint32_t o) { return NbrData{at(posm, o), at(velm, o), at(mp, o)}; },
+        [&](uint32_t o, const NbrData& nb, bool listed) {
+            pair_force<SHEAR>(A, P, (uint32_t)i << 4, o, pi, vi, Pi, nb, listed);
         },
         [&]() { A.merge(); });
     if (!act) return;
@@ -755,7 +770,7 @@ void launch_sort(hipStream_t st, const SceneTable& T, const SortBuffers& b) {
     if (T.ntiles > 0)
         hipLaunchKernelGGL(k_reorder, dim3(T.ntiles), dim3(256), 0, st, T, b.tmp, b.idtmp, b.ztmp,
                            b.cell, b.start, b.posm_in, b.vel_in, b.mat_in, b.body_in, b.posm_out, b.velm_out,
-                           b.id_out, b.mat_out, b.body_out, b.cell_out);
+                           b.id_out, b.mat_out, b.body_out, b.cell_out, b.posl_out);
 }
 
 void launch_exclusive_scan(hipStream_t st, uint32_t* counts, const uint32_t* total, uint32_t* partial,
@@ -772,35 +787,35 @@ bool tuning_valid(const Tuning& t) {
 
 template <int L>
 static void density_l(hipStream_t st, const SceneTable& T, bool split, const float4* posm,
-                      const uint32_t* cellv, const uint32_t* start, float* rho, float* prs,
-                      float4* velm, float* mp) {
+                      const float4* posl, const uint32_t* cellv, const uint32_t* start, float* rho,
+                      float* prs, float4* velm, float* mp) {
     if (split) {
         hipLaunchKernelGGL((k_density<L, kUniformWaves>), dim3(T.ntiles), dim3(256), 0, st, T, posm,
-                           cellv, start, rho, prs, velm, mp);
+                           posl, cellv, start, rho, prs, velm, mp);
         hipLaunchKernelGGL((k_density<L, kOtherWaves>), dim3(T.ntiles), dim3(256), 0, st, T, posm,
-                           cellv, start, rho, prs, velm, mp);
+                           posl, cellv, start, rho, prs, velm, mp);
     } else {
-        hipLaunchKernelGGL((k_density<L, kAllWaves>), dim3(T.ntiles), dim3(256), 0, st, T, posm, cellv,
-                           start, rho, prs, velm, mp);
+        hipLaunchKernelGGL((k_density<L, kAllWaves>), dim3(T.ntiles), dim3(256), 0, st, T, posm, posl,
+                           cellv, start, rho, prs, velm, mp);
     }
 }
 
-void launch_density(hipStream_t st, const SceneTable& T, const float4* posm, const uint32_t* cellv,
-                    const uint32_t* start, float* rho, float* prs, float4* velm, float* mp,
-                    const Tuning& tu) {
+void launch_density(hipStream_t st, const SceneTable& T, const float4* posm, const float4* posl,
+                    const uint32_t* cellv, const uint32_t* start, float* rho, float* prs,
+                    float4* velm, float* mp, const Tuning& tu) {
     if (T.ntiles == 0) return;
     const bool sp = tu.split != 0;
     switch (tu.density_list) {
-        case 8: density_l<8>(st, T, sp, posm, cellv, start, rho, prs, velm, mp); break;
-        case 16: density_l<16>(st, T, sp, posm, cellv, start, rho, prs, velm, mp); break;
-        default: density_l<32>(st, T, sp, posm, cellv, start, rho, prs, velm, mp); break;
+        case 8: density_l<8>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp); break;
+        case 16: density_l<16>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp); break;
+        default: density_l<32>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp); break;
     }
 }
 
 template <int L, bool SHEAR, int MODE>
 static void force_one(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a) {
     hipLaunchKernelGGL((k_force<L, SHEAR, MODE>), dim3(T.ntiles), dim3(256), 0, st, T, dt, a.posm,
-                       a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body, a.posm_out,
+                       a.posl, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body, a.posm_out,
                        a.vel_out, a.id_out, a.mat_out, a.acc_out, a.mu_out, a.xsph_out, a.fvisc_out);
 }
 
