@@ -528,8 +528,8 @@ __device__ __forceinline__ void col_run(int k, const CellCoord& cc, long long cz
     b = start[base + z1 + 1];
 }
 
-// S: list slots drained per round (1 or 2; 2 keeps two neighbour loads in
-// flight, 1 needs fewer registers).
+// S: list slots drained per round (1, 2 or 4: more keeps more neighbour
+// loads in flight, fewer needs fewer registers).
 // zt: the z half-widths of the uniform scan's trimmed rows. With LOC the
 // uniform scan filters with the column-local coordinates of cl instead of
 // test (which still filters the overflow segment and the per-lane scan).
@@ -542,8 +542,8 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                                        const float4* __restrict__ posm, uint32_t (*lst)[256],
                                        int tid, Test&& test, Load&& load, Proc&& proc,
                                        Flush&& flush) {
-    static_assert(S == 1 || S == 2, "drain width");
-    static_assert(L % 2 == 0, "even list length (two slots per drain round)");
+    static_assert(S == 1 || S == 2 || S == 4, "drain width");
+    static_assert(L % S == 0, "whole drain rounds per list");
     int cnt = 0;
     const uint32_t self_o = self_j << 4;
     for (int k = 0; k < L; ++k) lst[k][tid] = self_o;
@@ -557,7 +557,25 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
         }
         int k = 0;
         do {
-            if (S == 2) {
+            if (S == 4) {
+                const bool v0 = k < cnt, v1 = k + 1 < cnt, v2 = k + 2 < cnt, v3 = k + 3 < cnt;
+                uint32_t o0 = lst[k][tid], o1 = lst[k + 1][tid], o2 = lst[k + 2][tid],
+                         o3 = lst[k + 3][tid];
+                if (PAD) {
+                    o0 = v0 ? o0 : self_o;
+                    o1 = v1 ? o1 : self_o;
+                    o2 = v2 ? o2 : self_o;
+                    o3 = v3 ? o3 : self_o;
+                }
+                auto t0 = load(o0);
+                auto t1 = load(o1);
+                auto t2 = load(o2);
+                auto t3 = load(o3);
+                proc(o0, t0, v0);
+                proc(o1, t1, v1);
+                proc(o2, t2, v2);
+                proc(o3, t3, v3);
+            } else if (S == 2) {
                 const bool v0 = k < cnt, v1 = k + 1 < cnt;
                 uint32_t o0 = lst[k][tid], o1 = lst[k + 1][tid];
                 if (PAD) {

@@ -415,7 +415,9 @@ __global__ __launch_bounds__(256) void k_density(SceneTable T, const float4* __r
     }
     const double xi = pi.x, yi = pi.y, zi = pi.z;
     double sum = 0.0;
-    gather<L, MODE, (MODE == kUniformWaves ? 1 : 2), LOC, false>(
+    // four neighbour loads in flight: the register cost keeps this kernel's
+    // occupancy (force keeps two: four would cost it a wave per SIMD)
+    gather<L, MODE, (MODE == kUniformWaves ? 1 : 4), LOC, false>(
         act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), cl, start, posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return density_maybe(P, pi.x, pi.y, pi.z, pj); },
         [&](uint32_t o) { return at(posm, o); },
