@@ -176,7 +176,7 @@ double phy_fluid_lattice_mass(PhyWorldHandle *w, double spacing);
 
 /* Replace all particles of the first fluid subsystem (index order). vel, mass
  * and material may be NULL (zero velocity, SphParams::mass, material 0).
- * Values are rounded to FP32. 0 ok, -1 error. */
+ * Values are rounded to FP32. At most 2^27 particles. 0 ok, -1 error. */
 int32_t phy_fluid_set_particles(PhyWorldHandle *w, uintptr_t n, const double *pos3,
                                 const double *vel3, const double *mass,
                                 const uint32_t *material);
