@@ -188,13 +188,16 @@ struct Quad4 {
     float4 p[4];
 };
 
+// Coordinates of dense cell c < ncells. The table holds fewer than 2^32 cells,
+// so every extent fits 32 bits and 32-bit division suffices.
 __device__ __forceinline__ CellCoord decode(uint32_t c, const Domain& d) {
     CellCoord r;
-    long long dz = d.dims[2], dy = d.dims[1];
-    r.cz = c % dz;
-    long long t = c / dz;
-    r.cy = t % dy;
-    r.cx = t / dy;
+    const uint32_t dz = (uint32_t)d.dims[2], dy = (uint32_t)d.dims[1];
+    const uint32_t t = c / dz;
+    const uint32_t x = t / dy;
+    r.cz = (long long)(c - t * dz);
+    r.cy = (long long)(t - x * dy);
+    r.cx = (long long)x;
     return r;
 }
 

@@ -338,7 +338,9 @@ __global__ __launch_bounds__(256) void k_scatter(int n, const uint32_t* __restri
 // segment keeps ascending original index (grid.rs:37-38). Either order is a
 // function of the state alone. Slots of a scene stay inside the scene's
 // particle range, so the tile names the scene; the stored cell index is the
-// scene-local one.
+// scene-local one. A cell's members are consecutive slots, mostly inside one
+// tile: the tile's keys are staged in LDS and only the part of a cell outside
+// the tile is read from global memory.
 __global__ __launch_bounds__(256) void k_reorder(
     SceneTable T, const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ idtmp,
     const float* __restrict__ ztmp, const uint32_t* __restrict__ cell,
@@ -346,25 +348,41 @@ __global__ __launch_bounds__(256) void k_reorder(
     const float4* __restrict__ vel_in, const uint32_t* __restrict__ mat_in,
     const float4* __restrict__ body_in, float4* posm_out, float4* velm_out, uint32_t* id_out,
     uint32_t* mat_out, float4* body_out, uint32_t* cell_out, float4* posl_out) {
+    __shared__ float zsh[kTile];
+    __shared__ uint32_t idsh[kTile];
     const TileLoc L = tile_loc(T);
+    const int tid = threadIdx.x;
+    // the tile's slots [tb, te)
+    const SceneInfo* si = T.info + L.s;
+    const uint32_t tb = (uint32_t)(L.i - tid);
+    const uint32_t te = min(tb + (uint32_t)kTile, si->off + si->n);
+    if (L.act) {
+        zsh[tid] = ztmp[L.i];
+        idsh[tid] = idtmp[L.i];
+    }
+    __syncthreads();
     if (!L.act) return;
     const int q = L.i;
     const uint32_t toff = T.dom[L.s].toff;
     const uint32_t ovf = toff + T.dom[L.s].ncells;
     uint32_t s = tmp[q];
-    uint32_t my = idtmp[q];
+    const uint32_t my = idsh[tid];
     uint32_t c = cell[s];
     uint32_t lo = start[c], hi = start[c + 1];
+    // [lo, m0) and [m1, hi) lie outside the tile, [m0, m1) inside
+    const uint32_t m0 = min(max(lo, tb), hi), m1 = max(min(hi, te), m0);
     uint32_t r = 0;
     if (c == ovf) {
-        for (uint32_t t = lo; t < hi; ++t) r += idtmp[t] < my ? 1u : 0u;
+        for (uint32_t t = lo; t < m0; ++t) r += idtmp[t] < my ? 1u : 0u;
+        for (uint32_t t = m0; t < m1; ++t) r += idsh[t - tb] < my ? 1u : 0u;
+        for (uint32_t t = m1; t < hi; ++t) r += idtmp[t] < my ? 1u : 0u;
     } else {
         // dense cells hold finite positions only
-        const float mz = ztmp[q];
-        for (uint32_t t = lo; t < hi; ++t) {
-            const float zt = ztmp[t];
-            r += (zt < mz || (zt == mz && idtmp[t] < my)) ? 1u : 0u;
-        }
+        const float mz = zsh[tid];
+        auto before = [&](float zt, uint32_t it) { return zt < mz || (zt == mz && it < my); };
+        for (uint32_t t = lo; t < m0; ++t) r += before(ztmp[t], idtmp[t]) ? 1u : 0u;
+        for (uint32_t t = m0; t < m1; ++t) r += before(zsh[t - tb], idsh[t - tb]) ? 1u : 0u;
+        for (uint32_t t = m1; t < hi; ++t) r += before(ztmp[t], idtmp[t]) ? 1u : 0u;
     }
     uint32_t dst = lo + r;
     const float4 p = posm_in[s];
