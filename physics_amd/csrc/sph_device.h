@@ -540,7 +540,7 @@ template <int L, int MODE, int S, bool LOC, bool PAD, class Test, class Load, cl
           class Flush>
 __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi, float zi,
                                        uint32_t self_j, const Domain& d, double inv,
-                                       const ZWin& zt, const ColLocal& cl,
+                                       const ZWin zt, const ColLocal& cl,
                                        const uint32_t* __restrict__ start,
                                        const float4* __restrict__ posm, uint32_t (*lst)[256],
                                        int tid, Test&& test, Load&& load, Proc&& proc,
@@ -619,6 +619,9 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
         // below or above it is no lane's neighbour
         const double zmn = (double)wave_min_f(act ? zi : INFINITY);
         const double zmx = (double)wave_max_f(act ? zi : -INFINITY);
+        // the widths as plain values: a select between fields of a struct in
+        // memory becomes a load from a selected address, kept on the stack
+        const double zw0 = zt.w0, zw1 = zt.w1, zw2 = zt.w2;
         for (int row = 0; row < kRows; ++row) {
             uint32_t a, b;
             col_run(row, cc, cz1, d, start, a, b);
@@ -627,7 +630,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
             const int ex = row / kSpan - kReach, ey = row % kSpan - kReach;
             if (zt.on && a < b) {
                 const int cls = (ex == kReach || ex == -kReach) + (ey == kReach || ey == -kReach);
-                const double w = cls == 0 ? zt.w0 : (cls == 1 ? zt.w1 : zt.w2);
+                const double w = cls == 0 ? zw0 : (cls == 1 ? zw1 : zw2);
                 trim_run<LOC>(LOC ? cl.posl : posm, f32_down(zmn - w), f32_up(zmx + w), a, b);
                 a = uni(a);
                 b = uni(b);
