@@ -644,27 +644,41 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 const float t = fmaf(q.y, my, fmaf(q.x, mx, fmaf(dz, dz, q.z)));
                 return !(t >= thr);  // a NaN bound passes everything
             };
-            for (uint32_t j = a; j < b; j += 8) {
-                const uint32_t last = b - 1;
-                // two 64-byte wave-uniform loads for eight candidates (the
-                // array has kScanPad spare entries; candidates past the run
-                // are masked below)
-                const float4* __restrict__ src = LOC ? cl.posl : posm;
-                const Quad4 qa = *reinterpret_cast<const Quad4*>(src + j);
-                const Quad4 qb = *reinterpret_cast<const Quad4*>(src + j + 4);
-                // non-short-circuit: no branches; cnt <= L - 8 here, so the
-                // eight writes stay inside the list
-                // a candidate past the run is written as the run's last one
-                // (a valid index; its flag is false)
+            const float4* __restrict__ src = LOC ? cl.posl : posm;
+            auto maybe = [&](uint32_t j, const float4& q) { return LOC ? loc_maybe(q) : test(j, q); };
+            // Both ends of the z-sorted run at once: four candidates from the
+            // front and four from the back per round. Lanes sit at different z,
+            // so a front-to-back walk would fill the low lanes' lists first and
+            // the high lanes' last; taking both ends keeps the counts closer and
+            // the drains' lanes busier. Each round is two 64-byte wave-uniform
+            // loads; cnt <= L - 8 before it, so its eight writes stay inside
+            // the list (non-short-circuit flags: no branches).
+            uint32_t lo = a, hi = b;
+            while (hi - lo >= 8u) {
+                const Quad4 qa = *reinterpret_cast<const Quad4*>(src + lo);
+                const Quad4 qb = *reinterpret_cast<const Quad4*>(src + hi - 4u);
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) push(lo + u, maybe(lo + u, qa.p[u]) & act);
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) push(hi - 4u + u, maybe(hi - 4u + u, qb.p[u]) & act);
+                lo += 4u;
+                hi -= 4u;
+                if (wave_any(cnt > L - 8)) drain();
+            }
+            if (lo < hi) {
+                // the last 1..7 candidates: one round from lo (the arrays have
+                // kScanPad spare entries); a candidate past the run is written
+                // as the run's last one (a valid index) with a false flag
+                const uint32_t last = hi - 1u;
+                const Quad4 qa = *reinterpret_cast<const Quad4*>(src + lo);
+                const Quad4 qb = *reinterpret_cast<const Quad4*>(src + lo + 4u);
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u)
-                    push(min(j + u, last), (LOC ? loc_maybe(qa.p[u]) : test(j + u, qa.p[u])) &
-                                               (act & (j + u <= last)));
+                    push(min(lo + u, last), maybe(lo + u, qa.p[u]) & (act & (lo + u <= last)));
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u)
-                    push(min(j + 4 + u, last),
-                         (LOC ? loc_maybe(qb.p[u]) : test(j + 4 + u, qb.p[u])) &
-                             (act & (j + 4 + u <= last)));
+                    push(min(lo + 4u + u, last),
+                         maybe(lo + 4u + u, qb.p[u]) & (act & (lo + 4u + u <= last)));
                 if (wave_any(cnt > L - 8)) drain();
             }
         }
