@@ -622,7 +622,11 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
         // the widths as plain values: a select between fields of a struct in
         // memory becomes a load from a selected address, kept on the stack
         const double zw0 = zt.w0, zw1 = zt.w1, zw2 = zt.w2;
-        for (int row = 0; row < kRows; ++row) {
+        // rows in mirror order (0, 24, 1, 23, ..., 11, 13, 12): a row off to
+        // one side fills the lists of the lanes on that side first, and its
+        // mirror row right after evens the counts out
+        for (int kr = 0; kr < kRows; ++kr) {
+            const int row = (kr & 1) ? kRows - 1 - (kr >> 1) : (kr >> 1);
             uint32_t a, b;
             col_run(row, cc, cz1, d, start, a, b);
             a = uni(a);
