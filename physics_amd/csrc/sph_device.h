@@ -296,19 +296,37 @@ __device__ __forceinline__ void for_each_candidate(uint32_t ci, float xi, float 
 //  * per-lane: each lane walks its own rows with a cursor, one candidate per round.
 
 struct Cursor {
+    // The lane's window as rows of a rectangle of table columns [x0, x1] x
+    // [y0, y1] with z cells z0..z1 (empty when z0 > z1), walked row by row:
+    // x = x0 + row / step, y = y0 + row % step, step 5 (dense particle: the 5 x
+    // 5 half-cell window) or 6 (out-of-domain particle: the in-domain half
+    // cells of its 27 reference cells). 32-bit table-relative values with
+    // wrapping arithmetic: a row left of the table wraps to a huge x and fails
+    // x <= x1.
     int row, nrows;
+    uint32_t x0, x1, y0, y1, z0, z1, mul;
     uint32_t j, jend, o0, o1;
-    bool dense;
-    CellCoord cc;
-    long long lo[3], hi[3];  // out-of-domain particle: ref_block intervals
 
     __device__ __forceinline__ bool in_overflow() const { return row > nrows; }
+
+    __device__ __forceinline__ void run(int k, const Domain& d, const uint32_t* __restrict__ start) {
+        // k / step for k < 2^14 by multiply and shift (step 5: 0xCCCD, 6: 0xAAAB)
+        const uint32_t q = ((uint32_t)k * mul) >> 18;
+        const uint32_t step = mul == 0xAAABu ? 6u : 5u;
+        const uint32_t x = x0 + q, y = y0 + ((uint32_t)k - q * step);
+        const bool ok = x <= x1 && y <= y1 && z0 <= z1;
+        // loads from a valid address either way, then a select of values (a
+        // select of addresses would put a zero on the stack)
+        const uint32_t base = ok ? (x * (uint32_t)d.dims[1] + y) * (uint32_t)d.dims[2] : 0u;
+        const uint32_t sa = start[base + (ok ? z0 : 0u)], sb = start[base + (ok ? z1 + 1u : 0u)];
+        j = ok ? sa : 0u;
+        jend = ok ? sb : 0u;
+    }
 
     __device__ __forceinline__ void next_run(const Domain& d, const uint32_t* __restrict__ start) {
         while (j >= jend && row <= nrows) {
             if (row < nrows) {
-                if (dense) row_run(row, cc, d, start, j, jend);
-                else block_run(row, lo, hi, d, start, j, jend);
+                run(row, d, start);
             } else {
                 j = o0;
                 jend = o1;
@@ -324,20 +342,35 @@ struct Cursor {
         j = jend = 0;
         o0 = start[d.ncells];
         o1 = start[d.ncells + 1];
-        dense = ci < d.ncells;
+        const bool dense = ci < d.ncells;
+        mul = dense ? 0xCCCDu : 0xAAABu;
         nrows = dense ? kRows : 36;
-        cc = CellCoord{0, 0, 0};
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = 0;
-            hi[a] = -1;
-        }
+        x0 = y0 = z0 = 1u;
+        x1 = y1 = z1 = 0u;  // empty
         if (!act) return;
         if (dense) {
-            cc = decode(ci, d);
+            const CellCoord cc = decode(ci, d);
+            const uint32_t cx = (uint32_t)cc.cx, cy = (uint32_t)cc.cy, cz = (uint32_t)cc.cz;
+            const uint32_t r = (uint32_t)kReach;
+            x0 = cx - r;
+            y0 = cy - r;
+            x1 = min(cx + r, (uint32_t)d.dims[0] - 1u);
+            y1 = min(cy + r, (uint32_t)d.dims[1] - 1u);
+            z0 = cz > r ? cz - r : 0u;
+            z1 = min(cz + r, (uint32_t)d.dims[2] - 1u);
         } else {
-            ref_block(key1(xi, inv), d, 0, lo[0], hi[0]);
-            ref_block(key1(yi, inv), d, 1, lo[1], hi[1]);
-            ref_block(key1(zi, inv), d, 2, lo[2], hi[2]);
+            long long lx, hx, ly, hy, lz, hz;
+            ref_block(key1(xi, inv), d, 0, lx, hx);
+            ref_block(key1(yi, inv), d, 1, ly, hy);
+            ref_block(key1(zi, inv), d, 2, lz, hz);
+            if (lx <= hx && ly <= hy && lz <= hz) {
+                x0 = (uint32_t)lx;
+                x1 = (uint32_t)hx;
+                y0 = (uint32_t)ly;
+                y1 = (uint32_t)hy;
+                z0 = (uint32_t)lz;
+                z1 = (uint32_t)hz;
+            }
         }
         next_run(d, start);
     }
@@ -531,8 +564,17 @@ __device__ __forceinline__ void col_run(int k, const CellCoord& cc, long long cz
     b = start[base + z1 + 1];
 }
 
+// Slots per drain round as a type, so one drain can be instantiated at two
+// widths.
+template <int N>
+struct DrainWidth {
+    static constexpr int value = N;
+};
+
 // S: list slots drained per round (1, 2 or 4: more keeps more neighbour
-// loads in flight, fewer needs fewer registers).
+// loads in flight, fewer needs fewer registers) in the uniform scan; the
+// per-lane scan, whose cursor stays live across its drains, drains at most
+// two.
 // zt: the z half-widths of the uniform scan's trimmed rows. With LOC the
 // uniform scan filters with the column-local coordinates of cl instead of
 // test (which still filters the overflow segment and the per-lane scan).
@@ -550,7 +592,8 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
     int cnt = 0;
     const uint32_t self_o = self_j << 4;
     for (int k = 0; k < L; ++k) lst[k][tid] = self_o;
-    auto drain = [&]() {
+    auto drain = [&](auto width) {
+        constexpr int W = decltype(width)::value;
         // do-while: the loop test sits at the bottom, so the accumulators
         // stay in place across iterations
         if (!wave_any(0 < cnt)) {
@@ -560,7 +603,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
         }
         int k = 0;
         do {
-            if (S == 4) {
+            if (W == 4) {
                 const bool v0 = k < cnt, v1 = k + 1 < cnt, v2 = k + 2 < cnt, v3 = k + 3 < cnt;
                 uint32_t o0 = lst[k][tid], o1 = lst[k + 1][tid], o2 = lst[k + 2][tid],
                          o3 = lst[k + 3][tid];
@@ -578,7 +621,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 proc(o1, t1, v1);
                 proc(o2, t2, v2);
                 proc(o3, t3, v3);
-            } else if (S == 2) {
+            } else if (W == 2) {
                 const bool v0 = k < cnt, v1 = k + 1 < cnt;
                 uint32_t o0 = lst[k][tid], o1 = lst[k + 1][tid];
                 if (PAD) {
@@ -596,7 +639,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 auto t0 = load(o0);
                 proc(o0, t0, v0);
             }
-            k += S;
+            k += W;
         } while (wave_any(k < cnt));
         cnt = 0;
         flush();
@@ -667,7 +710,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 for (uint32_t u = 0; u < 4; ++u) push(hi - 4u + u, maybe(hi - 4u + u, qb.p[u]) & act);
                 lo += 4u;
                 hi -= 4u;
-                if (wave_any(cnt > L - 8)) drain();
+                if (wave_any(cnt > L - 8)) drain(DrainWidth<S>{});
             }
             if (lo < hi) {
                 // the last 1..7 candidates: one round from lo (the arrays have
@@ -683,14 +726,14 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 for (uint32_t u = 0; u < 4; ++u)
                     push(min(lo + 4u + u, last),
                          maybe(lo + 4u + u, qb.p[u]) & (act & (lo + 4u + u <= last)));
-                if (wave_any(cnt > L - 8)) drain();
+                if (wave_any(cnt > L - 8)) drain(DrainWidth<S>{});
             }
         }
         const uint32_t o0 = uni(start[d.ncells]), o1 = uni(start[d.ncells + 1]);
         for (uint32_t j = o0; j < o1; ++j) {
             const float4 pj = posm[j];
             push(j, act && ov_ok(pj) && test(j, pj));
-            if (wave_any(cnt == L)) drain();
+            if (wave_any(cnt == L)) drain(DrainWidth<S>{});
         }
     } else if (MODE != kUniformWaves) {
         Cursor cur;
@@ -717,10 +760,11 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 cur.j = min(j + 4, cur.jend);
                 if (cur.j >= cur.jend) cur.next_run(d, start);
             }
-            if (wave_any(cnt > L - 4)) drain();
+            if (wave_any(cnt > L - 4)) drain(DrainWidth<(S < 2 ? S : 2)>{});
         }
     }
-    drain();
+    if (uniform) drain(DrainWidth<S>{});
+    else drain(DrainWidth<(S < 2 ? S : 2)>{});
 }
 
 // Candidate filters of the compaction. Both are conservative: |s32 - s64| <=
