@@ -11,9 +11,9 @@
 //   k_diag_*                     FP64 mass / momentum / kinetic-energy reductions
 //                                (subsystem.rs:149-163)
 //
-// Every float sum is a gather in a fixed order (the reference's cell order
-// di, dj, dk with ascending original index inside a cell), compensated with
-// Kahan's algorithm, so repeated runs are bit-identical. The only atomics are
+// Every float sum is a gather in a fixed order (a function of the state: the
+// scan's row and candidate order over the cell order), compensated with
+// Kahan's algorithm or summed in f64, so repeated runs are bit-identical. The only atomics are
 // integer ones (cell counts, min/max of keys, watchdog indices) whose results do
 // not depend on arrival order.
 //
@@ -199,7 +199,7 @@ __global__ __launch_bounds__(256) void k_domain_finalize(SceneTable T) {
 // Particles arrive in the previous step's cell order, so equal cells form runs
 // of lanes: one atomic per run (wave-aggregated), each lane's arrival rank is
 // the run's base plus its position in the run. The arrival order is not
-// deterministic; k_reorder replaces it by the original-index order.
+// deterministic; k_reorder replaces it by the (z, original index) order.
 __global__ __launch_bounds__(256) void k_bin(SceneTable T, const float4* __restrict__ posm,
                                              uint32_t* cell, uint32_t* rank, uint32_t* counts) {
     const TileLoc L = tile_loc(T);

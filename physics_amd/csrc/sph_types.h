@@ -1,7 +1,8 @@
 // sph_types.h — plain-old-data shared by the host runtime and the gfx950 kernels.
 //
 // Particle state lives on the device in structure-of-arrays form, in cell order
-// (particles sorted by their uniform-grid cell, ties by original index):
+// (particles sorted by their uniform-grid cell; inside a dense cell by z, ties
+// by original index):
 //   posm[s] = float4(x, y, z, mass)        16 B
 //   velm[s] = float4(vx, vy, vz, m/rho)    16 B  (w filled by the density pass)
 //   mp[s]   = mass * p / rho^2             4 B   (filled by the density pass)
