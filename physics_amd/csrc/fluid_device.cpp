@@ -297,7 +297,7 @@ void DeviceFluid::ensure_capacity(size_t n) {
         posl_.ensure(c + kScanPad);
         id_s_.ensure(c); mat_s_.ensure(c); cell_s_.ensure(c);
         rho_.ensure(c); prs_.ensure(c); mp_.ensure(c);
-        cell_.ensure(c); rank_.ensure(c); tmp_.ensure(c); idtmp_.ensure(c); ztmp_.ensure(c);
+        cell_.ensure(c); rank_.ensure(c); tmp_.ensure(c); ctmp_.ensure(c); keytmp_.ensure(c);
         stage4_.ensure(c); stage1_.ensure(c);
         if (keep_aux_) { acc_.ensure(c); xsph_.ensure(c); mu_.ensure(c); }
         cap_n_ = c;
@@ -422,8 +422,8 @@ void DeviceFluid::run_domain_and_sort(const StepBufs& s) {
     b.partial = partial_.get();
     b.start = s.start;
     b.tmp = tmp_.get();
-    b.idtmp = idtmp_.get();
-    b.ztmp = ztmp_.get();
+    b.ctmp = ctmp_.get();
+    b.keytmp = keytmp_.get();
     b.posm_out = s.posm_s;
     b.velm_out = s.velm_s;
     b.id_out = s.id_s;

@@ -294,8 +294,8 @@ class DeviceFluid {
     DevBuf<uint32_t> id_s_, mat_s_, cell_s_;
     DevBuf<float> rho_, prs_, mp_;
     // sort scratch
-    DevBuf<uint32_t> cell_, rank_, counts_, partial_, start_, tmp_, idtmp_;
-    DevBuf<float> ztmp_;
+    DevBuf<uint32_t> cell_, rank_, counts_, partial_, start_, tmp_, ctmp_;
+    DevBuf<unsigned long long> keytmp_;
     // aux outputs
     DevBuf<float4> acc_, xsph_;
     DevBuf<float> mu_;

@@ -43,9 +43,9 @@ struct SortBuffers {
     uint32_t* counts;         // kept all-zero between steps
     uint32_t* partial;
     uint32_t* start;
-    uint32_t* tmp;
-    uint32_t* idtmp;
-    float* ztmp;              // z of each scatter slot (within-cell order)
+    uint32_t* tmp;                 // particle of each scatter slot
+    uint32_t* ctmp;                // its cell
+    unsigned long long* keytmp;    // its within-cell order key (z bits, index)
     // outputs (cell order)
     float4* posm_out;
     float4* velm_out;

@@ -48,9 +48,11 @@ __global__ __launch_bounds__(256) void k_domain_reset(Domain* dom, int nscenes) 
     d->bad_vel = 0xffffffffu;
 }
 
-// Key AABB per scene. Each workgroup walks a contiguous range of tiles and
-// flushes its running min / max with one set of atomics per scene it touched.
+// Key AABB per scene. Each workgroup walks a contiguous range of tiles, four
+// at a time with their loads issued together, and flushes its running min /
+// max with one set of atomics per scene it touched.
 __global__ __launch_bounds__(256) void k_aabb(SceneTable T, const float4* __restrict__ posm) {
+    constexpr int U = 4;
     __shared__ long long sh[6][4];
     const int per = (T.ntiles + (int)gridDim.x - 1) / (int)gridDim.x;
     const int t0 = (int)blockIdx.x * per;
@@ -90,23 +92,38 @@ __global__ __launch_bounds__(256) void k_aabb(SceneTable T, const float4* __rest
     };
     reset();
     int cur = -1;
-    double inv = 0.0;
-    for (int t = t0; t < t1; ++t) {
-        const int s = (int)T.tile_scene[t];
-        if (s != cur) {
-            if (cur >= 0) flush(cur);
-            reset();
-            cur = s;
-            inv = T.params[s].inv_cell;
+    for (int t = t0; t < t1; t += U) {
+        int sc[U];
+        bool ok[U];
+        float4 p[U];
+        double inv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) sc[u] = t + u < t1 ? (int)T.tile_scene[t + u] : -1;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            ok[u] = false;
+            inv[u] = 0.0;
+            p[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (sc[u] >= 0) {
+                const SceneInfo* si = T.info + sc[u];
+                const uint32_t local = (uint32_t)(t + u - (int)si->tile0) * (uint32_t)kTile + threadIdx.x;
+                ok[u] = local < si->n;
+                if (ok[u]) p[u] = posm[si->off + local];
+                inv[u] = T.params[sc[u]].inv_cell;
+            }
         }
-        const SceneInfo* si = T.info + s;
-        const uint32_t local = (uint32_t)(t - (int)si->tile0) * (uint32_t)kTile + threadIdx.x;
-        if (local < si->n) {
-            const float4 p = posm[si->off + local];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (sc[u] < 0) continue;
+            if (sc[u] != cur) {
+                if (cur >= 0) flush(cur);
+                reset();
+                cur = sc[u];
+            }
             long long k[3];
             // half keys; particles whose reference key fails go to the overflow segment
-            if (hkey((double)p.x * inv, k[0]) & hkey((double)p.y * inv, k[1]) &
-                hkey((double)p.z * inv, k[2])) {
+            if (ok[u] && (hkey((double)p[u].x * inv[u], k[0]) & hkey((double)p[u].y * inv[u], k[1]) &
+                          hkey((double)p[u].z * inv[u], k[2]))) {
                 for (int a = 0; a < 3; ++a) {
                     mn[a] = k[a] < mn[a] ? k[a] : mn[a];
                     mx[a] = k[a] > mx[a] ? k[a] : mx[a];
@@ -317,18 +334,30 @@ __global__ __launch_bounds__(256) void k_scan_final(uint32_t* counts,
     }
 }
 
+// Within-cell order key of a dense-cell member: z as order-preserving bits
+// (-0 folded onto +0) above the original index, so one unsigned compare is
+// the float compare of z with index tie-break. Overflow members compare the
+// low word (the index) only.
+__device__ __forceinline__ unsigned long long zid_key(float z, uint32_t id) {
+    uint32_t b = __float_as_uint(z);
+    b = b == 0x80000000u ? 0u : b;
+    b ^= (b & 0x80000000u) ? 0xffffffffu : 0x80000000u;
+    return ((unsigned long long)b << 32) | id;
+}
+
 __global__ __launch_bounds__(256) void k_scatter(int n, const uint32_t* __restrict__ cell,
                                                  const uint32_t* __restrict__ rank,
                                                  const uint32_t* __restrict__ start,
                                                  const uint32_t* __restrict__ id,
                                                  const float4* __restrict__ posm, uint32_t* tmp,
-                                                 uint32_t* idtmp, float* ztmp) {
+                                                 uint32_t* ctmp, unsigned long long* keytmp) {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    uint32_t q = start[cell[s]] + rank[s];
+    const uint32_t c = cell[s];
+    const uint32_t q = start[c] + rank[s];
     tmp[q] = (uint32_t)s;
-    idtmp[q] = id[s];
-    ztmp[q] = posm[s].z;
+    ctmp[q] = c;
+    keytmp[q] = zid_key(posm[s].z, id[s]);
 }
 
 // Slot q of the scatter holds some member of its cell in arrival order; its
@@ -338,52 +367,63 @@ __global__ __launch_bounds__(256) void k_scatter(int n, const uint32_t* __restri
 // segment keeps ascending original index (grid.rs:37-38). Either order is a
 // function of the state alone. Slots of a scene stay inside the scene's
 // particle range, so the tile names the scene; the stored cell index is the
-// scene-local one. A cell's members are consecutive slots, mostly inside one
-// tile: the tile's keys are staged in LDS and only the part of a cell outside
-// the tile is read from global memory.
+// scene-local one. The cells the tile touches span the slot window
+// [start of the first lane's cell, end of the last lane's cell); when that
+// window holds at most kReorderWin slots its keys are staged in LDS and every
+// lane counts from LDS. A longer window (a cell of more than ~1,800 members,
+// such as a crowded overflow segment) is counted from global memory.
+constexpr uint32_t kReorderWin = 2048;
+
 __global__ __launch_bounds__(256) void k_reorder(
-    SceneTable T, const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ idtmp,
-    const float* __restrict__ ztmp, const uint32_t* __restrict__ cell,
-    const uint32_t* __restrict__ start, const float4* __restrict__ posm_in,
-    const float4* __restrict__ vel_in, const uint32_t* __restrict__ mat_in,
-    const float4* __restrict__ body_in, float4* posm_out, float4* velm_out, uint32_t* id_out,
-    uint32_t* mat_out, float4* body_out, uint32_t* cell_out, float4* posl_out) {
-    __shared__ float zsh[kTile];
-    __shared__ uint32_t idsh[kTile];
+    SceneTable T, const uint32_t* __restrict__ tmp, const uint32_t* __restrict__ ctmp,
+    const unsigned long long* __restrict__ keytmp, const uint32_t* __restrict__ start,
+    const float4* __restrict__ posm_in, const float4* __restrict__ vel_in,
+    const uint32_t* __restrict__ mat_in, const float4* __restrict__ body_in, float4* posm_out,
+    float4* velm_out, uint32_t* id_out, uint32_t* mat_out, float4* body_out, uint32_t* cell_out,
+    float4* posl_out) {
+    __shared__ unsigned long long ksh[kReorderWin];
+    __shared__ uint32_t wsh[2];
     const TileLoc L = tile_loc(T);
     const int tid = threadIdx.x;
-    // the tile's slots [tb, te)
     const SceneInfo* si = T.info + L.s;
-    const uint32_t tb = (uint32_t)(L.i - tid);
-    const uint32_t te = min(tb + (uint32_t)kTile, si->off + si->n);
-    if (L.act) {
-        zsh[tid] = ztmp[L.i];
-        idsh[tid] = idtmp[L.i];
-    }
-    __syncthreads();
-    if (!L.act) return;
-    const int q = L.i;
+    const uint32_t te = si->off + si->n;  // end of the scene's slots
     const uint32_t toff = T.dom[L.s].toff;
     const uint32_t ovf = toff + T.dom[L.s].ncells;
-    uint32_t s = tmp[q];
-    const uint32_t my = idsh[tid];
-    uint32_t c = cell[s];
-    uint32_t lo = start[c], hi = start[c + 1];
-    // [lo, m0) and [m1, hi) lie outside the tile, [m0, m1) inside
-    const uint32_t m0 = min(max(lo, tb), hi), m1 = max(min(hi, te), m0);
-    uint32_t r = 0;
-    if (c == ovf) {
-        for (uint32_t t = lo; t < m0; ++t) r += idtmp[t] < my ? 1u : 0u;
-        for (uint32_t t = m0; t < m1; ++t) r += idsh[t - tb] < my ? 1u : 0u;
-        for (uint32_t t = m1; t < hi; ++t) r += idtmp[t] < my ? 1u : 0u;
-    } else {
-        // dense cells hold finite positions only
-        const float mz = zsh[tid];
-        auto before = [&](float zt, uint32_t it) { return zt < mz || (zt == mz && it < my); };
-        for (uint32_t t = lo; t < m0; ++t) r += before(ztmp[t], idtmp[t]) ? 1u : 0u;
-        for (uint32_t t = m0; t < m1; ++t) r += before(zsh[t - tb], idsh[t - tb]) ? 1u : 0u;
-        for (uint32_t t = m1; t < hi; ++t) r += before(ztmp[t], idtmp[t]) ? 1u : 0u;
+    uint32_t s = 0, c = 0, lo = 0, hi = 0;
+    if (L.act) {
+        s = tmp[L.i];
+        c = ctmp[L.i];
+        lo = start[c];
+        hi = start[c + 1];
     }
+    // lane 0 is active in every tile
+    if (tid == 0) wsh[0] = lo;
+    if (L.act && ((uint32_t)L.i + 1u == te || tid == kTile - 1)) wsh[1] = hi;
+    __syncthreads();
+    const uint32_t wlo = wsh[0], whi = wsh[1];
+    const bool win = whi - wlo <= kReorderWin;
+    if (win)
+        for (uint32_t t = wlo + (uint32_t)tid; t < whi; t += kTile) ksh[t - wlo] = keytmp[t];
+    __syncthreads();
+    if (!L.act) return;
+    uint32_t r = 0;
+    unsigned long long mk;
+    if (win) {
+        mk = ksh[(uint32_t)L.i - wlo];
+        if (c == ovf) {
+            for (uint32_t t = lo; t < hi; ++t) r += (uint32_t)ksh[t - wlo] < (uint32_t)mk ? 1u : 0u;
+        } else {
+            for (uint32_t t = lo; t < hi; ++t) r += ksh[t - wlo] < mk ? 1u : 0u;
+        }
+    } else {
+        mk = keytmp[L.i];
+        if (c == ovf) {
+            for (uint32_t t = lo; t < hi; ++t) r += (uint32_t)keytmp[t] < (uint32_t)mk ? 1u : 0u;
+        } else {
+            for (uint32_t t = lo; t < hi; ++t) r += keytmp[t] < mk ? 1u : 0u;
+        }
+    }
+    const uint32_t my = (uint32_t)mk;
     uint32_t dst = lo + r;
     const float4 p = posm_in[s];
     posm_out[dst] = p;
@@ -768,7 +808,10 @@ static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 void launch_domain(hipStream_t st, const SceneTable& T, const float4* posm) {
     hipLaunchKernelGGL(k_domain_reset, dim3(nblk(T.nscenes, 256)), dim3(256), 0, st, T.dom, T.nscenes);
     if (T.ntiles > 0) {
-        int nb = T.ntiles < 512 ? T.ntiles : 512;
+        // about eight workgroups per scene, at least 512: more workgroups per
+        // scene only add atomic traffic on that scene's domain
+        int nb = max(512, 8 * T.nscenes);
+        nb = T.ntiles < nb ? T.ntiles : nb;
         hipLaunchKernelGGL(k_aabb, dim3(nb), dim3(256), 0, st, T, posm);
     }
     hipLaunchKernelGGL(k_domain_finalize, dim3(1), dim3(256), 0, st, T);
@@ -786,10 +829,10 @@ void launch_sort(hipStream_t st, const SceneTable& T, const SortBuffers& b) {
                        b.partial, b.start);
     if (T.n > 0)
         hipLaunchKernelGGL(k_scatter, dim3(nblk(T.n, 256)), dim3(256), 0, st, T.n, b.cell, b.rank,
-                           b.start, b.id_in, b.posm_in, b.tmp, b.idtmp, b.ztmp);
+                           b.start, b.id_in, b.posm_in, b.tmp, b.ctmp, b.keytmp);
     if (T.ntiles > 0)
-        hipLaunchKernelGGL(k_reorder, dim3(T.ntiles), dim3(256), 0, st, T, b.tmp, b.idtmp, b.ztmp,
-                           b.cell, b.start, b.posm_in, b.vel_in, b.mat_in, b.body_in, b.posm_out, b.velm_out,
+        hipLaunchKernelGGL(k_reorder, dim3(T.ntiles), dim3(256), 0, st, T, b.tmp, b.ctmp, b.keytmp,
+                           b.start, b.posm_in, b.vel_in, b.mat_in, b.body_in, b.posm_out, b.velm_out,
                            b.id_out, b.mat_out, b.body_out, b.cell_out, b.posl_out);
 }
 
