@@ -181,10 +181,12 @@ __global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
         [&](uint32_t o) {
             return AdjNbr{at(S.posm, o), at(S.velm, o), at(Fb, o), at(Sb, o), at(mush, o)};
         },
-        [&](uint32_t o, const AdjNbr& nb, bool listed) {
+        // a slot at or past the lane's count names the lane itself, which the
+        // validity test rejects, so the listed flag is not needed
+        [&](uint32_t o, const AdjNbr& nb, bool) {
             const float Dx = nb.p.x - pa.x, Dy = nb.p.y - pa.y, Dz = nb.p.z - pa.z;
             float s = Dx * Dx + Dy * Dy + Dz * Dz;
-            const bool valid = force_valid(P, listed, (uint32_t)a << 4, o, pa.x, pa.y, pa.z, nb.p, s);
+            const bool valid = force_valid(P, true, (uint32_t)a << 4, o, pa.x, pa.y, pa.z, nb.p, s);
             s = __builtin_elementwise_maximum(s, 1.17549435e-38f);
             // padding slot (the particle itself, D = 0): evaluate the kernels at
             // r = h/2 so every term stays finite before its zero weight applies

@@ -407,12 +407,13 @@ __device__ __forceinline__ T at(const T* __restrict__ base, uint32_t o) {
 // nothing); flush() after every drain. self_j is a valid index: every list
 // slot starts as its offset, and every slot written later holds a valid
 // candidate, so drain slots beyond a lane's count read valid records. With
-// PAD those slots are replaced by self_j itself: a proc that multiplies its
-// zero weights into the neighbour's data needs finite data there (a stale
-// slot may name a NaN particle); a proc that selects its terms does not. Overflow-segment candidates
-// additionally need reference key adjacency. Appends are branch-free: every
-// candidate is written at the list head and the head advances only when the
-// filter passes.
+// PAD those slots hold self_j itself: a proc that multiplies its zero weights
+// into the neighbour's data needs finite data there (a stale slot may name a
+// NaN particle); a proc that selects its terms does not. Overflow-segment
+// candidates additionally need reference key adjacency. Without PAD appends
+// are branch-free: every candidate is written at the list head and the head
+// advances only when the filter passes; with PAD only passing candidates are
+// written and each drained slot is reset to self_j.
 // Which waves a gather launch serves: every wave, only waves whose active
 // lanes all sit in one dense cell (uniform scan), or only the others (per-lane
 // scan). The forward passes launch the last two separately, so the uniform
@@ -605,13 +606,13 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
         do {
             if (W == 4) {
                 const bool v0 = k < cnt, v1 = k + 1 < cnt, v2 = k + 2 < cnt, v3 = k + 3 < cnt;
-                uint32_t o0 = lst[k][tid], o1 = lst[k + 1][tid], o2 = lst[k + 2][tid],
-                         o3 = lst[k + 3][tid];
+                const uint32_t o0 = lst[k][tid], o1 = lst[k + 1][tid], o2 = lst[k + 2][tid],
+                               o3 = lst[k + 3][tid];
                 if (PAD) {
-                    o0 = v0 ? o0 : self_o;
-                    o1 = v1 ? o1 : self_o;
-                    o2 = v2 ? o2 : self_o;
-                    o3 = v3 ? o3 : self_o;
+                    lst[k][tid] = self_o;
+                    lst[k + 1][tid] = self_o;
+                    lst[k + 2][tid] = self_o;
+                    lst[k + 3][tid] = self_o;
                 }
                 auto t0 = load(o0);
                 auto t1 = load(o1);
@@ -623,10 +624,10 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 proc(o3, t3, v3);
             } else if (W == 2) {
                 const bool v0 = k < cnt, v1 = k + 1 < cnt;
-                uint32_t o0 = lst[k][tid], o1 = lst[k + 1][tid];
+                const uint32_t o0 = lst[k][tid], o1 = lst[k + 1][tid];
                 if (PAD) {
-                    o0 = v0 ? o0 : self_o;
-                    o1 = v1 ? o1 : self_o;
+                    lst[k][tid] = self_o;
+                    lst[k + 1][tid] = self_o;
                 }
                 auto t0 = load(o0);
                 auto t1 = load(o1);
@@ -634,8 +635,8 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 proc(o1, t1, v1);
             } else {
                 const bool v0 = k < cnt;
-                uint32_t o0 = lst[k][tid];
-                if (PAD) o0 = v0 ? o0 : self_o;
+                const uint32_t o0 = lst[k][tid];
+                if (PAD) lst[k][tid] = self_o;
                 auto t0 = load(o0);
                 proc(o0, t0, v0);
             }
@@ -644,10 +645,13 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
         cnt = 0;
         flush();
     };
-    // j must be a valid index even when ok is false (it stays in the list
-    // until overwritten, and drains read every slot)
+    // Without PAD the slot is written whatever ok says (j must then be a
+    // valid index: it stays in the list until overwritten, and drains read
+    // every slot). With PAD only passing candidates are written, and the
+    // drain puts the lane's own offset back into every slot it reads, so a
+    // slot at or past the count always names the lane itself.
     auto push = [&](uint32_t j, bool ok) {
-        lst[cnt][tid] = j << 4;
+        if (!PAD || ok) lst[cnt][tid] = j << 4;
         cnt += ok ? 1 : 0;
     };
     auto ov_ok = [&](const float4& pj) {

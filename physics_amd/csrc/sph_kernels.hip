@@ -609,8 +609,10 @@ __global__ __launch_bounds__(256) void k_force(
         act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), cl, start, posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_maybe(P, pi.x, pi.y, pi.z, pj); },
         [&](uint32_t o) { return NbrData{at(posm, o), at(velm, o), at(mp, o)}; },
-        [&](uint32_t o, const NbrData& nb, bool listed) {
-            pair_force<SHEAR>(A, P, (uint32_t)i << 4, o, pi, vi, Pi, nb, listed);
+        // every slot at or past the lane's count names the lane itself, which
+        // no validity test accepts, so the listed flag is not needed
+        [&](uint32_t o, const NbrData& nb, bool) {
+            pair_force<SHEAR>(A, P, (uint32_t)i << 4, o, pi, vi, Pi, nb, true);
         },
         [&]() { A.merge(); });
     if (!act) return;
