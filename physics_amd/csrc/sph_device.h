@@ -590,15 +590,21 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                                        Flush&& flush) {
     static_assert(S == 1 || S == 2 || S == 4, "drain width");
     static_assert(L % S == 0, "whole drain rounds per list");
-    int cnt = 0;
+    // the lane's list head as a 32-bit byte offset into the list array: it
+    // advances one row (1024 bytes) per append, so an append is one select
+    // and one add; the count is derived from it where needed
+    char* const lbase = reinterpret_cast<char*>(&lst[0][0]);
+    const uint32_t lp0 = (uint32_t)tid * 4u;
+    uint32_t lp = lp0;
+    auto count = [&]() { return (int)((lp - lp0) >> 10); };
     const uint32_t self_o = self_j << 4;
     for (int k = 0; k < L; ++k) lst[k][tid] = self_o;
     auto drain = [&](auto width) {
         constexpr int W = decltype(width)::value;
+        const int cnt = count();
         // do-while: the loop test sits at the bottom, so the accumulators
         // stay in place across iterations
         if (!wave_any(0 < cnt)) {
-            cnt = 0;
             flush();
             return;
         }
@@ -642,7 +648,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
             }
             k += W;
         } while (wave_any(k < cnt));
-        cnt = 0;
+        lp = lp0;
         flush();
     };
     // Without PAD the slot is written whatever ok says (j must then be a
@@ -651,9 +657,13 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
     // drain puts the lane's own offset back into every slot it reads, so a
     // slot at or past the count always names the lane itself.
     auto push = [&](uint32_t j, bool ok) {
-        if (!PAD || ok) lst[cnt][tid] = j << 4;
-        cnt += ok ? 1 : 0;
+        if (!PAD || ok) *reinterpret_cast<uint32_t*>(lbase + lp) = j << 4;
+        lp += ok ? 1024u : 0u;
     };
+    // list-full tests: at least L - 8 + 1, L - 4 + 1 or L entries
+    const uint32_t lp8 = lp0 + (L - 8) * 1024u;
+    const uint32_t lp4 = lp0 + (L - 4) * 1024u;
+    const uint32_t lpL = lp0 + L * 1024u;
     auto ov_ok = [&](const float4& pj) {
         return keys_adjacent(xi, yi, zi, pj.x, pj.y, pj.z, inv);
     };
@@ -714,7 +724,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 for (uint32_t u = 0; u < 4; ++u) push(hi - 4u + u, maybe(hi - 4u + u, qb.p[u]) & act);
                 lo += 4u;
                 hi -= 4u;
-                if (wave_any(cnt > L - 8)) drain(DrainWidth<S>{});
+                if (wave_any(lp > lp8)) drain(DrainWidth<S>{});
             }
             if (lo < hi) {
                 // the last 1..7 candidates: one round from lo (the arrays have
@@ -730,14 +740,14 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 for (uint32_t u = 0; u < 4; ++u)
                     push(min(lo + 4u + u, last),
                          maybe(lo + 4u + u, qb.p[u]) & (act & (lo + 4u + u <= last)));
-                if (wave_any(cnt > L - 8)) drain(DrainWidth<S>{});
+                if (wave_any(lp > lp8)) drain(DrainWidth<S>{});
             }
         }
         const uint32_t o0 = uni(start[d.ncells]), o1 = uni(start[d.ncells + 1]);
         for (uint32_t j = o0; j < o1; ++j) {
             const float4 pj = posm[j];
             push(j, act && ov_ok(pj) && test(j, pj));
-            if (wave_any(cnt == L)) drain(DrainWidth<S>{});
+            if (wave_any(lp == lpL)) drain(DrainWidth<S>{});
         }
     } else if (MODE != kUniformWaves) {
         Cursor cur;
@@ -764,7 +774,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 cur.j = min(j + 4, cur.jend);
                 if (cur.j >= cur.jend) cur.next_run(d, start);
             }
-            if (wave_any(cnt > L - 4)) drain(DrainWidth<(S < 2 ? S : 2)>{});
+            if (wave_any(lp > lp4)) drain(DrainWidth<(S < 2 ? S : 2)>{});
         }
     }
     if (uniform) drain(DrainWidth<S>{});
