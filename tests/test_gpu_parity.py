@@ -132,6 +132,39 @@ def test_overflow_and_nan_particles_match_oracle():
         assert P.normwise(np.where(fin, a, 0), np.where(fin, b, 0)) <= P.TOL_STEP, f
 
 
+def test_crowded_cell_matches_oracle():
+    # 2,600 particles in a cube of side h/8 inside one half cell: the cell holds
+    # more members than the reorder stages in LDS (2048 slots), so their
+    # within-cell ranks are counted from global memory
+    h = float(scenes.H32)
+    rng = np.random.default_rng(11)
+    pos = (0.1 * h + rng.random((2600, 3)) * (h / 8)).astype(np.float32)
+    g, o = P.make_pair(pos, mass=0.01)
+    x0 = P.step_both(g, o)
+    err = P.compare_step(g, o, x0)
+    _print("crowded cell", err)
+    for k, v in err.items():
+        assert v <= P.TOL_STEP, (k, v)
+
+
+def test_large_overflow_segment_matches_oracle():
+    # 2,100 particles far outside the bounds box (the table is clipped to the
+    # box, so they form the overflow segment) beside a dense block: the
+    # segment is longer than the reorder's LDS window
+    blk = _block(8, 6, jit=0.1, origin=(-0.4, -0.4, -0.4)).astype(np.float32)
+    far = np.zeros((2100, 3), np.float32)
+    far[:, 0] = 3.0e6 + np.arange(2100, dtype=np.float32)
+    far[:, 1] = 0.01
+    far[:, 2] = 0.02
+    pos = np.concatenate([blk, far[::-1]])
+    g, o = P.make_pair(pos, mass=1.2, bounds=((-5, -5, -5), (5, 5, 5)))
+    x0 = P.step_both(g, o)
+    err = P.compare_step(g, o, x0)
+    _print("overflow segment", err)
+    for k, v in err.items():
+        assert v <= P.TOL_STEP, (k, v)
+
+
 def test_coincident_single_and_empty():
     g, o = P.make_pair(np.array([[0.1, 0.2, 0.3], [0.1, 0.2, 0.3], [0.15, 0.2, 0.3]], np.float32),
                        mass=2.0)
