@@ -537,7 +537,7 @@ void DeviceFluid::set_graph(bool on) {
 }
 
 void DeviceFluid::set_tuning(const Tuning& t) {
-    if (!tuning_valid(t)) throw std::invalid_argument("tuning: list lengths must be 16, 32 or 64");
+    if (!tuning_valid(t)) throw std::invalid_argument("tuning: list lengths must be 8, 16 or 32 and split 0 or 1");
     tuning_ = t;
     graph_dirty_ = true;
 }
