@@ -309,7 +309,9 @@ struct Cursor {
 
     __device__ __forceinline__ bool in_overflow() const { return row > nrows; }
 
-    __device__ __forceinline__ void run(int k, const Domain& d, const uint32_t* __restrict__ start) {
+    // Bounds [ja, jb) of row k; a row past the window (k = nrows) is empty.
+    __device__ __forceinline__ void run(int k, const Domain& d, const uint32_t* __restrict__ start,
+                                        uint32_t& ja, uint32_t& jb) const {
         // k / step for k < 2^14 by multiply and shift (step 5: 0xCCCD, 6: 0xAAAB)
         const uint32_t q = ((uint32_t)k * mul) >> 18;
         const uint32_t step = mul == 0xAAABu ? 6u : 5u;
@@ -319,18 +321,20 @@ struct Cursor {
         // select of addresses would put a zero on the stack)
         const uint32_t base = ok ? (x * (uint32_t)d.dims[1] + y) * (uint32_t)d.dims[2] : 0u;
         const uint32_t sa = start[base + (ok ? z0 : 0u)], sb = start[base + (ok ? z1 + 1u : 0u)];
-        j = ok ? sa : 0u;
-        jend = ok ? sb : 0u;
+        ja = ok ? sa : 0u;
+        jb = ok ? sb : 0u;
     }
 
+    // Advances to the next non-empty run: the window's rows, then the
+    // overflow segment. Branch-free per iteration (row nrows reads as empty
+    // and is replaced by the overflow bounds).
     __device__ __forceinline__ void next_run(const Domain& d, const uint32_t* __restrict__ start) {
         while (j >= jend && row <= nrows) {
-            if (row < nrows) {
-                run(row, d, start);
-            } else {
-                j = o0;
-                jend = o1;
-            }
+            uint32_t ja, jb;
+            run(row, d, start, ja, jb);
+            const bool ovr = row == nrows;
+            j = ovr ? o0 : ja;
+            jend = ovr ? o1 : jb;
             ++row;
         }
     }
@@ -761,11 +765,18 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 const uint32_t j1 = min(j + 1, last), j2 = min(j + 2, last), j3 = min(j + 3, last);
                 const float4 p0 = ldg(posm, j), p1 = ldg(posm, j1), p2 = ldg(posm, j2),
                              p3 = ldg(posm, j3);
-                const bool ov = cur.in_overflow();
-                const bool t0 = (!ov || ov_ok(p0)) && test(j, p0);
-                const bool t1 = (j + 1 <= last) && (!ov || ov_ok(p1)) && test(j1, p1);
-                const bool t2 = (j + 2 <= last) && (!ov || ov_ok(p2)) && test(j2, p2);
-                const bool t3 = (j + 3 <= last) && (!ov || ov_ok(p3)) && test(j3, p3);
+                // the distance filters without branches; the reference key
+                // adjacency only where the lane walks the overflow segment
+                bool t0 = test(j, p0);
+                bool t1 = (j + 1 <= last) & test(j1, p1);
+                bool t2 = (j + 2 <= last) & test(j2, p2);
+                bool t3 = (j + 3 <= last) & test(j3, p3);
+                if (cur.in_overflow()) {
+                    t0 = t0 && ov_ok(p0);
+                    t1 = t1 && ov_ok(p1);
+                    t2 = t2 && ov_ok(p2);
+                    t3 = t3 && ov_ok(p3);
+                }
                 // cnt <= L - 4 here, so the four writes stay inside the list
                 push(j, t0);
                 push(j1, t1);
