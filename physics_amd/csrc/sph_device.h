@@ -409,8 +409,9 @@ __device__ __forceinline__ T at(const T* __restrict__ base, uint32_t o) {
 // take the byte offset o = 16 j of candidate j (proc accumulates; a slot that
 // is not listed, or that proc's exact test rejects, must contribute exactly
 // nothing); flush() after every drain. self_j is a valid index: every list
-// slot starts as its offset, and every slot written later holds a valid
-// candidate, so drain slots beyond a lane's count read valid records. With
+// slot starts as its offset, and every slot written later holds a candidate
+// or an entry at most seven past a run's end (inside the array's kScanPad
+// spare entries), so drain slots beyond a lane's count read in-bounds records. With
 // PAD those slots hold self_j itself: a proc that multiplies its zero weights
 // into the neighbour's data needs finite data there (a stale slot may name a
 // NaN particle); a proc that selects its terms does not. Overflow-segment
@@ -732,18 +733,17 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
             }
             if (lo < hi) {
                 // the last 1..7 candidates: one round from lo (the arrays have
-                // kScanPad spare entries); a candidate past the run is written
-                // as the run's last one (a valid index) with a false flag
+                // kScanPad spare entries, so every index of the round is in
+                // bounds); a candidate past the run gets a false flag
                 const uint32_t last = hi - 1u;
                 const Quad4 qa = *reinterpret_cast<const Quad4*>(src + lo);
                 const Quad4 qb = *reinterpret_cast<const Quad4*>(src + lo + 4u);
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u)
-                    push(min(lo + u, last), maybe(lo + u, qa.p[u]) & (act & (lo + u <= last)));
+                    push(lo + u, maybe(lo + u, qa.p[u]) & (act & (lo + u <= last)));
 #pragma unroll
                 for (uint32_t u = 0; u < 4; ++u)
-                    push(min(lo + 4u + u, last),
-                         maybe(lo + 4u + u, qb.p[u]) & (act & (lo + 4u + u <= last)));
+                    push(lo + 4u + u, maybe(lo + 4u + u, qb.p[u]) & (act & (lo + 4u + u <= last)));
                 if (wave_any(lp > lp8)) drain(DrainWidth<S>{});
             }
         }
@@ -762,15 +762,22 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
         while (wave_any(act && cur.j < cur.jend)) {
             if (act && cur.j < cur.jend) {
                 const uint32_t j = cur.j, last = cur.jend - 1;
-                const uint32_t j1 = min(j + 1, last), j2 = min(j + 2, last), j3 = min(j + 3, last);
-                const float4 p0 = ldg(posm, j), p1 = ldg(posm, j1), p2 = ldg(posm, j2),
-                             p3 = ldg(posm, j3);
+                const uint32_t j1 = j + 1, j2 = j + 2, j3 = j + 3;
+                // four consecutive entries from one base offset: j + 3 <= last
+                // + 3 stays inside the array (kScanPad spare entries), and an
+                // entry past the run gets a false flag (an append without PAD
+                // writes it unlisted, and the drain selects its term away)
+                const char* pb = reinterpret_cast<const char*>(posm) + (size_t)(j << 4);
+                const float4 p0 = *reinterpret_cast<const float4*>(pb);
+                const float4 p1 = *reinterpret_cast<const float4*>(pb + 16);
+                const float4 p2 = *reinterpret_cast<const float4*>(pb + 32);
+                const float4 p3 = *reinterpret_cast<const float4*>(pb + 48);
                 // the distance filters without branches; the reference key
                 // adjacency only where the lane walks the overflow segment
                 bool t0 = test(j, p0);
-                bool t1 = (j + 1 <= last) & test(j1, p1);
-                bool t2 = (j + 2 <= last) & test(j2, p2);
-                bool t3 = (j + 3 <= last) & test(j3, p3);
+                bool t1 = (j1 <= last) & test(j1, p1);
+                bool t2 = (j2 <= last) & test(j2, p2);
+                bool t3 = (j3 <= last) & test(j3, p3);
                 if (cur.in_overflow()) {
                     t0 = t0 && ov_ok(p0);
                     t1 = t1 && ov_ok(p1);
