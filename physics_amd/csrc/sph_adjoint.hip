@@ -11,7 +11,9 @@
 //                  cotangent s_bar, the local rho_bar term, and the power-law
 //                  viscosity backwards (dL/d visc_k, dL/d visc_n per material,
 //                  the shear-rate cotangent).
-//   k_adj_force    gather over the force neighbours: every pair term of the
+//   k_adj_force_newton / k_adj_force_shear (one body, without and with the
+//                  shear-rate terms of power-law materials)
+//                  gather over the force neighbours: every pair term of the
 //                  pressure, viscosity (each particle's own mu), XSPH and
 //                  shear-rate sums, differentiated with respect to the
 //                  particle's own position, velocity, P = p / rho^2 and rho,
@@ -141,16 +143,11 @@ struct ForceAdjAcc {
 };
 
 template <int L, bool SHEAR>
-__global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
-                                                   const float4* __restrict__ Fb,
-                                                   const float4* __restrict__ Sb,
-                                                   const float4* __restrict__ xb_loc,
-                                                   const float4* __restrict__ vb_loc,
-                                                   const float* __restrict__ rhob_loc,
-                                                   const float2* __restrict__ mush,
-                                                   const float* __restrict__ rhob_seed,
-                                                   float* rhob_out, float4* xb_acc, float4* vbar_out,
-                                                   double* part) {
+__device__ __forceinline__ void adj_force_body(
+    SceneTable T, AdjSlot S, const float4* __restrict__ Fb, const float4* __restrict__ Sb,
+    const float4* __restrict__ xb_loc, const float4* __restrict__ vb_loc, const float* __restrict__ rhob_loc,
+    const float2* __restrict__ mush, const float* __restrict__ rhob_seed, float* rhob_out, float4* xb_acc,
+    float4* vbar_out, double* part) {
     __shared__ uint32_t lst[L][256];
     __shared__ double sh[4];
     const int tid = threadIdx.x;
@@ -175,7 +172,7 @@ __global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
     const float mua = msa.x, shba = msa.y;
     const float h = P.h, h2 = P.h2;
     ForceAdjAcc<SHEAR> A = {};
-    gather<L, kAllWaves, 2, false, true>(
+    gather<L, kAllWaves, 2, false, true, 4>(
         act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, zwin(P, d), ColLocal{}, start, S.posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_maybe(P, pa.x, pa.y, pa.z, pj); },
         [&](uint32_t o) {
@@ -297,6 +294,28 @@ __global__ __launch_bounds__(256) void k_adj_force(SceneTable T, AdjSlot S,
     if (tid == 0) part[(size_t)TL.t * kAdjParams] = kb;
 }
 
+// Newtonian materials: held to four waves per SIMD (128 VGPRs, one value
+// spilled), which the unconstrained build misses by one register; the
+// shear-rate terms of power-law materials would spill far more, so that
+// kernel keeps three waves.
+template <int L>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_force_newton(
+    SceneTable T, AdjSlot S, const float4* __restrict__ Fb, const float4* __restrict__ Sb,
+    const float4* __restrict__ xb_loc, const float4* __restrict__ vb_loc, const float* __restrict__ rhob_loc,
+    const float2* __restrict__ mush, const float* __restrict__ rhob_seed, float* rhob_out, float4* xb_acc,
+    float4* vbar_out, double* part) {
+    adj_force_body<L, false>(T, S, Fb, Sb, xb_loc, vb_loc, rhob_loc, mush, rhob_seed, rhob_out, xb_acc, vbar_out, part);
+}
+
+template <int L>
+__global__ __launch_bounds__(256) void k_adj_force_shear(
+    SceneTable T, AdjSlot S, const float4* __restrict__ Fb, const float4* __restrict__ Sb,
+    const float4* __restrict__ xb_loc, const float4* __restrict__ vb_loc, const float* __restrict__ rhob_loc,
+    const float2* __restrict__ mush, const float* __restrict__ rhob_seed, float* rhob_out, float4* xb_acc,
+    float4* vbar_out, double* part) {
+    adj_force_body<L, true>(T, S, Fb, Sb, xb_loc, vb_loc, rhob_loc, mush, rhob_seed, rhob_out, xb_acc, vbar_out, part);
+}
+
 template <int L>
 __global__ __launch_bounds__(256) void k_adj_density(SceneTable T, AdjSlot S,
                                                      const float* __restrict__ rhob,
@@ -321,7 +340,7 @@ __global__ __launch_bounds__(256) void k_adj_density(SceneTable T, AdjSlot S,
         float4 p;
         float rb;
     };
-    gather<L, kAllWaves, 2, false, true>(
+    gather<L, kAllWaves, 2, false, true, 4>(
         act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, zwin(P, d), ColLocal{}, start, S.posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return density_maybe(P, pa.x, pa.y, pa.z, pj); },
         [&](uint32_t o) { return DN{at(S.posm, o), at(rhob, o)}; },
@@ -379,11 +398,11 @@ void launch_adjoint_step(hipStream_t st, const SceneTable& T, const AdjSlot& S, 
         hipLaunchKernelGGL(k_adj_local, dim3(nb), dim3(256), 0, st, T, S, B.xbar_in, B.vbar_in, B.Fb,
                            B.Sb, B.xb_loc, B.vb_loc, B.rhob_loc, B.mush, B.part);
         if (shear)
-            hipLaunchKernelGGL((k_adj_force<kAdjList, true>), dim3(nb), dim3(256), 0, st, T, S, B.Fb,
+            hipLaunchKernelGGL((k_adj_force_shear<kAdjList>), dim3(nb), dim3(256), 0, st, T, S, B.Fb,
                                B.Sb, B.xb_loc, B.vb_loc, B.rhob_loc, B.mush, rhob_seed, B.rhob, B.xb_acc,
                                B.vbar_out, B.part);
         else
-            hipLaunchKernelGGL((k_adj_force<kAdjList, false>), dim3(nb), dim3(256), 0, st, T, S, B.Fb,
+            hipLaunchKernelGGL((k_adj_force_newton<kAdjList>), dim3(nb), dim3(256), 0, st, T, S, B.Fb,
                                B.Sb, B.xb_loc, B.vb_loc, B.rhob_loc, B.mush, rhob_seed, B.rhob, B.xb_acc,
                                B.vbar_out, B.part);
         hipLaunchKernelGGL(k_adj_density<kAdjList>, dim3(nb), dim3(256), 0, st, T, S, B.rhob,

@@ -584,7 +584,7 @@ struct DrainWidth {
 // zt: the z half-widths of the uniform scan's trimmed rows. With LOC the
 // uniform scan filters with the column-local coordinates of cl instead of
 // test (which still filters the overflow segment and the per-lane scan).
-template <int L, int MODE, int S, bool LOC, bool PAD, class Test, class Load, class Proc,
+template <int L, int MODE, int S, bool LOC, bool PAD, int R, class Test, class Load, class Proc,
           class Flush>
 __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi, float zi,
                                        uint32_t self_j, const Domain& d, double inv,
@@ -595,6 +595,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                                        Flush&& flush) {
     static_assert(S == 1 || S == 2 || S == 4, "drain width");
     static_assert(L % S == 0, "whole drain rounds per list");
+    static_assert(R >= 1 && R <= 8 && R <= L, "per-lane round width");
     // the lane's list head as a 32-bit byte offset into the list array: it
     // advances one row (1024 bytes) per append, so an append is one select
     // and one add; the count is derived from it where needed
@@ -667,7 +668,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
     };
     // list-full tests: at least L - 8 + 1, L - 4 + 1 or L entries
     const uint32_t lp8 = lp0 + (L - 8) * 1024u;
-    const uint32_t lp4 = lp0 + (L - 4) * 1024u;
+    const uint32_t lpR = lp0 + (L - R) * 1024u;
     const uint32_t lpL = lp0 + L * 1024u;
     auto ov_ok = [&](const float4& pj) {
         return keys_adjacent(xi, yi, zi, pj.x, pj.y, pj.z, inv);
@@ -756,43 +757,37 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
     } else if (MODE != kUniformWaves) {
         Cursor cur;
         cur.init(act, ci, xi, yi, zi, d, inv, start);
-        // up to four consecutive candidates of the lane's current run per
-        // round: four independent loads in flight instead of one dependent
-        // load per candidate
+        // up to R consecutive candidates of the lane's current run per round:
+        // R independent loads in flight instead of one dependent load per
+        // candidate
         while (wave_any(act && cur.j < cur.jend)) {
             if (act && cur.j < cur.jend) {
                 const uint32_t j = cur.j, last = cur.jend - 1;
-                const uint32_t j1 = j + 1, j2 = j + 2, j3 = j + 3;
-                // four consecutive entries from one base offset: j + 3 <= last
-                // + 3 stays inside the array (kScanPad spare entries), and an
-                // entry past the run gets a false flag (an append without PAD
-                // writes it unlisted, and the drain selects its term away)
+                // R consecutive entries from one base offset: j + R - 1 <=
+                // last + 7 stays inside the array (kScanPad spare entries),
+                // and an entry past the run gets a false flag (an append
+                // without PAD writes it unlisted, and the drain selects its
+                // term away)
                 const char* pb = reinterpret_cast<const char*>(posm) + (size_t)(j << 4);
-                const float4 p0 = *reinterpret_cast<const float4*>(pb);
-                const float4 p1 = *reinterpret_cast<const float4*>(pb + 16);
-                const float4 p2 = *reinterpret_cast<const float4*>(pb + 32);
-                const float4 p3 = *reinterpret_cast<const float4*>(pb + 48);
+                float4 pq[R];
+#pragma unroll
+                for (int u = 0; u < R; ++u) pq[u] = *reinterpret_cast<const float4*>(pb + 16 * u);
                 // the distance filters without branches; the reference key
                 // adjacency only where the lane walks the overflow segment
-                bool t0 = test(j, p0);
-                bool t1 = (j1 <= last) & test(j1, p1);
-                bool t2 = (j2 <= last) & test(j2, p2);
-                bool t3 = (j3 <= last) & test(j3, p3);
+                bool t[R];
+#pragma unroll
+                for (int u = 0; u < R; ++u) t[u] = (j + (uint32_t)u <= last) & test(j + u, pq[u]);
                 if (cur.in_overflow()) {
-                    t0 = t0 && ov_ok(p0);
-                    t1 = t1 && ov_ok(p1);
-                    t2 = t2 && ov_ok(p2);
-                    t3 = t3 && ov_ok(p3);
+#pragma unroll
+                    for (int u = 0; u < R; ++u) t[u] = t[u] && ov_ok(pq[u]);
                 }
-                // cnt <= L - 4 here, so the four writes stay inside the list
-                push(j, t0);
-                push(j1, t1);
-                push(j2, t2);
-                push(j3, t3);
-                cur.j = min(j + 4, cur.jend);
+                // the count is at most L - R here, so the writes stay inside the list
+#pragma unroll
+                for (int u = 0; u < R; ++u) push(j + u, t[u]);
+                cur.j = min(j + (uint32_t)R, cur.jend);
                 if (cur.j >= cur.jend) cur.next_run(d, start);
             }
-            if (wave_any(lp > lp4)) drain(DrainWidth<(S < 2 ? S : 2)>{});
+            if (wave_any(lp > lpR)) drain(DrainWidth<(S < 2 ? S : 2)>{});
         }
     }
     if (uniform) drain(DrainWidth<S>{});

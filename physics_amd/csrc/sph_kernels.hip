@@ -474,8 +474,9 @@ __global__ __launch_bounds__(256) void k_density(SceneTable T, const float4* __r
     const double xi = pi.x, yi = pi.y, zi = pi.z;
     double sum = 0.0;
     // four neighbour loads in flight: the register cost keeps this kernel's
-    // occupancy (force keeps two: four would cost it a wave per SIMD)
-    gather<L, MODE, (MODE == kUniformWaves ? 1 : 4), LOC, false>(
+    // occupancy (force keeps two: four would cost it a wave per SIMD); the
+    // per-lane scan takes four candidates per round (six would cost a wave too)
+    gather<L, MODE, (MODE == kUniformWaves ? 1 : 4), LOC, false, 4>(
         act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), cl, start, posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return density_maybe(P, pi.x, pi.y, pi.z, pj); },
         [&](uint32_t o) { return at(posm, o); },
@@ -605,7 +606,9 @@ __global__ __launch_bounds__(256) void k_force(
     if (LOC && act) cl.li = posl[i];
     ForceAcc<SHEAR> A;
     A.zero();
-    gather<L, MODE, (MODE == kUniformWaves ? 1 : 2), LOC, true>(
+    // per-lane rounds of six candidates: a row of five half cells at rest
+    // spacing holds about five, so a row takes one round instead of two
+    gather<L, MODE, (MODE == kUniformWaves ? 1 : 2), LOC, true, 6>(
         act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), cl, start, posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_maybe(P, pi.x, pi.y, pi.z, pj); },
         [&](uint32_t o) { return NbrData{at(posm, o), at(velm, o), at(mp, o)}; },
