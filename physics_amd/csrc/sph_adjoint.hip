@@ -172,7 +172,8 @@ __device__ __forceinline__ void adj_force_body(
     const float mua = msa.x, shba = msa.y;
     const float h = P.h, h2 = P.h2;
     ForceAdjAcc<SHEAR> A = {};
-    gather<L, kAllWaves, 2, false, true, 4>(
+    // per-lane rounds of six candidates, as in the forward force pass
+    gather<L, kAllWaves, 2, false, true, 6>(
         act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, zwin(P, d), ColLocal{}, start, S.posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_maybe(P, pa.x, pa.y, pa.z, pj); },
         [&](uint32_t o) {
@@ -340,7 +341,7 @@ __global__ __launch_bounds__(256) void k_adj_density(SceneTable T, AdjSlot S,
         float4 p;
         float rb;
     };
-    gather<L, kAllWaves, 2, false, true, 4>(
+    gather<L, kAllWaves, 2, false, true, 6>(
         act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, zwin(P, d), ColLocal{}, start, S.posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return density_maybe(P, pa.x, pa.y, pa.z, pj); },
         [&](uint32_t o) { return DN{at(S.posm, o), at(rhob, o)}; },
