@@ -415,10 +415,10 @@ __device__ __forceinline__ T at(const T* __restrict__ base, uint32_t o) {
 // PAD those slots hold self_j itself: a proc that multiplies its zero weights
 // into the neighbour's data needs finite data there (a stale slot may name a
 // NaN particle); a proc that selects its terms does not. Overflow-segment
-// candidates additionally need reference key adjacency. Without PAD appends
-// are branch-free: every candidate is written at the list head and the head
-// advances only when the filter passes; with PAD only passing candidates are
-// written and each drained slot is reset to self_j.
+// candidates additionally need reference key adjacency. Appends are
+// branch-free: every candidate writes the list head (with PAD a failing one
+// writes self_j's offset) and the head advances only when the filter passes;
+// with PAD each drained slot is reset to self_j.
 // Which waves a gather launch serves: every wave, only waves whose active
 // lanes all sit in one dense cell (uniform scan), or only the others (per-lane
 // scan). The forward passes launch the last two separately, so the uniform
@@ -657,13 +657,14 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
         lp = lp0;
         flush();
     };
-    // Without PAD the slot is written whatever ok says (j must then be a
-    // valid index: it stays in the list until overwritten, and drains read
-    // every slot). With PAD only passing candidates are written, and the
-    // drain puts the lane's own offset back into every slot it reads, so a
-    // slot at or past the count always names the lane itself.
+    // The slot at the head is written whatever ok says, and the head
+    // advances only when ok (no branch). Without PAD it gets j (which must
+    // then be a valid index: it stays in the list until overwritten, and
+    // drains read every slot). With PAD a failing candidate writes the lane's
+    // own offset instead, and the drain puts that offset back into every
+    // slot it reads, so a slot at or past the count always names the lane.
     auto push = [&](uint32_t j, bool ok) {
-        if (!PAD || ok) *reinterpret_cast<uint32_t*>(lbase + lp) = j << 4;
+        *reinterpret_cast<uint32_t*>(lbase + lp) = (PAD && !ok) ? self_o : j << 4;
         lp += ok ? 1024u : 0u;
     };
     // list-full tests: at least L - 8 + 1, L - 4 + 1 or L entries
