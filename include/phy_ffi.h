@@ -273,6 +273,16 @@ int32_t phy_fluid_diagnostics_f64(PhyWorldHandle *w, double *out5);
 int32_t phy_fluid_neighbor_stats(PhyWorldHandle *w, uint32_t *counts, uint64_t *sums,
                                  uint64_t *xors, uintptr_t len);
 
+/* Neighbour statistics, in the layout of phy_fluid_neighbor_stats, taken by
+ * the step's own gathers on the current state with the current tuning: pass 0
+ * the density pass (j with r < h, self included; world.rs:220-230), pass 1 the
+ * force pass (j != i with 0 < r < h, a NaN distance kept; world.rs:292-299).
+ * These are the sets the density and force sums run over, for exact
+ * comparison with the reference's. rho / p / aux of the last step are kept.
+ * 0 ok, -1 error. */
+int32_t phy_fluid_gather_stats(PhyWorldHandle *w, int32_t pass, uint32_t *counts, uint64_t *sums,
+                               uint64_t *xors, uintptr_t len);
+
 /* n steps of dt in one call (same result as n phy_world_step calls). */
 int32_t phy_world_step_n(PhyWorldHandle *w, double dt, uint32_t n);
 
@@ -427,6 +437,10 @@ int32_t phy_fluid_batch_diagnostics_f64(PhyFluidBatch *b, double *out);
 /* As phy_fluid_neighbor_stats, concatenated; neighbour indices are scene-local. */
 int32_t phy_fluid_batch_neighbor_stats(PhyFluidBatch *b, uint32_t *counts, uint64_t *sums,
                                        uint64_t *xors, uintptr_t len);
+
+/* As phy_fluid_gather_stats, concatenated; neighbour indices are scene-local. */
+int32_t phy_fluid_batch_gather_stats(PhyFluidBatch *b, int32_t pass, uint32_t *counts, uint64_t *sums,
+                                     uint64_t *xors, uintptr_t len);
 
 /* Adjoint tape over every scene (as phy_fluid_tape_*, FP32 cotangents in
  * concatenated order); dparams: 2 per scene (dL/d stiffness, dL/d visc_k[0]). */

@@ -27,6 +27,12 @@ MUTATIONS = {
     "bounds_no_damp": 6,
     "poly6_h2": 7,
     "dt_in_xsph": 8,
+    "shear_no_reps": 9,
+    "shear_mass_i": 10,
+    "power_exp_n": 11,
+    "shear_no_floor": 12,
+    "shear_dv_sq": 13,
+    "mat_oor_last": 14,
 }
 
 
@@ -122,6 +128,8 @@ def lib():
     L.or_world_neighbors.argtypes = [vp, sz, C.POINTER(C.c_uint64), sz]
     L.or_world_neighbors.restype = sz
     L.or_world_neighbor_stats.argtypes = [
+        vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.or_world_force_neighbor_stats.argtypes = [
         vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.or_world_time_steps.argtypes = [vp, C.c_double, C.c_int]
     L.or_world_time_steps.restype = C.c_double
@@ -276,6 +284,18 @@ class World:
         s = np.zeros(n, dtype=np.uint64)
         x = np.zeros(n, dtype=np.uint64)
         lib().or_world_neighbor_stats(
+            self._h, c.ctypes.data_as(C.POINTER(C.c_uint32)),
+            s.ctypes.data_as(C.POINTER(C.c_uint64)), x.ctypes.data_as(C.POINTER(C.c_uint64)))
+        return c, s, x
+
+    def force_neighbor_stats(self):
+        """Count / index sum / hashed-index xor of each particle's force-pass
+        neighbours (j != i, 0 < r < h or r NaN; world.rs:292-299)."""
+        n = len(self)
+        c = np.zeros(n, dtype=np.uint32)
+        s = np.zeros(n, dtype=np.uint64)
+        x = np.zeros(n, dtype=np.uint64)
+        lib().or_world_force_neighbor_stats(
             self._h, c.ctypes.data_as(C.POINTER(C.c_uint32)),
             s.ctypes.data_as(C.POINTER(C.c_uint64)), x.ctypes.data_as(C.POINTER(C.c_uint64)))
         return c, s, x

@@ -71,6 +71,13 @@ enum Mutation : int {
     MUT_BOUNDS_NO_DAMP = 6,   // reflect without the damping factor
     MUT_POLY6_H2 = 7,         // poly6 with (h - r) instead of (h^2 - r^2)
     MUT_DT_IN_XSPH = 8,       // v += xsph * eps * dt instead of v += xsph * eps
+    // the power-law branch (world.rs:313-327)
+    MUT_SHEAR_NO_REPS = 9,    // |dv| / r instead of |dv| / (r + 1e-4)
+    MUT_SHEAR_MASS_I = 10,    // m_i / rho_i instead of m_j / rho_j
+    MUT_POWER_EXP_N = 11,     // max'(shear, shear_min)^n instead of ^(n - 1)
+    MUT_SHEAR_NO_FLOOR = 12,  // shear^(n - 1) without the shear_min floor
+    MUT_SHEAR_DV_SQ = 13,     // |dv|^2 instead of |dv|
+    MUT_MAT_OOR_LAST = 14,    // an out-of-table material uses the last entry, not 0
 };
 int g_mutation = MUT_NONE;
 
@@ -296,14 +303,20 @@ struct World {
                 double w = kernels.poly6(r);
                 xsph += (vel[j] - v_i) * (mass[j] / rho_j * w);
                 double dv = norm(vel[j] - v_i);
-                shear += dv / (r + r_eps) * (mass[j] / rho_j);
+                if (g_mutation == MUT_SHEAR_DV_SQ) dv = norm_sq(vel[j] - v_i);
+                const double rr = (g_mutation == MUT_SHEAR_NO_REPS) ? r : r + r_eps;
+                const double mr = (g_mutation == MUT_SHEAR_MASS_I) ? m_i / rho_i : mass[j] / rho_j;
+                shear += dv / rr * mr;
             });
             int n_k = params.n_visc_k;
             size_t m_idx = (mat[i] < (uint64_t)n_k) ? (size_t)mat[i] : 0;
+            if (g_mutation == MUT_MAT_OOR_LAST && mat[i] >= (uint64_t)n_k) m_idx = (size_t)(n_k - 1);
             double kc = params.visc_k[m_idx];
             double nn = params.visc_n[m_idx];
             double sreg = (shear > shear_min) ? shear : shear_min;
-            double mu_eff = kc * std::pow(sreg, nn - 1.0);
+            if (g_mutation == MUT_SHEAR_NO_FLOOR) sreg = shear;
+            const double ex = (g_mutation == MUT_POWER_EXP_N) ? nn : nn - 1.0;
+            double mu_eff = kc * std::pow(sreg, ex);
             V3 f_visc = f_visc_raw * mu_eff;
             V3 acc = (rho_i > 0.0) ? (f_press + f_visc) / rho_i : v3(0, 0, 0);
             acc_out[i] = acc;
@@ -835,6 +848,38 @@ void or_world_neighbor_stats(void* w, uint32_t* counts, uint64_t* sum_ids, uint6
                 z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
                 x ^= z ^ (z >> 31);
             }
+        });
+        counts[i] = c;
+        sum_ids[i] = s;
+        xor_hash[i] = x;
+    }
+}
+
+// The same statistics for the force pass's neighbour set (world.rs:292-299):
+// j != i whose distance r = |p_j - p_i| fails both r >= h and r <= 0, so
+// 0 < r < h or r NaN (a NaN distance falls through both tests and poisons the
+// sums, as in the reference).
+void or_world_force_neighbor_stats(void* w, uint32_t* counts, uint64_t* sum_ids, uint64_t* xor_hash) {
+    World* W = (World*)w;
+    W->grid.build(W->particles);
+    const size_t n = W->particles.size();
+    const double h = W->params.h;
+#pragma omp parallel for schedule(dynamic, 256)
+    for (long long ii = 0; ii < (long long)n; ++ii) {
+        size_t i = (size_t)ii;
+        V3 p_i = W->particles[i].pos;
+        uint32_t c = 0;
+        uint64_t s = 0, x = 0;
+        W->grid.for_each_neighbor(p_i, [&](size_t j) {
+            if (j == i) return;
+            double r = norm(W->particles[j].pos - p_i);
+            if (r >= h || r <= 0.0) return;
+            ++c;
+            s += (uint64_t)j;
+            uint64_t z = (uint64_t)j + 0x9E3779B97F4A7C15ull;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            x ^= z ^ (z >> 31);
         });
         counts[i] = c;
         sum_ids[i] = s;

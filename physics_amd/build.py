@@ -1,6 +1,8 @@
 """Builds physics_amd/libphy_ffi.so (gfx950) from physics_amd/csrc with hipcc.
 
 Usage: python -m physics_amd.build   (or physics_amd.build.build())
+       python -m physics_amd.build --hbm-copy   also builds the plain-copy
+       program whose rocprofv3 --pmc counts calibrate FETCH_SIZE / WRITE_SIZE
 """
 from __future__ import annotations
 
@@ -35,7 +37,7 @@ COMMON = [
 ]
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, hbm_copy: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     procs = []
     objs = []
@@ -65,13 +67,20 @@ def build(verbose: bool = False) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    if hbm_copy:
+        build_hbm_copy()
+    return LIB
+
+
+def build_hbm_copy() -> str:
+    """The counter-calibration copy program (profiling only, not the library)."""
     cmd = [HIPCC, *COMMON, os.path.join(CSRC, "hbm_copy.hip"), "-o", COPY_EXE + ".tmp"]
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed for hbm_copy.hip:\n" + r.stdout.decode(errors="replace"))
     os.replace(COPY_EXE + ".tmp", COPY_EXE)
-    return LIB
+    return COPY_EXE
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, hbm_copy="--hbm-copy" in sys.argv))

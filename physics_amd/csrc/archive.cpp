@@ -74,6 +74,18 @@ class Parser {
    private:
     const std::string& t_;
     size_t p_ = 0;
+    // Nesting depth of the open objects / arrays. serde_json stops at 128
+    // levels and reports an error; so does this reader, instead of recursing
+    // until the stack overflows.
+    int depth_ = 0;
+    static constexpr int kMaxDepth = 128;
+    struct Nest {
+        int& d;
+        explicit Nest(int& depth) : d(depth) {
+            if (++d > kMaxDepth) bad("nesting too deep (more than 128 levels)");
+        }
+        ~Nest() { --d; }
+    };
 
     void ws() {
         while (p_ < t_.size() && (t_[p_] == ' ' || t_[p_] == '\n' || t_[p_] == '\r' || t_[p_] == '\t')) ++p_;
@@ -119,6 +131,7 @@ class Parser {
         return number();
     }
     Json object() {
+        Nest nest(depth_);
         expect('{');
         Json j = Json::object();
         if (peek() == '}') {
@@ -137,6 +150,7 @@ class Parser {
         }
     }
     Json array() {
+        Nest nest(depth_);
         expect('[');
         Json j = Json::array();
         if (peek() == ']') {

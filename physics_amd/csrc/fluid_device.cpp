@@ -279,7 +279,7 @@ void DeviceFluid::set_scene_params(size_t s, const HostParams& hp) {
     size_tables(nullptr);
     push_params();
     dens_valid_ = aux_valid_ = false;
-    tape_invalidate();
+    tape_invalidate("the parameters changed");
 }
 
 void DeviceFluid::set_params(const HostParams& hp) {
@@ -358,7 +358,7 @@ void DeviceFluid::upload_scenes(const uint32_t* counts, const float* pos3, const
     has_body_ = false;
     sync();  // host vectors go out of scope
     dens_valid_ = aux_valid_ = false;
-    tape_invalidate();
+    tape_invalidate("the particles were replaced");
     graph_dirty_ = true;
 }
 
@@ -538,6 +538,11 @@ void DeviceFluid::set_graph(bool on) {
 
 void DeviceFluid::set_tuning(const Tuning& t) {
     if (!tuning_valid(t)) throw std::invalid_argument("tuning: list lengths must be 8, 16 or 32 and split 0 or 1");
+    // the force pass's list length is part of its summation order: recomputed
+    // segments would not reproduce the recorded steps
+    if (!(t.density_list == tuning_.density_list && t.force_list == tuning_.force_list &&
+          t.split == tuning_.split))
+        tape_invalidate("the launch tuning changed");
     tuning_ = t;
     graph_dirty_ = true;
 }
@@ -724,8 +729,11 @@ void DeviceFluid::tape_record(float dt) {
     }
 }
 
-void DeviceFluid::tape_invalidate() {
-    if (taping_) tape_broken_ = true;
+void DeviceFluid::tape_invalidate(const char* why) {
+    if (taping_ && !tape_broken_) {
+        tape_broken_ = true;
+        tape_broken_why_ = why;
+    }
 }
 
 void DeviceFluid::tape_end() {
@@ -735,6 +743,7 @@ void DeviceFluid::tape_end() {
     if (taping_) graph_dirty_ = true;
     taping_ = false;
     tape_broken_ = false;
+    tape_broken_why_.clear();
 }
 
 DeviceFluid::StepBufs DeviceFluid::slot_bufs(TapeSlot& t) {
@@ -761,7 +770,9 @@ DeviceFluid::StepBufs DeviceFluid::slot_bufs(TapeSlot& t) {
 void DeviceFluid::tape_backward(const float* xbar_T, const float* vbar_T, const float* rhobar_T,
                                 float* xbar0, float* vbar0, double* dparams) {
     if (!taping_) throw std::logic_error("tape: no recording (call tape_begin first)");
-    if (tape_broken_) throw std::logic_error("tape: particles were replaced while recording");
+    if (tape_broken_)
+        throw std::logic_error("tape: the recording is invalid (" + tape_broken_why_ +
+                               " while recording); call tape_end and record again");
     const size_t T = tape_dt_.size();
     const int n = (int)n_;
     const size_t S = scn_.size();
@@ -972,6 +983,45 @@ void DeviceFluid::neighbor_stats(uint32_t* cnt, uint64_t* sum, uint64_t* xr) {
     sync();
 }
 
+void DeviceFluid::gather_stats(int pass, uint32_t* cnt, uint64_t* sum, uint64_t* xr) {
+    if (pass != 0 && pass != 1) throw std::invalid_argument("gather_stats: pass must be 0 (density) or 1 (force)");
+    const int n = (int)n_;
+    if (n == 0) return;
+    // the step's binning into the step's sorted buffers, with a domain array
+    // of its own (the last step's watchdog survives); rho / p / aux of the
+    // last step are not touched, since the statistics kernels write only the
+    // statistics
+    DevBuf<Domain> dom;
+    dom.ensure(scn_.size());
+    StepBufs sb = own_bufs();
+    sb.dom = dom.get();
+    run_domain_and_sort(sb);
+    DevBuf<uint32_t> c;
+    DevBuf<unsigned long long> s, x;
+    c.ensure(n);
+    s.ensure(n);
+    x.ensure(n);
+    ForceArgs a{};
+    a.posm = sb.posm_s;
+    a.posl = sb.posl;
+    a.velm = sb.velm_s;
+    a.rho = sb.rho;
+    a.prs = sb.prs;
+    a.mp = sb.mp;
+    a.cell = sb.cell_s;
+    a.start = sb.start;
+    a.id = sb.id_s;
+    a.mat = sb.mat_s;
+    GatherStats g{sb.id_s, c.get(), s.get(), x.get()};
+    // the force variant the step would launch (recorded steps sum the shear rate)
+    launch_gather_stats(st_, table(sb.dom), pass, a, any_shear_ || taping_, tuning_, g);
+    hip_check(hipGetLastError(), "gather_stats");
+    hip_check(hipMemcpyAsync(cnt, c.get(), n * sizeof(uint32_t), hipMemcpyDeviceToHost, st_), "gstats");
+    hip_check(hipMemcpyAsync(sum, s.get(), n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_), "gstats");
+    hip_check(hipMemcpyAsync(xr, x.get(), n * sizeof(uint64_t), hipMemcpyDeviceToHost, st_), "gstats");
+    sync();
+}
+
 // ---------------------------------------------------------------------------
 // pending body_acc and coupling
 
@@ -985,6 +1035,9 @@ void DeviceFluid::set_body_acc(const float* b3) {
     }
     has_body_ = any;
     if (!any) return;
+    // body_acc is not differentiated: the recomputed segments would step
+    // without it
+    tape_invalidate("a body acceleration was set");
     stage4_.ensure(n_);
     hip_check(hipMemcpyAsync(stage4_.get(), b4.data(), n_ * sizeof(float4), hipMemcpyHostToDevice, st_), "body_acc");
     launch_gather_by_id4(st_, (int)n_, id_.get(), stage4_.get(), body_.get());
@@ -1039,7 +1092,7 @@ void DeviceFluid::sort_current() {
 void DeviceFluid::couple_bodies(CoupleBody* bodies, uint32_t nb, double dt) {
     one_scene("couple_bodies");
     if (n_ == 0 || nb == 0) return;  // world.rs:415-417
-    tape_invalidate();
+    tape_invalidate("a coupling pass changed the particles");
     const int n = (int)n_;
     cbodies_.ensure(nb);
     cdisp_.ensure(nb);
@@ -1071,7 +1124,7 @@ void DeviceFluid::couple_bodies(CoupleBody* bodies, uint32_t nb, double dt) {
 void DeviceFluid::couple_points(CouplePoint* pts, uint32_t np, double dt, double drag, double soft_density) {
     one_scene("couple_points");
     if (np == 0 || n_ == 0) return;  // world.rs:692-694
-    tape_invalidate();
+    tape_invalidate("a coupling pass changed the particles");
     const int n = (int)n_;
     sort_current();
     vel64_.ensure(3 * n_);
@@ -1094,7 +1147,7 @@ void DeviceFluid::couple_heat(const HeatGrid& hgh, double* src, double dt, doubl
     const uint64_t ncell = (uint64_t)hgh.nx * hgh.ny * hgh.nz;
     if (ncell >= 0xffffffffull) throw std::invalid_argument("couple_heat: heat grid too large");
     if (heat_gain > 0.0 && !src) throw std::invalid_argument("couple_heat: heat_gain > 0 needs a source grid");
-    tape_invalidate();
+    tape_invalidate("a coupling pass changed the particles");
     const int n = (int)n_;
     if (!has_body_) {
         hip_check(hipMemsetAsync(body_.get(), 0, n_ * sizeof(float4), st_), "body_acc");

@@ -148,6 +148,11 @@ class DeviceFluid {
     // (original order): count, sum of neighbour indices, xor of hashed
     // indices; indices are scene-local.
     void neighbor_stats(uint32_t* cnt, uint64_t* sum, uint64_t* xr);
+    // The same statistics from the step's own gathers (pass 0: density, j
+    // with r < h, self included; pass 1: forces, j != i with 0 < r < h or a
+    // NaN distance), run on the current state with the current tuning: the
+    // neighbour sets the density and force sums actually run over.
+    void gather_stats(int pass, uint32_t* cnt, uint64_t* sum, uint64_t* xr);
 
     hipStream_t stream() const { return st_; }
     void sync();
@@ -260,7 +265,10 @@ class DeviceFluid {
     void save_state(Snapshot& c);
     void load_state(const Snapshot& c);
     void tape_record(float dt);
-    void tape_invalidate();
+    // A recording becomes unusable when the state, the parameters, the launch
+    // tuning or the forces change outside a recorded step; tape_backward then
+    // fails with the reason.
+    void tape_invalidate(const char* why);
     CoupleParams couple_params() const;
     void one_scene(const char* what) const;
     // bins the current positions into the step's sorted buffers with a domain
@@ -316,6 +324,7 @@ class DeviceFluid {
     hipEvent_t ev();
 
     bool taping_ = false, tape_broken_ = false;
+    std::string tape_broken_why_;
     uint32_t tape_K_ = 0;
     size_t tape_n_ = 0;
     std::vector<float> tape_dt_;

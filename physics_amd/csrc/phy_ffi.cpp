@@ -549,6 +549,16 @@ int32_t phy_fluid_neighbor_stats(PhyWorldHandle* w, uint32_t* counts, uint64_t* 
     });
 }
 
+int32_t phy_fluid_gather_stats(PhyWorldHandle* w, int32_t pass, uint32_t* counts, uint64_t* sums,
+                               uint64_t* xors, uintptr_t len) {
+    return guard<int32_t>(-1, [&] {
+        FluidWorld& f = need_fluid(w)->world();
+        if (!counts || !sums || !xors || len < f.len()) throw std::invalid_argument("buffers too short");
+        f.gather_stats(pass, counts, sums, xors);
+        return 0;
+    });
+}
+
 int32_t phy_fluid_tape_begin(PhyWorldHandle* w, uint32_t checkpoint_every) {
     return guard<int32_t>(-1, [&] {
         need_fluid(w)->world().tape_begin(checkpoint_every);
@@ -874,6 +884,16 @@ int32_t phy_fluid_batch_neighbor_stats(PhyFluidBatch* b, uint32_t* counts, uint6
         phy::DeviceFluid& d = need_batch(b)->device();
         if (!counts || !sums || !xors || len < d.size()) throw std::invalid_argument("buffers too short");
         d.neighbor_stats(counts, sums, xors);
+        return 0;
+    });
+}
+
+int32_t phy_fluid_batch_gather_stats(PhyFluidBatch* b, int32_t pass, uint32_t* counts, uint64_t* sums,
+                                     uint64_t* xors, uintptr_t len) {
+    return guard<int32_t>(-1, [&] {
+        phy::DeviceFluid& d = need_batch(b)->device();
+        if (!counts || !sums || !xors || len < d.size()) throw std::invalid_argument("buffers too short");
+        d.gather_stats(pass, counts, sums, xors);
         return 0;
     });
 }

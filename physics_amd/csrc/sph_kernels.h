@@ -78,6 +78,17 @@ struct ForceArgs {
     float4* fvisc_out;        // optional (adjoint tape): viscous force per unit mu, w: shear sum
 };
 
+// Outputs of the statistics form of the density / force gathers (original
+// index order): per particle the count, the sum and the xor of SplitMix64
+// hashes of the scene-local original indices of the neighbours the pass's
+// exact test accepts.
+struct GatherStats {
+    const uint32_t* id;        // original index of each sorted slot
+    uint32_t* cnt;
+    unsigned long long* sum;
+    unsigned long long* xr;
+};
+
 // One recorded step for the adjoint, in that step's cell order.
 struct AdjSlot {
     const float4* posm;       // x, m
@@ -135,6 +146,11 @@ void launch_diag(hipStream_t st, const SceneTable& T, const float4* posm, const 
                  double* partial, double* out);
 void launch_unsort4(hipStream_t st, int n, const uint32_t* id, const float4* in, float4* out);
 void launch_unsort1(hipStream_t st, int n, const uint32_t* id, const float* in, float* out);
+// The step's own density (pass 0) or force (pass 1) gather on a sorted state
+// (a: the step's sorted buffers; rho / p / mp need not hold values: the
+// accept tests read positions only), with statistics instead of outputs.
+void launch_gather_stats(hipStream_t st, const SceneTable& T, int pass, const ForceArgs& a,
+                         bool shear, const Tuning& tu, const GatherStats& gs);
 void launch_neighbor_stats(hipStream_t st, const SceneTable& T, const float4* posm,
                            const uint32_t* cellv, const uint32_t* start, const uint32_t* id,
                            uint32_t* cnt, unsigned long long* sum, unsigned long long* xr);

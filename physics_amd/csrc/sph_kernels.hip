@@ -440,6 +440,38 @@ __global__ __launch_bounds__(256) void k_reorder(
 }
 
 // ---------------------------------------------------------------------------
+// neighbour-set statistics of the gathers themselves
+//
+// With STATS the density and force kernels run their own gather (the same
+// scan, filters, lists and exact tests as the step) and, instead of writing
+// the step's outputs, record for each particle the count, the sum and the xor
+// of SplitMix64-hashed scene-local original indices of every listed slot their
+// exact test accepts: the neighbour sets the physics sums over, for exact
+// comparison with the reference's sets.
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+struct SetAcc {
+    uint32_t c = 0;
+    unsigned long long s = 0, x = 0;
+    __device__ __forceinline__ void add(bool ok, uint32_t id) {
+        c += ok ? 1u : 0u;
+        s += ok ? (unsigned long long)id : 0ull;
+        x ^= ok ? mix64(id) : 0ull;
+    }
+    __device__ __forceinline__ void store(const GatherStats& g, uint32_t o) const {
+        g.cnt[o] = c;
+        g.sum[o] = s;
+        g.xr[o] = x;
+    }
+};
+
+// ---------------------------------------------------------------------------
 // density / EOS (world.rs:204-246)
 //
 // The pair geometry and the sum run in f64: p = k (rho - rho0) cancels near
@@ -447,13 +479,13 @@ __global__ __launch_bounds__(256) void k_reorder(
 // rho / (rho - rho0). MI355X executes f64 FMA at the unpacked FP32 rate, and the
 // compaction limits it to true neighbours.
 
-template <int L, int MODE>
+template <int L, int MODE, bool STATS>
 __global__ __launch_bounds__(256) void k_density(SceneTable T, const float4* __restrict__ posm,
                                                  const float4* __restrict__ posl,
                                                  const uint32_t* __restrict__ cellv,
                                                  const uint32_t* __restrict__ start_all,
                                                  float* rho_out, float* p_out, float4* velm,
-                                                 float* mp_out) {
+                                                 float* mp_out, GatherStats gs) {
     __shared__ uint32_t lst[L][256];
     const int tid = threadIdx.x;
     const TileLoc TL = tile_loc(T);
@@ -473,6 +505,8 @@ __global__ __launch_bounds__(256) void k_density(SceneTable T, const float4* __r
     }
     const double xi = pi.x, yi = pi.y, zi = pi.z;
     double sum = 0.0;
+    SetAcc sa;
+    const uint32_t soff = STATS ? T.info[TL.s].off : 0u;
     // four neighbour loads in flight: the register cost keeps this kernel's
     // occupancy (force keeps two: four would cost it a wave per SIMD); the
     // per-lane scan takes four candidates per round (six would cost a wave too)
@@ -488,9 +522,14 @@ __global__ __launch_bounds__(256) void k_density(SceneTable T, const float4* __r
             const double q = P.h2_64 - s;
             const double t = (double)pj.w * (q * q * q);
             sum += valid ? t : 0.0;
+            if (STATS) sa.add(valid, at(gs.id, j) - soff);
         },
         [&]() {});
     if (!act) return;
+    if (STATS) {
+        sa.store(gs, gs.id[i]);
+        return;
+    }
     double rho = P.c6_64 * sum;
     double over = rho - P.rest64;
     double p = over > 0.0 ? P.stiff64 * over : 0.0;
@@ -527,8 +566,10 @@ struct NbrData {
     float mp;
 };
 
+// Returns the exact force-pass decision for the slot (the pair's terms are
+// added only when it holds).
 template <bool SHEAR>
-__device__ __forceinline__ void pair_force(ForceAcc<SHEAR>& A, const DevParams& P, uint32_t i,
+__device__ __forceinline__ bool pair_force(ForceAcc<SHEAR>& A, const DevParams& P, uint32_t i,
                                            uint32_t j, float4 pi, float4 vi, float Pi,
                                            const NbrData& nb, bool listed) {
     const float4 pj = nb.p;
@@ -572,9 +613,10 @@ __device__ __forceinline__ void pair_force(ForceAcc<SHEAR>& A, const DevParams& 
         float t = dv * __builtin_amdgcn_rcpf(r + P.r_eps) * mr;
         A.b[9] += valid ? t : 0.f;
     }
+    return valid;
 }
 
-template <int L, bool SHEAR, int MODE>
+template <int L, bool SHEAR, int MODE, bool STATS>
 __global__ __launch_bounds__(256) void k_force(
     SceneTable T, float dt, const float4* __restrict__ posm, const float4* __restrict__ posl,
     const float4* __restrict__ velm,
@@ -582,7 +624,8 @@ __global__ __launch_bounds__(256) void k_force(
     const uint32_t* __restrict__ cellv, const uint32_t* __restrict__ start_all,
     const uint32_t* __restrict__ idv, const uint32_t* __restrict__ matv,
     const float4* __restrict__ body, float4* posm_out, float4* vel_out, uint32_t* id_out,
-    uint32_t* mat_out, float4* acc_out, float* mu_out, float4* xsph_out, float4* fvisc_out) {
+    uint32_t* mat_out, float4* acc_out, float* mu_out, float4* xsph_out, float4* fvisc_out,
+    GatherStats gs) {
     __shared__ uint32_t lst[L][256];
     const int tid = threadIdx.x;
     const TileLoc TL = tile_loc(T);
@@ -606,6 +649,8 @@ __global__ __launch_bounds__(256) void k_force(
     if (LOC && act) cl.li = posl[i];
     ForceAcc<SHEAR> A;
     A.zero();
+    SetAcc sa;
+    const uint32_t soff = STATS ? T.info[TL.s].off : 0u;
     // per-lane rounds of six candidates: a row of five half cells at rest
     // spacing holds about five, so a row takes one round instead of two
     gather<L, MODE, (MODE == kUniformWaves ? 1 : 2), LOC, true, 6>(
@@ -615,10 +660,15 @@ __global__ __launch_bounds__(256) void k_force(
         // every slot at or past the lane's count names the lane itself, which
         // no validity test accepts, so the listed flag is not needed
         [&](uint32_t o, const NbrData& nb, bool) {
-            pair_force<SHEAR>(A, P, (uint32_t)i << 4, o, pi, vi, Pi, nb, true);
+            const bool ok = pair_force<SHEAR>(A, P, (uint32_t)i << 4, o, pi, vi, Pi, nb, true);
+            if (STATS) sa.add(ok, at(gs.id, o) - soff);
         },
         [&]() { A.merge(); });
     if (!act) return;
+    if (STATS) {
+        sa.store(gs, gs.id[i]);
+        return;
+    }
 
     // power-law effective viscosity (world.rs:319-327)
     int m = 0;
@@ -853,62 +903,86 @@ bool tuning_valid(const Tuning& t) {
     return ok(t.density_list) && ok(t.force_list) && (t.split == 0 || t.split == 1);
 }
 
-template <int L>
+template <int L, bool STATS>
 static void density_l(hipStream_t st, const SceneTable& T, bool split, const float4* posm,
                       const float4* posl, const uint32_t* cellv, const uint32_t* start, float* rho,
-                      float* prs, float4* velm, float* mp) {
+                      float* prs, float4* velm, float* mp, const GatherStats& gs) {
     if (split) {
-        hipLaunchKernelGGL((k_density<L, kUniformWaves>), dim3(T.ntiles), dim3(256), 0, st, T, posm,
-                           posl, cellv, start, rho, prs, velm, mp);
-        hipLaunchKernelGGL((k_density<L, kOtherWaves>), dim3(T.ntiles), dim3(256), 0, st, T, posm,
-                           posl, cellv, start, rho, prs, velm, mp);
+        hipLaunchKernelGGL((k_density<L, kUniformWaves, STATS>), dim3(T.ntiles), dim3(256), 0, st, T,
+                           posm, posl, cellv, start, rho, prs, velm, mp, gs);
+        hipLaunchKernelGGL((k_density<L, kOtherWaves, STATS>), dim3(T.ntiles), dim3(256), 0, st, T,
+                           posm, posl, cellv, start, rho, prs, velm, mp, gs);
     } else {
-        hipLaunchKernelGGL((k_density<L, kAllWaves>), dim3(T.ntiles), dim3(256), 0, st, T, posm, posl,
-                           cellv, start, rho, prs, velm, mp);
+        hipLaunchKernelGGL((k_density<L, kAllWaves, STATS>), dim3(T.ntiles), dim3(256), 0, st, T, posm,
+                           posl, cellv, start, rho, prs, velm, mp, gs);
+    }
+}
+
+template <bool STATS>
+static void density_any(hipStream_t st, const SceneTable& T, const float4* posm, const float4* posl,
+                        const uint32_t* cellv, const uint32_t* start, float* rho, float* prs,
+                        float4* velm, float* mp, const Tuning& tu, const GatherStats& gs) {
+    if (T.ntiles == 0) return;
+    const bool sp = tu.split != 0;
+    switch (tu.density_list) {
+        case 8: density_l<8, STATS>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
+        case 16: density_l<16, STATS>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
+        default: density_l<32, STATS>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
     }
 }
 
 void launch_density(hipStream_t st, const SceneTable& T, const float4* posm, const float4* posl,
                     const uint32_t* cellv, const uint32_t* start, float* rho, float* prs,
                     float4* velm, float* mp, const Tuning& tu) {
-    if (T.ntiles == 0) return;
-    const bool sp = tu.split != 0;
-    switch (tu.density_list) {
-        case 8: density_l<8>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp); break;
-        case 16: density_l<16>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp); break;
-        default: density_l<32>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp); break;
+    density_any<false>(st, T, posm, posl, cellv, start, rho, prs, velm, mp, tu, GatherStats{});
+}
+
+template <int L, bool SHEAR, int MODE, bool STATS>
+static void force_one(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a,
+                      const GatherStats& gs) {
+    hipLaunchKernelGGL((k_force<L, SHEAR, MODE, STATS>), dim3(T.ntiles), dim3(256), 0, st, T, dt, a.posm,
+                       a.posl, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body, a.posm_out,
+                       a.vel_out, a.id_out, a.mat_out, a.acc_out, a.mu_out, a.xsph_out, a.fvisc_out, gs);
+}
+
+template <int L, bool SHEAR, bool STATS>
+static void force_l(hipStream_t st, const SceneTable& T, bool split, float dt, const ForceArgs& a,
+                    const GatherStats& gs) {
+    if (split) {
+        force_one<L, SHEAR, kUniformWaves, STATS>(st, T, dt, a, gs);
+        force_one<L, SHEAR, kOtherWaves, STATS>(st, T, dt, a, gs);
+    } else {
+        force_one<L, SHEAR, kAllWaves, STATS>(st, T, dt, a, gs);
     }
 }
 
-template <int L, bool SHEAR, int MODE>
-static void force_one(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a) {
-    hipLaunchKernelGGL((k_force<L, SHEAR, MODE>), dim3(T.ntiles), dim3(256), 0, st, T, dt, a.posm,
-                       a.posl, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body, a.posm_out,
-                       a.vel_out, a.id_out, a.mat_out, a.acc_out, a.mu_out, a.xsph_out, a.fvisc_out);
-}
-
-template <int L, bool SHEAR>
-static void force_l(hipStream_t st, const SceneTable& T, bool split, float dt, const ForceArgs& a) {
-    if (split) {
-        force_one<L, SHEAR, kUniformWaves>(st, T, dt, a);
-        force_one<L, SHEAR, kOtherWaves>(st, T, dt, a);
-    } else {
-        force_one<L, SHEAR, kAllWaves>(st, T, dt, a);
+template <bool STATS>
+static void force_any(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a, bool shear,
+                      const Tuning& tu, const GatherStats& gs) {
+    if (T.ntiles == 0) return;
+    const bool sp = tu.split != 0;
+    switch (tu.force_list * 2 + (shear ? 1 : 0)) {
+        case 16: force_l<8, false, STATS>(st, T, sp, dt, a, gs); break;
+        case 17: force_l<8, true, STATS>(st, T, sp, dt, a, gs); break;
+        case 32: force_l<16, false, STATS>(st, T, sp, dt, a, gs); break;
+        case 33: force_l<16, true, STATS>(st, T, sp, dt, a, gs); break;
+        case 65: force_l<32, true, STATS>(st, T, sp, dt, a, gs); break;
+        default: force_l<32, false, STATS>(st, T, sp, dt, a, gs); break;
     }
 }
 
 void launch_force(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a, bool shear,
                   const Tuning& tu) {
-    if (T.ntiles == 0) return;
-    const bool sp = tu.split != 0;
-    switch (tu.force_list * 2 + (shear ? 1 : 0)) {
-        case 16: force_l<8, false>(st, T, sp, dt, a); break;
-        case 17: force_l<8, true>(st, T, sp, dt, a); break;
-        case 32: force_l<16, false>(st, T, sp, dt, a); break;
-        case 33: force_l<16, true>(st, T, sp, dt, a); break;
-        case 65: force_l<32, true>(st, T, sp, dt, a); break;
-        default: force_l<32, false>(st, T, sp, dt, a); break;
-    }
+    force_any<false>(st, T, dt, a, shear, tu, GatherStats{});
+}
+
+void launch_gather_stats(hipStream_t st, const SceneTable& T, int pass, const ForceArgs& a,
+                         bool shear, const Tuning& tu, const GatherStats& gs) {
+    if (pass == 0)
+        density_any<true>(st, T, a.posm, a.posl, a.cell, a.start, nullptr, nullptr, nullptr, nullptr,
+                          tu, gs);
+    else
+        force_any<true>(st, T, 0.f, a, shear, tu, gs);
 }
 
 void launch_diag(hipStream_t st, const SceneTable& T, const float4* posm, const float4* vel,

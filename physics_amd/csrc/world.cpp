@@ -355,6 +355,10 @@ void FluidWorld::neighbor_stats(uint32_t* cnt, uint64_t* sum, uint64_t* xr) {
     flush_host();
     dev_.neighbor_stats(cnt, sum, xr);
 }
+void FluidWorld::gather_stats(int pass, uint32_t* cnt, uint64_t* sum, uint64_t* xr) {
+    flush_host();
+    dev_.gather_stats(pass, cnt, sum, xr);
+}
 
 // ---------------------------------------------------------------------------
 

@@ -128,6 +128,7 @@ class FluidWorld {
     int last_validate();
     void diagnostics(double out5[5]);
     void neighbor_stats(uint32_t* cnt, uint64_t* sum, uint64_t* xr);
+    void gather_stats(int pass, uint32_t* cnt, uint64_t* sum, uint64_t* xr);
     DeviceFluid& device() { return dev_; }
 
     // FluidWorld::to_gpu_flat over Grid::to_flat (world.rs:782-859,

@@ -38,6 +38,7 @@ EXPORTS = [
     "phy_fluid_fill_box", "phy_fluid_lattice_mass", "phy_fluid_set_particles",
     "phy_fluid_set_particles_f32", "phy_fluid_get_field", "phy_fluid_get_field_f32",
     "phy_fluid_set_keep_aux", "phy_fluid_set_body_acc", "phy_fluid_diagnostics_f64",
+    "phy_fluid_gather_stats", "phy_fluid_batch_gather_stats",
     "phy_fluid_neighbor_stats", "phy_world_step_n", "phy_fluid_sync", "phy_fluid_profile_begin",
     "phy_fluid_profile_end", "phy_fluid_tape_begin", "phy_fluid_tape_steps", "phy_fluid_tape_backward",
     "phy_fluid_tape_end", "phy_fluid_set_graph", "phy_fluid_batch_create", "phy_fluid_batch_destroy",
@@ -214,6 +215,7 @@ def lib():
         "phy_fluid_sample_velocity": ([H, dp, C.c_uint32, dp], C.c_int32),
         "phy_fluid_diagnostics_f64": ([H, dp], C.c_int32),
         "phy_fluid_neighbor_stats": ([H, u32p, u64p, u64p, up], C.c_int32),
+        "phy_fluid_gather_stats": ([H, C.c_int32, u32p, u64p, u64p, up], C.c_int32),
         "phy_fluid_sync": ([H], C.c_int32),
         "phy_fluid_profile_begin": ([H], C.c_int32),
         "phy_fluid_profile_end": ([H, dp, u32p], C.c_int32),
@@ -238,6 +240,7 @@ def lib():
         "phy_fluid_batch_get_field_f32": ([H, C.c_int32, fp, up], up),
         "phy_fluid_batch_diagnostics_f64": ([H, dp], C.c_int32),
         "phy_fluid_batch_neighbor_stats": ([H, u32p, u64p, u64p, up], C.c_int32),
+        "phy_fluid_batch_gather_stats": ([H, C.c_int32, u32p, u64p, u64p, up], C.c_int32),
         "phy_fluid_batch_tape_begin": ([H, C.c_uint32], C.c_int32),
         "phy_fluid_batch_tape_steps": ([H], C.c_uint32),
         "phy_fluid_batch_tape_backward": ([H, fp, fp, fp, fp, fp, dp], C.c_int32),
@@ -607,6 +610,24 @@ class FluidWorld(World):
             raise PhyError(last_error())
         return c, s, x
 
+    def gather_stats(self, which: str):
+        """Neighbour statistics taken by the step's own gather: ``which`` is
+        "density" (r < h, self included) or "force" (j != i, 0 < r < h, NaN
+        kept)."""
+        n = len(self)
+        c = np.zeros(n, dtype=np.uint32)
+        s = np.zeros(n, dtype=np.uint64)
+        x = np.zeros(n, dtype=np.uint64)
+        rc = lib().phy_fluid_gather_stats(
+            self._h, _PASS[which], c.ctypes.data_as(C.POINTER(C.c_uint32)),
+            s.ctypes.data_as(C.POINTER(C.c_uint64)), x.ctypes.data_as(C.POINTER(C.c_uint64)), n)
+        if rc != 0:
+            raise PhyError(last_error())
+        return c, s, x
+
+
+_PASS = {"density": 0, "force": 1}
+
 
 def _autotune(fn, h, dt, steps, cache_path):
     t = PhyTuning()
@@ -740,6 +761,17 @@ class FluidBatch:
         x = np.zeros(n, dtype=np.uint64)
         self._ok(lib().phy_fluid_batch_neighbor_stats(
             self._h, _u32(c), s.ctypes.data_as(C.POINTER(C.c_uint64)),
+            x.ctypes.data_as(C.POINTER(C.c_uint64)), n))
+        return c, s, x
+
+    def gather_stats(self, which: str):
+        """As FluidWorld.gather_stats, concatenated; indices are scene-local."""
+        n = len(self)
+        c = np.zeros(n, dtype=np.uint32)
+        s = np.zeros(n, dtype=np.uint64)
+        x = np.zeros(n, dtype=np.uint64)
+        self._ok(lib().phy_fluid_batch_gather_stats(
+            self._h, _PASS[which], _u32(c), s.ctypes.data_as(C.POINTER(C.c_uint64)),
             x.ctypes.data_as(C.POINTER(C.c_uint64)), n))
         return c, s, x
 

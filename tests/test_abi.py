@@ -82,6 +82,21 @@ def test_archive_error_paths(tmp_path):
         assert "archive" in ffi.last_error(), (name, ffi.last_error())
 
 
+def test_archive_nesting_limit(tmp_path):
+    """serde_json stops at 128 nested levels with an error (its recursion
+    limit), so phy_world_load returns NULL for a deeply nested file rather than
+    exhausting the stack; 128 levels still parse (and then fail on content)."""
+    L = ffi.lib()
+    deep = tmp_path / "deep.json"
+    deep.write_text("[" * 200000 + "]" * 200000)
+    assert L.phy_world_load(str(deep).encode()) is None
+    assert "nesting too deep" in ffi.last_error()
+    ok = tmp_path / "ok.json"
+    ok.write_text('{"version": 1, "time": ' + "[" * 127 + "]" * 127 + ', "subsystems": []}')
+    assert L.phy_world_load(str(ok).encode()) is None
+    assert "nesting" not in ffi.last_error(), ffi.last_error()
+
+
 def test_archive_fixture_layout():
     """The committed fixture follows the reference layout (tests/golden/
     make_archive_fixture.py): tagged subsystems, {"__f64": bits} scalars, u64

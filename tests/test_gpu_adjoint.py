@@ -10,8 +10,11 @@ a 1e-4 central difference only O(eps)-accurate, so they use a short rollout
 and eps = 1e-6. The walls are out of reach: a wall reflection is
 discontinuous in the initial state, so no finite difference can check it.
 Tolerance: 1e-3 relative (FP32 adjoint along the FP32 trajectory vs f64 FD),
-per direction for the small blocks, and against the RMS directional
-derivative for the 32^3 rollout block (whose FD is itself noisy at 1e-4).
+per direction. On the 32^3 rollout block the eps = 1e-4 difference is itself
+noisy at ~1e-4 of the RMS directional derivative (C0 kinks at r = h), so a
+direction that misses the bar with it is re-differenced with eps = 1e-5, and
+directions whose derivative cancels below 5 % of the RMS are held to an
+absolute bound (see _fd_check).
 """
 import numpy as np
 import pytest
@@ -141,19 +144,82 @@ def test_backward_leaves_the_world_state_and_reruns_identically():
         g.tape_backward(np.ones((len(pos), 3)), None, None)
 
 
+def test_changes_while_recording_invalidate_the_tape():
+    # the recomputed segments step without body_acc and with the tuning of
+    # the moment, so either change while recording must fail the backward
+    # pass instead of returning gradients of another trajectory
+    pos, vel, mass = _setup(nside=5, seed=7)
+    n = len(pos)
+    for change, why in ((lambda g: g.set_body_acc(np.full((n, 3), 0.5)), "body acceleration"),
+                        (lambda g: g.set_tuning(8, 8, 1), "tuning")):
+        g = ffi.FluidWorld(_params(ffi.PhySphParams))
+        g.set_particles_f32(pos, vel, np.full(n, mass, np.float32))
+        g.tape_begin(4)
+        g.step(DT32, 3)
+        change(g)
+        g.step(DT32, 2)
+        with pytest.raises(ffi.PhyError, match=why):
+            g.tape_backward(np.ones((n, 3)), None, None)
+        g.tape_end()
+        # a fresh recording works again
+        g.tape_begin(4)
+        g.step(DT32, 2)
+        g.tape_backward(np.ones((n, 3)), None, None)
+        g.tape_end()
+    # setting the same tuning again changes nothing and keeps the recording
+    g = ffi.FluidWorld(_params(ffi.PhySphParams))
+    g.set_particles_f32(pos, vel, np.full(n, mass, np.float32))
+    t = g.tuning()
+    g.tape_begin(4)
+    g.step(DT32, 2)
+    g.set_tuning(*t)
+    g.tape_backward(np.ones((n, 3)), None, None)
+    g.tape_end()
+
+
+def _fd_check(ads, fds, refd, floor_frac=0.05):
+    """Per-direction bar: |adjoint - fd| <= TOL |fd| for every direction with
+    |fd| above floor_frac x the RMS directional derivative. A direction that
+    misses it with the eps = 1e-4 difference is retried with the finer
+    difference refd(t) (C0 kinks of the kernels at r = h make the 1e-4
+    difference itself move by ~1e-4 of the scale) and must pass with that
+    one. Directions below the floor (derivatives that cancel) are held to
+    TOL x floor_frac x RMS in absolute terms and reported."""
+    ads, fds = np.asarray(ads), np.asarray(fds)
+    rms = np.sqrt(np.mean(fds ** 2))
+    floor = floor_frac * rms
+    worst = 0.0
+    for t, (ad, fd) in enumerate(zip(ads, fds)):
+        if abs(fd) < floor:
+            print(f"direction {t}: |fd| {abs(fd):.3e} below the floor {floor:.3e}: "
+                  f"|adjoint - fd| = {abs(ad - fd):.3e}", flush=True)
+            assert abs(ad - fd) <= TOL * floor, t
+            continue
+        rel = abs(ad - fd) / abs(fd)
+        if rel >= TOL:
+            fd2 = refd(t)
+            rel2 = abs(ad - fd2) / abs(fd2)
+            print(f"direction {t}: rel {rel:.2e} at eps 1e-4; finer difference {fd2:.8e} "
+                  f"gives rel {rel2:.2e}", flush=True)
+            rel = rel2
+        assert rel < TOL, (t, rel)
+        worst = max(worst, rel)
+    return worst
+
+
 @pytest.mark.timeout(600)
 def test_rollout_block_gradients_match_fd():
     """The differentiable-rollout configuration: 32^3 block, 100 FP32 steps,
     checkpoint every 10, central FD (eps = 1e-4) of the f64 oracle along 64
     seeded random directions in (v0, stiffness, visc_k[0]).
 
-    The error of each direction is taken relative to the RMS of the 64
-    directional derivatives, not to the direction's own value: at this size
-    the f64 central difference itself moves by 1e-4..4e-4 of that scale
-    between eps = 1e-3 and 1e-6 (thousands of pairs cross the C0 kinks at
-    r = h per rollout), so a direction whose derivative cancels to ~10 % of
-    the scale can show a 2e-3 per-direction relative difference that is FD
-    noise, not adjoint error. PHY_FD_DIRECTIONS overrides the count."""
+    Each direction is held to 1e-3 of its own FD value (_fd_check). At this
+    size the f64 central difference itself moves by 1e-4..4e-4 of the RMS
+    directional derivative between eps = 1e-3 and 1e-6 (thousands of pairs
+    cross the C0 kinks at r = h per rollout), so a direction whose derivative
+    cancels to ~10 % of that scale can miss 1e-3 with eps = 1e-4 on FD noise
+    alone; such a direction is re-differenced with eps = 1e-5 and must pass
+    with it. PHY_FD_DIRECTIONS overrides the count."""
     import os
     ndir = int(os.environ.get("PHY_FD_DIRECTIONS", "64"))
     sc = scenes.rollout_block(32, seed=1)
@@ -182,13 +248,18 @@ def test_rollout_block_gradients_match_fd():
     L64 = loss(0.0, 0.0, 0.0)
     print(f"loss GPU {L:.10e} f64 {L64:.10e}", flush=True)
     assert abs(L - L64) / abs(L64) < 1e-5
-    eps = 1e-4
+    def direction(t):
+        r = np.random.default_rng(1000 + t)
+        return r.normal(size=(n, 3)), r.normal() * 10.0, r.normal() * 0.1
+
+    def fd_of(t, eps):
+        dv, dkk, dmm = direction(t)
+        return (loss(eps * dv, eps * dkk, eps * dmm) - loss(-eps * dv, -eps * dkk, -eps * dmm)) / (2 * eps)
+
     ads, fds = [], []
     for t in range(ndir):
-        r = np.random.default_rng(1000 + t)
-        dv = r.normal(size=(n, 3))
-        dkk, dmm = r.normal() * 10.0, r.normal() * 0.1
-        fd = (loss(eps * dv, eps * dkk, eps * dmm) - loss(-eps * dv, -eps * dkk, -eps * dmm)) / (2 * eps)
+        dv, dkk, dmm = direction(t)
+        fd = fd_of(t, 1e-4)
         ad = float((v0b * dv).sum() + dk * dkk + dmu * dmm)
         ads.append(ad)
         fds.append(fd)
@@ -198,16 +269,18 @@ def test_rollout_block_gradients_match_fd():
     err = np.abs(ads - fds) / scale
     print(f"{ndir} directions: max |adjoint - fd| / rms(fd) = {err.max():.2e}, "
           f"median per-direction relative {np.median(np.abs(ads - fds) / np.abs(fds)):.2e}", flush=True)
-    assert err.max() < TOL
+    worst = _fd_check(ads, fds, lambda t: fd_of(t, 1e-5))
+    print(f"worst per-direction relative error (after re-differencing) {worst:.2e}", flush=True)
 
 
 @pytest.mark.timeout(900)
 def test_inverse_design_rollout_gradients_match_fd():
     """The batched inverse-design configuration: 16 rollouts of 64x64x32
     particles with per-rollout viscosity, 200 FP32 steps, checkpoint every 20,
-    differentiated together in one batch. Rollout 0's dL/d stiffness,
-    dL/d visc_k[0] and one seeded v0 direction are checked against central FD
-    (eps = 1e-4) of the f64 oracle on that rollout alone."""
+    differentiated together in one batch. For rollouts 0, 7 and 15: dL/d
+    stiffness, dL/d visc_k[0] and three seeded v0 directions are checked
+    against central FD (eps = 1e-4) of the f64 oracle on that rollout alone,
+    each to 1e-3 of its own value."""
     ro = scenes.inverse_design_rollouts(16, seed=5)
     params = []
     for sc, mu in ro:
@@ -233,33 +306,37 @@ def test_inverse_design_rollout_gradients_match_fd():
     _, v0b, dk, dmu = b.tape_backward(xbar, None, rhobar)
     b.tape_end()
 
-    sc, mu0 = ro[0]
-    n = counts[0]
-    op = scenes.apply_scene(O.Params.defaults(), sc)
-    p64, v64 = sc.pos.astype(np.float64), sc.vel.astype(np.float64)
-    m64 = np.full(n, float(np.float32(sc.mass)))
+    for s in (0, 7, 15):
+        sc, mu0 = ro[s]
+        n = counts[s]
+        op = scenes.apply_scene(O.Params.defaults(), sc)
+        p64, v64 = sc.pos.astype(np.float64), sc.vel.astype(np.float64)
+        m64 = np.full(n, float(np.float32(sc.mass)))
 
-    def loss(dv, dkk, dmm):
-        q = op.copy()
-        q.stiffness = op.stiffness + dkk
-        q.set_visc([mu0 + dmm], [1.0])
-        w = O.World(q)
-        w.set_particles(p64, v64 + dv, m64)
-        w.step(DT32, 200)
-        return scenes.rollout_loss_cotangents(w.get("pos"), w.get("rho"))[0]
+        def loss(dv, dkk, dmm):
+            q = op.copy()
+            q.stiffness = op.stiffness + dkk
+            q.set_visc([mu0 + dmm], [1.0])
+            w = O.World(q)
+            w.set_particles(p64, v64 + dv, m64)
+            w.step(DT32, 200)
+            return scenes.rollout_loss_cotangents(w.get("pos"), w.get("rho"))[0]
 
-    L64 = loss(0.0, 0.0, 0.0)
-    print(f"rollout 0: loss GPU {losses[0]:.10e} f64 {L64:.10e}", flush=True)
-    # independent FP32 and f64 trajectories over 200 steps: the loss agrees to
-    # ~9e-6 (the density term is the sensitive one); bound it at 1e-4
-    assert abs(losses[0] - L64) / abs(L64) < 1e-4
-    eps = 1e-4
-    dv = np.random.default_rng(77).normal(size=(n, 3))
-    cases = [("stiffness", 0.0, 1.0, 0.0, float(dk[0])),
-             ("visc_k[0]", 0.0, 0.0, 1.0, float(dmu[0])),
-             ("v0 direction", 1.0, 0.0, 0.0, float((v0b[off[0]:off[1]].astype(np.float64) * dv).sum()))]
-    for name, cv, ck, cm, ad in cases:
-        fd = (loss(eps * cv * dv, eps * ck, eps * cm) - loss(-eps * cv * dv, -eps * ck, -eps * cm)) / (2 * eps)
-        rel = abs(ad - fd) / abs(fd)
-        print(f"{name}: adjoint {ad:.8e} fd {fd:.8e} rel {rel:.2e}", flush=True)
-        assert rel < TOL
+        L64 = loss(0.0, 0.0, 0.0)
+        print(f"rollout {s}: loss GPU {losses[s]:.10e} f64 {L64:.10e}", flush=True)
+        # independent FP32 and f64 trajectories over 200 steps: the loss agrees
+        # to ~9e-6 (the density term is the sensitive one); bound it at 1e-4
+        assert abs(losses[s] - L64) / abs(L64) < 1e-4
+        eps = 1e-4
+        vb = v0b[off[s]:off[s + 1]].astype(np.float64)
+        cases = [("stiffness", None, 1.0, 0.0, float(dk[s])),
+                 ("visc_k[0]", None, 0.0, 1.0, float(dmu[s]))]
+        for t in range(3):
+            dv = np.random.default_rng(77 + 100 * s + t).normal(size=(n, 3))
+            cases.append((f"v0 direction {t}", dv, 0.0, 0.0, float((vb * dv).sum())))
+        for name, dv, ck, cm, ad in cases:
+            d = 0.0 if dv is None else eps * dv
+            fd = (loss(d, eps * ck, eps * cm) - loss(-d, -eps * ck, -eps * cm)) / (2 * eps)
+            rel = abs(ad - fd) / abs(fd)
+            print(f"rollout {s} {name}: adjoint {ad:.8e} fd {fd:.8e} rel {rel:.2e}", flush=True)
+            assert rel < TOL

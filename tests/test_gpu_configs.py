@@ -1,0 +1,252 @@
+"""The benchmark configurations at their real sizes, and the reference's own
+invariant and extreme-parameter tests, on the HIP path through the C ABI.
+
+  README dam break   343 particles, 200 steps (world.rs:79-93): closed-form
+                     trajectory, one-step parity along it, neighbour sets
+  1M column          first 20 steps against independent f64 oracle runs, and
+                     one-step parity from the collapsed sheet (step 300)
+  256-scene batch    4.6M particles: per-scene one-step parity and neighbour
+                     sets of every scene
+  stability.rs:55-106            closed system, 1000 steps: |dKE| <= 1e-2,
+                                 |p| <= 1e-3
+  extreme_parameters.rs:132-151  dt 0.1 / 0.5 / 1.0: no silent NaN, same
+                                 outcome as the oracle
+  extreme_parameters.rs:259-270  stiffness 1e4 (viscosity 100 is never read
+                                 on this path): no silent NaN
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import parity as P
+from physics_amd import ffi, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _print(tag, err):
+    print(tag, " ".join(f"{k}={v:.2e}" for k, v in err.items()), flush=True)
+
+
+def _readme_world():
+    g = ffi.FluidWorld(ffi.PhySphParams.defaults())
+    g.fill_box((-1, 1, -1), (1, 3, 1), 0.3, 0.1)
+    return g
+
+
+# ---- README dam break ------------------------------------
+
+def _closed_form_y(y0, steps, dt=0.01, g=-9.81, lo=-5.0, damp=0.4):
+    """Semi-implicit Euler free fall with the floor reflection, in f64."""
+    y, v, out = y0.astype(np.float64).copy(), np.zeros_like(y0, dtype=np.float64), []
+    for _ in range(steps):
+        v = v + g * dt
+        y = y + v * dt
+        hit = y < lo
+        y = np.where(hit, lo, y)
+        v = np.where(hit, -v * damp, v)
+        out.append((y.copy(), v.copy()))
+    return out
+
+
+def test_readme_dam_break_on_the_gpu():
+    g = _readme_world()
+    g.set_keep_aux(True)
+    assert len(g) == 343
+    x0 = g.get("pos")
+    # spacing 0.3 > h: no pair ever interacts while the lattice keeps its shape
+    traj = _closed_form_y(x0[:, 1], 200)
+    o = O.World(O.Params.defaults())
+    c6 = 315.0 / (64 * math.pi * 0.2 ** 9)
+    m = g.get("mass")
+    worst, checked = {}, 0
+    for k in range(200):
+        if k in (0, 100, 199):
+            dc, _, _ = g.gather_stats("density")
+            fc, _, _ = g.gather_stats("force")
+            assert (dc == 1).all() and (fc == 0).all(), k    # only self / none
+        o.set_particles(g.get("pos"), g.get("vel"), m)
+        xg = g.get("pos")
+        g.step(0.01)                       # the README call; the device rounds dt to FP32
+        o.step(P.DT32)
+        err = P.compare_step(g, o, xg)
+        for f, v in err.items():
+            worst[f] = max(worst.get(f, 0.0), v)
+        # density of an isolated particle: m W(0), rounded once to FP32
+        np.testing.assert_allclose(g.get("rho"), m * c6 * 0.04 ** 3, rtol=1.2e-7)
+        y_cf, v_cf = traj[k]
+        y = g.get("pos")[:, 1]
+        assert np.abs(y - y_cf).max() <= 1e-5 * 5.0, k
+        assert np.abs(g.get("vel")[:, 1] - v_cf).max() <= 1e-5 * 9.81 * 2, k
+        assert np.array_equal(g.get("pos")[:, [0, 2]], x0[:, [0, 2]])
+        checked += 1
+    _print("README dam break, worst one-step error over 200 steps", worst)
+    for f, v in worst.items():
+        assert v <= P.TOL_STEP, (f, v)
+    # the column reached the floor and bounced
+    assert (g.get("pos")[:, 1] == -5.0).any() or (g.get("vel")[:, 1] > 0).any()
+
+
+# ---- the 1M column ------------------------------------------
+
+def test_full_size_column_first_20_steps_against_the_oracle():
+    sc = scenes.dam_break((100, 100, 100), seed=1)
+    g, o = P.make_pair(sc.pos, mass=sc.mass, bounds=(sc.bounds_min, sc.bounds_max))
+    for k in range(20):
+        g.step(P.DT32)
+        o.step(P.DT32)
+        if k in (0, 9, 19):
+            e = {f: P.normwise(g.get(f), o.get(f)) for f in ("pos", "vel", "rho", "p")}
+            _print(f"1M independent trajectories, step {k + 1}", e)
+    # independent FP32 and f64 runs: rounding differences grow along the
+    # trajectory; the 20th step stays within the bounds of the 20^3 column's
+    # test (test_gpu_parity.py)
+    assert e["pos"] < 1e-5 and e["vel"] < 1e-3 and e["rho"] < 1e-4, e
+
+
+def test_full_size_column_one_step_parity_in_the_collapsed_sheet():
+    sc = scenes.dam_break((100, 100, 100), seed=1)
+    gp, op = P.params_pair(bounds=(sc.bounds_min, sc.bounds_max))
+    g = ffi.FluidWorld(gp)
+    g.set_particles_f32(sc.pos, None, np.full(len(sc.pos), sc.mass, np.float32))
+    g.set_keep_aux(True)
+    g.step(P.DT32, 300)
+    o = O.World(op)
+    P.sync_oracle_to_gpu(g, o)
+    c, _, _ = o.force_neighbor_stats()
+    print(f"collapsed sheet: {c.mean():.0f} force neighbours per particle (max {c.max()})", flush=True)
+    assert c.mean() > 300
+    x0 = P.step_both(g, o)
+    err = P.compare_step(g, o, x0)
+    _print("1M one step at step 301", err)
+    for f, v in err.items():
+        assert v <= P.TOL_STEP, (f, v)
+
+
+# ---- 256 batched scenes at full size -------------------------
+
+@pytest.mark.timeout(600)
+def test_full_size_batch_per_scene_parity_and_sets():
+    scs = scenes.batch_scenes(256, seed=7)
+    counts = [len(sc.pos) for sc in scs]
+    assert 4_000_000 < sum(counts) < 5_200_000
+    b = ffi.FluidBatch([scenes.apply_scene(ffi.PhySphParams.defaults(), sc) for sc in scs])
+    b.set_particles_f32(counts, np.concatenate([sc.pos for sc in scs]),
+                        np.concatenate([sc.vel for sc in scs]),
+                        np.concatenate([np.full(len(sc.pos), sc.mass, np.float32) for sc in scs]))
+    b.step(P.DT32, 40)
+    off = b.offsets()
+    x0, v0, m = b.get("pos"), b.get("vel"), b.get("mass")
+    sets = {w: b.gather_stats(w) for w in ("density", "force")}
+    b.step(P.DT32)
+    x1, v1, r1 = b.get("pos"), b.get("vel"), b.get("rho")
+    worst = {"rho": 0.0, "vel": 0.0, "dx": 0.0}
+    for s, sc in enumerate(scs):
+        lo, hi = off[s], off[s + 1]
+        o = O.World(scenes.apply_scene(O.Params.defaults(), sc))
+        o.set_particles(x0[lo:hi], v0[lo:hi], m[lo:hi])
+        for w, ofn in (("density", o.neighbor_stats), ("force", o.force_neighbor_stats)):
+            oc, os_, ox = ofn()
+            gc, gs, gx = sets[w]
+            assert np.array_equal(gc[lo:hi], oc) and np.array_equal(gs[lo:hi], os_) and \
+                np.array_equal(gx[lo:hi], ox), (s, w)
+        o.step(P.DT32)
+        xo = o.get("pos")
+        ulp = np.abs(np.spacing(xo.astype(np.float32)).astype(np.float64))
+        exc = np.maximum(np.abs(x1[lo:hi] - xo) - ulp, 0.0)
+        disp = np.sqrt(((xo - x0[lo:hi]) ** 2).sum(1)).max()
+        e = {"rho": P.normwise(r1[lo:hi], o.get("rho")), "vel": P.normwise(v1[lo:hi], o.get("vel")),
+             "dx": float(np.sqrt((exc ** 2).sum(1)).max() / disp)}
+        for f in worst:
+            worst[f] = max(worst[f], e[f])
+    _print(f"256 scenes ({sum(counts)} particles), worst one-step error", worst)
+    for f, v in worst.items():
+        assert v <= P.TOL_STEP, (f, v)
+    b.close()
+
+
+# ---- the reference's invariants and extreme parameters ------------------------------
+
+def _sph_world_with(gp, n):
+    # crates/phy-demo/tests/stability.rs:22-37: spacing from n in a 10 m box
+    half = 5.0
+    spacing = (8.0 * half ** 3 / n) ** (1.0 / 3.0)
+    gp.h = spacing * 2.0
+    g = ffi.FluidWorld(gp)
+    g.fill_box((-half, 1.0, -half), (half, 1.0 + 2 * half, half), spacing, 0.1)
+    return g
+
+
+def test_closed_system_conserves_energy_and_momentum():   # stability.rs:55-106
+    gp = ffi.PhySphParams.defaults()
+    gp.gravity[1] = 0.0
+    for a, (lo, hi) in enumerate(((-7.0, 7.0), (-1.0, 13.0), (-7.0, 7.0))):
+        gp.bounds_min[a], gp.bounds_max[a] = lo, hi
+    g = _sph_world_with(gp, 800)
+    d0 = g.diagnostics()
+    e0 = d0[4]
+    for k in range(1000):
+        assert g.step_checked(0.01) == 0, k
+    d1 = g.diagnostics()
+    p1 = float(np.linalg.norm(d1[1:4]))
+    print(f"closed system, 1000 steps on the GPU: e0={e0:.3e} e1={d1[4]:.3e} |p1|={p1:.3e} "
+          f"(mass {d1[0]:.6f})", flush=True)
+    assert abs(d1[4] - e0) <= 1e-2
+    assert p1 <= 1e-3
+    assert d1[0] == d0[0]
+
+
+def _run_checked(g, dt, steps):
+    # extreme_parameters.rs:100-121: every Ok step leaves a finite world; an
+    # Err (the watchdog) ends the run
+    ok = 0
+    for step in range(steps):
+        rc = g.step_checked(dt)
+        if rc != 0:
+            assert rc == 2
+            return ok, step
+        ok += 1
+        assert np.isfinite(g.get("pos")).all() and np.isfinite(g.get("vel")).all()
+    return ok, None
+
+
+def _oracle_run_checked(op, x, m, dt, steps):
+    o = O.World(op)
+    o.set_particles(x, None, m)
+    ok = 0
+    for step in range(steps):
+        o.step(dt)
+        if o.validate() != 0:
+            return ok, step
+        ok += 1
+    return ok, None
+
+
+@pytest.mark.parametrize("dt", [0.1, 0.5, 1.0])
+def test_extreme_dt_no_silent_nan(dt):                     # extreme_parameters.rs:132-151
+    g = _readme_world()
+    x, m = g.get("pos"), g.get("mass")
+    res = _run_checked(g, dt, 50)
+    ref = _oracle_run_checked(O.Params.defaults(), x, m, float(np.float32(dt)), 50)
+    print(f"dt {dt}: GPU ok={res[0]} diverged_at={res[1]}; oracle ok={ref[0]} diverged_at={ref[1]}",
+          flush=True)
+    assert res == ref
+
+
+def test_extreme_stiffness_no_silent_nan():               # extreme_parameters.rs:259-270
+    gp = ffi.PhySphParams.defaults()
+    gp.stiffness = 1e4
+    gp.viscosity = 100.0           # SphParams::viscosity: never read by the step (world.rs:260, 324)
+    g = ffi.FluidWorld(gp)
+    g.fill_box((-1, 1, -1), (1, 3, 1), 0.3, 0.1)
+    x, m = g.get("pos"), g.get("mass")
+    res = _run_checked(g, 0.01, 50)
+    op = O.Params.defaults()
+    op.stiffness = 1e4
+    op.viscosity = 100.0
+    ref = _oracle_run_checked(op, x, m, P.DT32, 50)
+    print(f"stiffness 1e4: GPU ok={res[0]} diverged_at={res[1]}; oracle ok={ref[0]} "
+          f"diverged_at={ref[1]}", flush=True)
+    assert res == ref

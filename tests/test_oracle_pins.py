@@ -11,6 +11,10 @@ mistake fails:
   * density          brute force, single-particle closed form, lattice interior
   * forces           hand-derived two-particle closed forms, pairwise
                      antisymmetry (momentum), lattice symmetry
+  * power law        two-particle closed form with unequal masses, velocity
+                     scaling, Galilean invariance, the floor at rest, the
+                     pair identity, the out-of-table material rule
+  * force sets       O(N^2) brute force of the force-pass predicate
   * integrate/bounds README dam break (343 free particles) in closed form
   * whole step       the reference's own test assertions (crates/phy-fluid/src/
                      sph/tests.rs, crates/phy-demo/tests/stability.rs)
@@ -147,6 +151,48 @@ def test_neighbor_sets_equal_brute_force(seed, jit, origin):
     c, s, x = w.neighbor_stats()
     assert np.array_equal(c, adj.sum(1))
     assert np.array_equal(s, (adj * np.arange(len(pos))[None, :]).sum(1).astype(np.uint64))
+
+
+def _mix64(j):
+    z = (np.asarray(j, dtype=np.uint64) + np.uint64(0x9E3779B97F4A7C15))
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def _stats_of(adj):
+    idx = np.arange(adj.shape[1], dtype=np.uint64)
+    h = _mix64(idx)
+    c = adj.sum(1)
+    s = np.array([idx[row].sum(dtype=np.uint64) for row in adj], dtype=np.uint64)
+    x = np.array([np.bitwise_xor.reduce(h[row]) if row.any() else np.uint64(0) for row in adj],
+                 dtype=np.uint64)
+    return c, s, x
+
+
+def test_force_neighbor_sets_equal_brute_force():
+    # world.rs:292-299: j != i, skip r >= h and r <= 0, so coincident points
+    # drop out and a NaN distance stays in (for candidates of the 27 cells: a
+    # NaN coordinate has reference key 0 on that axis, grid.rs:29-31)
+    with np.errstate(invalid="ignore", over="ignore"):
+        pos = _jittered_block(6, 0.1, 0.3, 8, (-0.3, -0.3, -0.3))
+        pos = np.concatenate([pos, pos[[5, 5, 40]], [[np.nan, 0.02, 0.03], [0.05, 0.01, np.nan]]])
+        w = O.World()
+        w.set_particles(pos, mass=1.0)
+        d = pos[None, :, :] - pos[:, None, :]
+        s2 = (d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]
+        r = np.sqrt(s2)
+        key = np.where(np.isnan(pos), 0, np.floor(np.nan_to_num(pos) * 5.0)).astype(np.int64)
+        adjacent = (np.abs(key[:, None, :] - key[None, :, :]) <= 1).all(-1)
+        adj = adjacent & ~(r >= H) & ~(r <= 0.0) & ~np.eye(len(pos), dtype=bool)
+    c, s, x = w.force_neighbor_stats()
+    bc, bs, bx = _stats_of(adj)
+    assert np.array_equal(c, bc) and np.array_equal(s, bs) and np.array_equal(x, bx)
+    n0 = len(pos) - 5
+    assert c[n0 + 3] > 0                       # the NaN particle is a force neighbour
+    dc, _, _ = w.neighbor_stats()
+    assert dc[n0 + 3] == 0                     # ... but no density neighbour
+    assert c[5] == c[n0] and dc[5] == dc[n0]   # coincident copies
 
 
 def test_density_matches_brute_force_and_single_particle():
@@ -353,6 +399,90 @@ def test_ref_non_newtonian_shear_thinning_and_thickening():  # tests.rs:393-404
     assert mt < mn < mk
 
 
+# ---- the power-law branch (world.rs:313-327) ---------------------------------
+#
+# shear_i = sum_j |v_j - v_i| / (r + 1e-4) * (m_j / rho_j), mu_i = k_m *
+# max'(shear_i, shear_min)^(n_m - 1) with m = material_i if it indexes visc_k,
+# else 0. Pinned by a two-particle closed form with unequal masses (which of
+# m_i / rho_i and m_j / rho_j enters), relations the formula implies without
+# restating it (velocity scaling, Galilean invariance, the floor at rest, the
+# pair identity through the oracle's own densities), and the reference's
+# out-of-table material rule. max'(a, b) = a if a > b else b equals the ">="
+# form for every input (equal arguments give the same value either way; NaN
+# fails both), so that variant is not a distinct mistake.
+
+def _pl_world(pos, vel, mass, k=(3.5,), n=(1.0,), shear_min=0.01, material=None):
+    p = _test_params()
+    p.gravity[1] = 0.0
+    p.shear_min = shear_min
+    p.set_visc(list(k), list(n))
+    w = O.World(p)
+    w.set_particles(pos, vel=vel, mass=mass, material=material)
+    w.step(1e-4)
+    return w
+
+
+@pytest.mark.parametrize("nexp", [0.7, 1.3])
+@pytest.mark.parametrize("r", [0.03, 0.15])
+def test_power_law_two_particle_closed_form(nexp, r):
+    ma, mb, u, k = 0.05, 0.08, 0.9, 3.5
+    w = _pl_world([[0, 0, 0], [r, 0, 0]], [[0, -u, 0], [0, u, 0]], [ma, mb], k=(k,), n=(nexp,))
+    c6 = 315.0 / (64 * math.pi * H ** 9)
+    W = lambda q: c6 * (H * H - q * q) ** 3
+    rho_a = ma * W(0.0) + mb * W(r)
+    rho_b = mb * W(0.0) + ma * W(r)
+    sh_a = 2 * u / (r + 1e-4) * (mb / rho_b)
+    sh_b = 2 * u / (r + 1e-4) * (ma / rho_a)
+    mu = w.get("mu_eff")
+    assert mu[0] == pytest.approx(k * max(sh_a, 0.01) ** (nexp - 1.0), rel=1e-12)
+    assert mu[1] == pytest.approx(k * max(sh_b, 0.01) ** (nexp - 1.0), rel=1e-12)
+
+
+def _pl_block(seed):
+    pos = _jittered_block(4, 0.07, 0.3, seed)
+    n = len(pos)
+    vel = scenes.rng_f64(seed + 50, 3 * n).reshape(n, 3) - 0.5
+    mass = 0.03 + 0.04 * scenes.rng_f64(seed + 90, n)
+    return pos, vel, mass
+
+
+def test_power_law_scales_with_velocity_and_is_galilean():
+    nexp, lam = 0.6, 3.0
+    pos, vel, mass = _pl_block(3)
+    mu1 = _pl_world(pos, vel, mass, n=(nexp,), shear_min=1e-6).get("mu_eff")
+    mu2 = _pl_world(pos, lam * vel, mass, n=(nexp,), shear_min=1e-6).get("mu_eff")
+    mu3 = _pl_world(pos, vel + np.array([4.0, -2.0, 1.0]), mass, n=(nexp,), shear_min=1e-6).get("mu_eff")
+    # shear is linear in the velocity differences and blind to a common velocity
+    np.testing.assert_allclose(mu2 / mu1, lam ** (nexp - 1.0), rtol=1e-12)
+    np.testing.assert_allclose(mu3, mu1, rtol=1e-12)
+
+
+def test_power_law_floor_at_rest_and_pair_identity():
+    nexp, k, smin = 1.4, 2.0, 0.03
+    pos, _, mass = _pl_block(4)
+    mu = _pl_world(pos, np.zeros_like(pos), mass, k=(k,), n=(nexp,), shear_min=smin).get("mu_eff")
+    assert (mu == k * smin ** (nexp - 1.0)).all()
+    # two particles: shear_a rho_b / m_b = shear_b rho_a / m_a = |dv| / (r + 1e-4),
+    # with the densities the oracle itself computed
+    ma, mb, r, u = 0.05, 0.11, 0.08, 1.3
+    w = _pl_world([[0, 0, 0], [0, r, 0]], [[u, 0, 0], [-u, 0, 0]], [ma, mb], k=(k,), n=(nexp,),
+                  shear_min=1e-9)
+    rho = w.get("rho")
+    sh = (w.get("mu_eff") / k) ** (1.0 / (nexp - 1.0))
+    want = 2 * u / (r + 1e-4)
+    assert sh[0] * rho[1] / mb == pytest.approx(want, rel=1e-11)
+    assert sh[1] * rho[0] / ma == pytest.approx(want, rel=1e-11)
+
+
+def test_power_law_material_outside_the_table_uses_material_0():
+    pos, vel, mass = _pl_block(5)
+    n = len(pos)
+    mat = np.where(np.arange(n) % 2 == 0, 7, 1).astype(np.uint64)
+    w = _pl_world(pos, vel, mass, k=(3.5, 10.0), n=(0.8, 1.2), material=mat)
+    w0 = _pl_world(pos, vel, mass, k=(3.5, 10.0), n=(0.8, 1.2), material=np.where(mat == 7, 0, 1))
+    assert np.array_equal(w.get("mu_eff"), w0.get("mu_eff"))
+
+
 def test_ref_material_tag_distinguishes_viscosity():         # tests.rs:407-427
     p = _test_params()
     p.gravity[1] = 0.0
@@ -433,6 +563,11 @@ def _checks():
     run("interior", test_lattice_mass_survey_values_and_interior_density)
     run("c1", test_readme_dam_break_is_closed_form)
     run("material", test_ref_material_tag_distinguishes_viscosity)
+    run("power_law_closed_form", lambda: [test_power_law_two_particle_closed_form(n, r)
+                                          for n in (0.7, 1.3) for r in (0.03, 0.15)])
+    run("power_law_scaling", test_power_law_scales_with_velocity_and_is_galilean)
+    run("power_law_floor", test_power_law_floor_at_rest_and_pair_identity)
+    run("power_law_material", test_power_law_material_outside_the_table_uses_material_0)
 
     def bounce():
         # a particle driven through the floor reflects with damping 0.4
@@ -454,6 +589,21 @@ def _checks():
 
     run("clamp", low_density_pressure)
     return out
+
+
+POWER_LAW_MUTATIONS = ("shear_no_reps", "shear_mass_i", "power_exp_n", "shear_no_floor",
+                       "shear_dv_sq", "mat_oor_last")
+
+
+@pytest.mark.parametrize("mutation", POWER_LAW_MUTATIONS)
+def test_power_law_mutations_pass_the_ordering_test(mutation):
+    # the reference's own power-law test (thin < Newtonian < thick) cannot see
+    # these mistakes; the checks above must (test_mutations_are_caught)
+    O.set_mutation(mutation)
+    try:
+        test_ref_non_newtonian_shear_thinning_and_thickening()
+    finally:
+        O.set_mutation("none")
 
 
 @pytest.mark.parametrize("mutation", [m for m in O.MUTATIONS if m not in ("none", "self_in_force")])
