@@ -393,6 +393,13 @@ int32_t phy_fluid_flat_export(PhyWorldHandle *w, PhyFlatInfo *info, float *pos4,
  * bytes. 0 ok, -1 error. */
 int32_t phy_device_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
 
+/* Device memory held by this library's own buffers over every world and
+ * batch of the process: bytes in use now and the high-water mark since the
+ * last phy_device_mem_reset_peak (which sets it to the bytes in use). Either
+ * pointer may be NULL. 0 ok, -1 error. */
+int32_t phy_device_mem_stats(uint64_t *in_use, uint64_t *peak);
+int32_t phy_device_mem_reset_peak(void);
+
 /* ---- Batches of independent scenes (no counterpart in the reference) ----
  * A batch holds n_scenes independent FluidWorlds of uneven particle counts in
  * one set of device arrays: scene s owns particles [offsets[s], offsets[s+1])

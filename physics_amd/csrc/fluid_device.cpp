@@ -1,6 +1,8 @@
 // fluid_device.cpp — DeviceFluid: buffers, uploads and the step sequence.
 #include "fluid_device.h"
 
+#include <atomic>
+
 #include <math.h>
 #include <string.h>
 
@@ -13,6 +15,20 @@
 #include <sstream>
 
 namespace phy {
+
+namespace {
+std::atomic<unsigned long long> g_mem_in_use{0}, g_mem_peak{0};
+}
+
+void mem_note(long long delta) {
+    const unsigned long long now = g_mem_in_use.fetch_add((unsigned long long)delta) + (unsigned long long)delta;
+    unsigned long long pk = g_mem_peak.load();
+    while (delta > 0 && now > pk && !g_mem_peak.compare_exchange_weak(pk, now)) {
+    }
+}
+unsigned long long mem_in_use() { return g_mem_in_use.load(); }
+unsigned long long mem_peak() { return g_mem_peak.load(); }
+void mem_reset_peak() { g_mem_peak.store(g_mem_in_use.load()); }
 
 void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));

@@ -27,6 +27,13 @@ struct HipError : std::runtime_error {
 
 void hip_check(hipError_t e, const char* what);
 
+// Bytes of device memory the library's buffers hold (every world and batch of
+// the process) and the largest value since the last reset_peak.
+void mem_note(long long delta_bytes);
+unsigned long long mem_in_use();
+unsigned long long mem_peak();
+void mem_reset_peak();
+
 // Device buffer of T elements (hipMalloc / hipFree).
 template <class T>
 class DevBuf {
@@ -42,9 +49,13 @@ class DevBuf {
         hip_check(hipMalloc(&p, (n ? n : 1) * sizeof(T)), "hipMalloc");
         p_ = (T*)p;
         cap_ = n;
+        mem_note((long long)((n ? n : 1) * sizeof(T)));
     }
     void release() {
-        if (p_) (void)hipFree(p_);
+        if (p_) {
+            (void)hipFree(p_);
+            mem_note(-(long long)((cap_ ? cap_ : 1) * sizeof(T)));
+        }
         p_ = nullptr;
         cap_ = 0;
     }

@@ -752,6 +752,21 @@ int32_t phy_device_mem_info(uint64_t* free_bytes, uint64_t* total_bytes) {
     });
 }
 
+int32_t phy_device_mem_stats(uint64_t* in_use, uint64_t* peak) {
+    return guard<int32_t>(-1, [&] {
+        if (in_use) *in_use = phy::mem_in_use();
+        if (peak) *peak = phy::mem_peak();
+        return 0;
+    });
+}
+
+int32_t phy_device_mem_reset_peak(void) {
+    return guard<int32_t>(-1, [&] {
+        phy::mem_reset_peak();
+        return 0;
+    });
+}
+
 // ---------------------------------------------------------------------------
 // batches
 

@@ -38,7 +38,8 @@ EXPORTS = [
     "phy_fluid_fill_box", "phy_fluid_lattice_mass", "phy_fluid_set_particles",
     "phy_fluid_set_particles_f32", "phy_fluid_get_field", "phy_fluid_get_field_f32",
     "phy_fluid_set_keep_aux", "phy_fluid_set_body_acc", "phy_fluid_diagnostics_f64",
-    "phy_fluid_gather_stats", "phy_fluid_batch_gather_stats",
+    "phy_fluid_gather_stats", "phy_fluid_batch_gather_stats", "phy_device_mem_stats",
+    "phy_device_mem_reset_peak",
     "phy_fluid_neighbor_stats", "phy_world_step_n", "phy_fluid_sync", "phy_fluid_profile_begin",
     "phy_fluid_profile_end", "phy_fluid_tape_begin", "phy_fluid_tape_steps", "phy_fluid_tape_backward",
     "phy_fluid_tape_end", "phy_fluid_set_graph", "phy_fluid_batch_create", "phy_fluid_batch_destroy",
@@ -249,6 +250,8 @@ def lib():
         "phy_fluid_batch_profile_begin": ([H], C.c_int32),
         "phy_fluid_batch_profile_end": ([H, dp, u32p], C.c_int32),
         "phy_device_mem_info": ([u64p, u64p], C.c_int32),
+        "phy_device_mem_stats": ([u64p, u64p], C.c_int32),
+        "phy_device_mem_reset_peak": ([], C.c_int32),
         "phy_fluid_get_tuning": ([H, C.POINTER(PhyTuning)], C.c_int32),
         "phy_fluid_set_tuning": ([H, C.POINTER(PhyTuning)], C.c_int32),
         "phy_fluid_autotune": ([H, C.c_double, C.c_uint32, C.c_char_p, C.POINTER(PhyTuning), dp],
@@ -645,6 +648,19 @@ def device_mem_info():
     if lib().phy_device_mem_info(C.byref(f), C.byref(t)) != 0:
         raise PhyError(last_error())
     return int(f.value), int(t.value)
+
+
+def device_mem_stats():
+    """(in_use, peak) bytes of device memory held by the library's buffers."""
+    a, b = C.c_uint64(0), C.c_uint64(0)
+    if lib().phy_device_mem_stats(C.byref(a), C.byref(b)) != 0:
+        raise PhyError(last_error())
+    return int(a.value), int(b.value)
+
+
+def device_mem_reset_peak() -> None:
+    if lib().phy_device_mem_reset_peak() != 0:
+        raise PhyError(last_error())
 
 
 def _u32(a):
