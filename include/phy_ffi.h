@@ -267,6 +267,22 @@ int32_t phy_fluid_sample_velocity(PhyWorldHandle *w, const double *q3, uint32_t 
  * 0 ok, -1 error. */
 int32_t phy_fluid_diagnostics_f64(PhyWorldHandle *w, double *out5);
 
+/* As phy_fluid_diagnostics_f64 for one scene held in caller-owned DEVICE
+ * arrays, allocating nothing (no counterpart in the reference): for callers
+ * that cannot allocate device memory, such as code under HIP graph capture
+ * and tight serving loops. posm4: n x (x, y, z, mass) floats, vel4: n x (vx,
+ * vy, vz, unused) floats, the library's particle layout; out5: 5 doubles in
+ * device memory (mass, momentum x y z, kinetic energy), written
+ * asynchronously on `stream` (a hipStream_t; NULL for the null stream). The
+ * sums are bit-identical to phy_fluid_diagnostics_f64 of a world holding the
+ * same particles in the same order. At most 2^24 particles per call (the
+ * per-tile FP64 partials live in a fixed 2.5 MiB device array that each GPU
+ * holds until the process ends); larger n returns -1 without launching.
+ * Calls that overlap on one GPU share that array and race: serialise them
+ * (one stream, or events). 0 ok (launched), -1 error. */
+int32_t phy_fluid_diagnostics_f64_noalloc(const float *posm4, const float *vel4, uint64_t n, double *out5,
+                                          void *stream);
+
 /* Density-predicate neighbour statistics of the current state (index order):
  * neighbour count (self included), sum of neighbour indices and xor of their
  * SplitMix64 hashes. For exact neighbour-set comparison. 0 ok, -1 error. */

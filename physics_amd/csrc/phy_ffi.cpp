@@ -752,6 +752,19 @@ int32_t phy_device_mem_info(uint64_t* free_bytes, uint64_t* total_bytes) {
     });
 }
 
+int32_t phy_fluid_diagnostics_f64_noalloc(const float* posm4, const float* vel4, uint64_t n, double* out5,
+                                          void* stream) {
+    return guard<int32_t>(-1, [&] {
+        if (n > phy::kDiagNoAllocMaxParticles)
+            throw std::invalid_argument("phy_fluid_diagnostics_f64_noalloc: n exceeds 2^24 particles");
+        if (!out5 || (n > 0 && (!posm4 || !vel4)))
+            throw std::invalid_argument("phy_fluid_diagnostics_f64_noalloc: null pointer");
+        phy::launch_diag_noalloc(static_cast<hipStream_t>(stream), n, reinterpret_cast<const float4*>(posm4),
+                                 reinterpret_cast<const float4*>(vel4), out5);
+        return 0;
+    });
+}
+
 int32_t phy_device_mem_stats(uint64_t* in_use, uint64_t* peak) {
     return guard<int32_t>(-1, [&] {
         if (in_use) *in_use = phy::mem_in_use();

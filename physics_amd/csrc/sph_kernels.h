@@ -142,6 +142,16 @@ void launch_density(hipStream_t st, const SceneTable& T, const float4* posm, con
                     float4* velm, float* mp, const Tuning& tu);
 void launch_force(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a, bool shear,
                   const Tuning& tu);
+// FP64 diagnostics of one scene held in caller-owned device arrays (posm:
+// x, y, z, mass; vel: vx, vy, vz, unused), with no allocation: the per-tile
+// partials live in a file-scope device array of kDiagNoAllocMaxTiles tiles.
+// out5 (device) receives mass, momentum x y z, kinetic energy, bit-identical
+// to launch_diag on a one-scene world holding the same arrays in the same
+// order. Calls on one GPU share the array: overlapping calls race.
+constexpr uint64_t kDiagNoAllocMaxParticles = 1ull << 24;
+constexpr uint32_t kDiagNoAllocMaxTiles = (uint32_t)(kDiagNoAllocMaxParticles / 256);
+constexpr int kMaxDevices = 64;
+void launch_diag_noalloc(hipStream_t st, uint64_t n, const float4* posm, const float4* vel, double* out5);
 void launch_diag(hipStream_t st, const SceneTable& T, const float4* posm, const float4* vel,
                  double* partial, double* out);
 void launch_unsort4(hipStream_t st, int n, const uint32_t* id, const float4* in, float4* out);

@@ -26,6 +26,10 @@
 #include <stdint.h>
 #include <math.h>
 
+#include <mutex>
+#include <stdexcept>
+#include <string>
+
 #include "sph_types.h"
 #include "sph_device.h"
 #include "sph_couple.h"
@@ -764,15 +768,12 @@ __device__ __forceinline__ double block_tree_sum(double v, double* sh) {
     return r;
 }
 
-__global__ __launch_bounds__(256) void k_diag_partial(SceneTable T, const float4* __restrict__ posm,
-                                                      const float4* __restrict__ vel,
-                                                      double* partial) {
-    __shared__ double sh[256];
-    const TileLoc L = tile_loc(T);
+// The five sums of one tile (256 particles, one per lane) into partial[t].
+__device__ __forceinline__ void diag_tile(bool act, const float4& pm, const float4& v, size_t t,
+                                          double* partial, double* sh) {
     double acc[5] = {0, 0, 0, 0, 0};
-    if (L.act) {
-        const double m = posm[L.i].w;
-        const float4 v = vel[L.i];
+    if (act) {
+        const double m = pm.w;
         const double vx = v.x, vy = v.y, vz = v.z;
         acc[0] = m;
         acc[1] = vx * m;
@@ -781,9 +782,34 @@ __global__ __launch_bounds__(256) void k_diag_partial(SceneTable T, const float4
         acc[4] = (((vx * vx + vy * vy) + vz * vz) * m) / 2.0;
     }
     for (int k = 0; k < 5; ++k) {
-        const double t = block_tree_sum(acc[k], sh);
-        if (threadIdx.x == 0) partial[(size_t)L.t * 5 + k] = t;
+        const double r = block_tree_sum(acc[k], sh);
+        if (threadIdx.x == 0) partial[t * 5 + k] = r;
     }
+}
+
+// Tiles tile0 .. tile0 + ntiles - 1 of partial summed in a fixed shape into out[0..5).
+__device__ __forceinline__ void diag_sum_tiles(const double* __restrict__ partial, uint32_t tile0,
+                                               uint32_t ntiles, double* out, double* sh) {
+    double acc[5] = {0, 0, 0, 0, 0};
+    for (uint32_t t = threadIdx.x; t < ntiles; t += 256)
+        for (int k = 0; k < 5; ++k) acc[k] += partial[(size_t)(tile0 + t) * 5 + k];
+    for (int k = 0; k < 5; ++k) {
+        const double r = block_tree_sum(acc[k], sh);
+        if (threadIdx.x == 0) out[k] = r;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_diag_partial(SceneTable T, const float4* __restrict__ posm,
+                                                      const float4* __restrict__ vel,
+                                                      double* partial) {
+    __shared__ double sh[256];
+    const TileLoc L = tile_loc(T);
+    float4 pm = make_float4(0.f, 0.f, 0.f, 0.f), v = pm;
+    if (L.act) {
+        pm = posm[L.i];
+        v = vel[L.i];
+    }
+    diag_tile(L.act, pm, v, (size_t)L.t, partial, sh);
 }
 
 // one workgroup per scene
@@ -791,13 +817,30 @@ __global__ __launch_bounds__(256) void k_diag_final(SceneTable T, const double* 
                                                     double* out) {
     __shared__ double sh[256];
     const SceneInfo si = T.info[blockIdx.x];
-    double acc[5] = {0, 0, 0, 0, 0};
-    for (uint32_t t = threadIdx.x; t < si.ntiles; t += 256)
-        for (int k = 0; k < 5; ++k) acc[k] += partial[(size_t)(si.tile0 + t) * 5 + k];
-    for (int k = 0; k < 5; ++k) {
-        const double r = block_tree_sum(acc[k], sh);
-        if (threadIdx.x == 0) out[(size_t)blockIdx.x * 5 + k] = r;
+    diag_sum_tiles(partial, si.tile0, si.ntiles, out + (size_t)blockIdx.x * 5, sh);
+}
+
+// The same two passes over caller-owned arrays of one scene (tile t = particles
+// 256 t .. 256 t + 255), so a scene's sums are bit-identical to k_diag_*'s.
+__global__ __launch_bounds__(256) void k_diag_partial_flat(uint32_t n, const float4* __restrict__ posm,
+                                                           const float4* __restrict__ vel,
+                                                           double* partial) {
+    __shared__ double sh[256];
+    const size_t t = blockIdx.x;
+    const size_t i = t * 256 + threadIdx.x;
+    const bool act = i < n;
+    float4 pm = make_float4(0.f, 0.f, 0.f, 0.f), v = pm;
+    if (act) {
+        pm = posm[i];
+        v = vel[i];
     }
+    diag_tile(act, pm, v, t, partial, sh);
+}
+
+__global__ __launch_bounds__(256) void k_diag_final_flat(uint32_t ntiles, const double* __restrict__ partial,
+                                                         double* out) {
+    __shared__ double sh[256];
+    diag_sum_tiles(partial, 0u, ntiles, out, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -990,6 +1033,50 @@ void launch_diag(hipStream_t st, const SceneTable& T, const float4* posm, const 
     if (T.ntiles > 0)
         hipLaunchKernelGGL(k_diag_partial, dim3(T.ntiles), dim3(256), 0, st, T, posm, vel, partial);
     hipLaunchKernelGGL(k_diag_final, dim3(T.nscenes), dim3(256), 0, st, T, partial, out);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 diagnostics without allocation: the tile partials live in a file-scope
+// device array instead of a buffer of the world. Each GPU holds its own copy;
+// host code reaches it only through the address hipGetSymbolAddress returns
+// for that device (the symbol's name is a host-side stand-in).
+
+static __device__ __align__(256) double scratch[(size_t)kDiagNoAllocMaxTiles * 5];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+}
+
+// The scratch array's address on the current device, looked up once per
+// device with that device current.
+static double* scratch_on_current_device() {
+    int dev = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("diagnostics_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) throw std::runtime_error("diagnostics_noalloc: no address for the scratch array");
+        g_scratch_ptr[dev] = p;
+    }
+    return static_cast<double*>(g_scratch_ptr[dev]);
+}
+
+void launch_diag_noalloc(hipStream_t st, uint64_t n, const float4* posm, const float4* vel, double* out5) {
+    if (n > kDiagNoAllocMaxParticles)
+        throw std::invalid_argument("diagnostics_noalloc: at most 2^24 particles per call");
+    double* part = scratch_on_current_device();
+    const uint32_t ntiles = (uint32_t)((n + 255) / 256);
+    if (ntiles > 0)
+        hipLaunchKernelGGL(k_diag_partial_flat, dim3(ntiles), dim3(256), 0, st, (uint32_t)n, posm, vel, part);
+    hipLaunchKernelGGL(k_diag_final_flat, dim3(1), dim3(256), 0, st, ntiles, (const double*)part, out5);
+    check(hipGetLastError(), "diagnostics_noalloc launch");
 }
 
 void launch_unsort4(hipStream_t st, int n, const uint32_t* id, const float4* in, float4* out) {

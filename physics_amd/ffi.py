@@ -39,7 +39,7 @@ EXPORTS = [
     "phy_fluid_set_particles_f32", "phy_fluid_get_field", "phy_fluid_get_field_f32",
     "phy_fluid_set_keep_aux", "phy_fluid_set_body_acc", "phy_fluid_diagnostics_f64",
     "phy_fluid_gather_stats", "phy_fluid_batch_gather_stats", "phy_device_mem_stats",
-    "phy_device_mem_reset_peak",
+    "phy_device_mem_reset_peak", "phy_fluid_diagnostics_f64_noalloc",
     "phy_fluid_neighbor_stats", "phy_world_step_n", "phy_fluid_sync", "phy_fluid_profile_begin",
     "phy_fluid_profile_end", "phy_fluid_tape_begin", "phy_fluid_tape_steps", "phy_fluid_tape_backward",
     "phy_fluid_tape_end", "phy_fluid_set_graph", "phy_fluid_batch_create", "phy_fluid_batch_destroy",
@@ -251,6 +251,8 @@ def lib():
         "phy_fluid_batch_profile_end": ([H, dp, u32p], C.c_int32),
         "phy_device_mem_info": ([u64p, u64p], C.c_int32),
         "phy_device_mem_stats": ([u64p, u64p], C.c_int32),
+        "phy_fluid_diagnostics_f64_noalloc": ([C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p],
+                                              C.c_int32),
         "phy_device_mem_reset_peak": ([], C.c_int32),
         "phy_fluid_get_tuning": ([H, C.POINTER(PhyTuning)], C.c_int32),
         "phy_fluid_set_tuning": ([H, C.POINTER(PhyTuning)], C.c_int32),
