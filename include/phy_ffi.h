@@ -26,7 +26,7 @@ extern "C" {
 /* phy_ffi.h:15-24 */
 #define PHY_FFI_ABI_VERSION 2
 #define PHY_FFI_VERSION_MAJOR 0
-#define PHY_FFI_VERSION_MINOR 2
+#define PHY_FFI_VERSION_MINOR 3
 #define PHY_FFI_VERSION_PATCH 0
 
 /* Opaque world handle (phy_ffi.h:28-30). */
@@ -349,17 +349,23 @@ int32_t phy_fluid_profile_end(PhyWorldHandle *w, double *ms4, uint32_t *steps);
  * Results are bit-identical to eager steps. 0 ok, -1 error. */
 int32_t phy_fluid_set_graph(PhyWorldHandle *w, int32_t on);
 
-/* Launch choices of the gather kernels: per-lane LDS neighbour-list length
- * of the density and of the force pass (8, 16 or 32; default 16), and the
- * launch form (split = 1: each pass is one launch for the waves whose lanes
- * share a grid cell plus one for the others; 0: one launch). The force pass's
- * compensated sums are blocked per list drain, so force_list is part of its
- * summation order: keep it fixed (for example through the autotuner's cache)
- * when runs must repeat bit for bit. */
+/* Launch choices of the step: per-lane LDS neighbour-list length of the
+ * density and of the force pass (8, 16 or 32; default 16 / 32), the launch
+ * form (split = 1: each pass is one launch for the waves whose lanes share a
+ * grid cell plus one for the others; 0: one launch), the neighbour grid's
+ * cell size (cell_div = 2: cells of h / 2, a 5 x 5 x 5-cell window; 1: cells
+ * of h, the reference's 27 cells) and the gathers' workgroup size (block =
+ * 128 or 256 lanes). The force pass's compensated sums are blocked per list
+ * drain, and the cell size orders the candidates, so force_list and cell_div
+ * are part of the summation order: keep them fixed (for example through the
+ * autotuner's cache) when runs must repeat bit for bit. The block size
+ * changes no result. */
 typedef struct PhyTuning {
     int32_t density_list;
     int32_t force_list;
     int32_t split;
+    int32_t cell_div;
+    int32_t block;
 } PhyTuning;
 
 int32_t phy_fluid_get_tuning(PhyWorldHandle *w, PhyTuning *out);
@@ -368,13 +374,15 @@ int32_t phy_fluid_set_tuning(PhyWorldHandle *w, const PhyTuning *t);
 /* Autotuner. With cache_path naming a file that holds a choice for this
  * device and workload class (architecture, log2 particle count, log2 scene
  * count, non-Newtonian flag), that choice is applied and 1 is returned.
- * Otherwise `steps` steps of dt are timed from the current state for every
- * candidate (list lengths 8, 16, 32, each in one and in two launches), the
- * fastest combination is applied (and stored in cache_path when given), the
- * state is restored, and 0 is returned. rho / p / aux fields are undefined
- * until the next step. out (may be NULL) receives the applied choice; ms12
- * (may be NULL) the timed milliseconds per step: density for lists 8, 16, 32
- * in one launch, then in two, then force in the same order. -1 on error. */
+ * Otherwise `steps` steps of dt are timed from the current state for each
+ * candidate: the cell size (h / 2 and h, whole steps), then at that cell size
+ * the list lengths 8, 16, 32 in one and in two launches, then the workgroup
+ * size (256 and 128 lanes); the fastest choice is applied (and stored in
+ * cache_path when given), the state is restored, and 0 is returned. rho / p /
+ * aux fields are undefined until the next step. out (may be NULL) receives
+ * the applied choice; ms12 (may be NULL) the timed milliseconds per step of
+ * the list trials: density for lists 8, 16, 32 in one launch, then in two,
+ * then force in the same order. -1 on error. */
 int32_t phy_fluid_autotune(PhyWorldHandle *w, double dt, uint32_t steps, const char *cache_path,
                            PhyTuning *out, double *ms12);
 

@@ -19,7 +19,7 @@ COPY_EXE = os.path.join(HERE, "hbm_copy")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["sph_kernels.hip", "sph_adjoint.hip", "sph_couple.hip", "fluid_device.cpp", "world.cpp",
+SOURCES = ["sph_kernels.hip", "sph_stats.hip", "sph_adjoint.hip", "sph_couple.hip", "fluid_device.cpp", "world.cpp",
            "archive.cpp", "phy_ffi.cpp"]
 # the coupling's f64 arithmetic rounds operation by operation like the reference
 EXTRA = {"sph_couple.hip": ["-ffp-contract=off"]}

@@ -34,12 +34,12 @@ void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-// to_i64 of the floored key, as in the kernels (grid.rs:26-33)
-// Half-cell key of the dense table (sph_device.h hkey) on the host; 0 where
-// the key fails (such positions are binned to the overflow segment).
-static long long host_key(float x, double inv) {
+// Cell key of the dense table (sph_device.h hkey: floor(cdiv fl(x (1/h))),
+// the reference key for cdiv = 1, grid.rs:26-33) on the host; 0 where the key
+// fails (such positions are binned to the overflow segment).
+static long long host_key(float x, double inv, int cdiv) {
     const double v = (double)x * inv;
-    const double f = std::floor(v * 2.0);
+    const double f = std::floor(v * (double)cdiv);
     if (f >= -9223372036854775808.0 && f < 9223372036854775808.0 && v >= -9223372036854775808.0 &&
         v < 9223372036854775808.0)
         return (long long)f;
@@ -52,12 +52,15 @@ static long long host_key(float x, double inv) {
 static constexpr uint64_t kMaxParticles = 1ull << 27;
 static const char* const kTooMany = "phy-fluid: at most 2^27 particles per world or batch";
 
-DevParams make_dev_params(const HostParams& hp) {
+DevParams make_dev_params(const HostParams& hp, int cdiv) {
+    if (cdiv != 1 && cdiv != 2) throw std::invalid_argument("phy-fluid: cells of h or h / 2 only");
     DevParams P;
     memset(&P, 0, sizeof(P));
     const double h = hp.h;
     P.h64 = h;
     P.inv_cell = 1.0 / h;
+    P.cdiv = cdiv;
+    P.cell64 = h / (double)cdiv;
     // smallest f64 s with sqrt(s) >= h: the reference's r < h is s < s_thresh
     double s = h * h;
     if (std::isfinite(s) && h > 0.0) {
@@ -86,7 +89,9 @@ DevParams make_dev_params(const HostParams& hp) {
     // s64). Two half-cell columns 2 apart hold points at least h/2 apart, less
     // the key rounding (|x| 2^-52 per point; the scan trims only domains with
     // keys below 2^40 in magnitude, so less than h 2^-11): (h/2)(1 - 2^-8) is a
-    // safe gap. Degenerate h: no trimming (infinite widths).
+    // safe gap. Cells of h: the outer rows are adjacent columns, with no gap,
+    // so every row takes the full width. Degenerate h: no trimming (infinite
+    // widths).
     {
         const double gap2 = 0.25 * (1.0 - std::ldexp(1.0, -8)) * (1.0 - std::ldexp(1.0, -8));
         const double top = 1.0 + std::ldexp(1.0, -40), up = 1.0 + std::ldexp(1.0, -30);
@@ -94,6 +99,7 @@ DevParams make_dev_params(const HostParams& hp) {
         P.zw[0] = ok ? h * std::sqrt(top) * up : INFINITY;
         P.zw[1] = ok ? h * std::sqrt(top - gap2) * up : INFINITY;
         P.zw[2] = ok ? h * std::sqrt(top - 2.0 * gap2) * up : INFINITY;
+        if (cdiv == 1) P.zw[1] = P.zw[2] = P.zw[0];
     }
     // kernels.rs:22-35 in f64
     const double pi = 3.141592653589793;
@@ -153,7 +159,7 @@ DeviceFluid::DeviceFluid(const std::vector<HostParams>& scenes) {
     if (scenes.empty()) throw std::invalid_argument("phy-fluid: a batch needs at least one scene");
     if (scenes.size() > (1u << 24)) throw std::invalid_argument("phy-fluid: at most 2^24 scenes");
     std::vector<DevParams> ps;
-    for (const HostParams& hp : scenes) ps.push_back(make_dev_params(hp));
+    for (const HostParams& hp : scenes) ps.push_back(make_dev_params(hp, tuning_.cell_div));
     int count = 0;
     hipError_t e = hipGetDeviceCount(&count);
     if (e != hipSuccess || count <= 0)
@@ -166,8 +172,8 @@ DeviceFluid::DeviceFluid(const std::vector<HostParams>& scenes) {
     scn_.assign(S, SceneInfo{});
     for (size_t s = 0; s < S; ++s) {
         for (int a = 0; a < 3; ++a) {
-            scn_[s].bkey[a] = host_key(Ps_[s].lo[a], Ps_[s].inv_cell);
-            scn_[s].bkey[3 + a] = host_key(Ps_[s].hi[a], Ps_[s].inv_cell);
+            scn_[s].bkey[a] = host_key(Ps_[s].lo[a], Ps_[s].inv_cell, Ps_[s].cdiv);
+            scn_[s].bkey[3 + a] = host_key(Ps_[s].hi[a], Ps_[s].inv_cell, Ps_[s].cdiv);
         }
     }
     dom_.ensure(S);
@@ -237,7 +243,7 @@ void DeviceFluid::size_tables(const float* pos3) {
                 if (!std::isfinite(p[0]) || !std::isfinite(p[1]) || !std::isfinite(p[2])) continue;
                 any = true;
                 for (int a = 0; a < 3; ++a) {
-                    long long key = host_key(p[a], Ps_[s].inv_cell);
+                    long long key = host_key(p[a], Ps_[s].inv_cell, Ps_[s].cdiv);
                     plo[a] = std::min(plo[a], key);
                     phi[a] = std::max(phi[a], key);
                 }
@@ -285,12 +291,12 @@ void DeviceFluid::push_params() {
 
 void DeviceFluid::set_scene_params(size_t s, const HostParams& hp) {
     if (s >= scn_.size()) throw std::out_of_range("phy-fluid: scene index");
-    DevParams P = make_dev_params(hp);
+    DevParams P = make_dev_params(hp, tuning_.cell_div);
     Ps_[s] = P;
     hps_[s] = hp;
     for (int a = 0; a < 3; ++a) {
-        scn_[s].bkey[a] = host_key(P.lo[a], P.inv_cell);
-        scn_[s].bkey[3 + a] = host_key(P.hi[a], P.inv_cell);
+        scn_[s].bkey[a] = host_key(P.lo[a], P.inv_cell, P.cdiv);
+        scn_[s].bkey[3 + a] = host_key(P.hi[a], P.inv_cell, P.cdiv);
     }
     size_tables(nullptr);
     push_params();
@@ -299,7 +305,7 @@ void DeviceFluid::set_scene_params(size_t s, const HostParams& hp) {
 }
 
 void DeviceFluid::set_params(const HostParams& hp) {
-    make_dev_params(hp);  // validate before changing anything
+    make_dev_params(hp, tuning_.cell_div);  // validate before changing anything
     for (size_t s = 0; s < scn_.size(); ++s) set_scene_params(s, hp);
 }
 
@@ -553,14 +559,38 @@ void DeviceFluid::set_graph(bool on) {
 }
 
 void DeviceFluid::set_tuning(const Tuning& t) {
-    if (!tuning_valid(t)) throw std::invalid_argument("tuning: list lengths must be 8, 16 or 32 and split 0 or 1");
-    // the force pass's list length is part of its summation order: recomputed
-    // segments would not reproduce the recorded steps
+    if (!tuning_valid(t))
+        throw std::invalid_argument(
+            "tuning: list lengths must be 8, 16 or 32, split 0 or 1, cell_div 1 or 2 and block 128 or 256");
+    // the force pass's list length and the cell size are part of the
+    // summation order: recomputed segments would not reproduce the recorded
+    // steps
     if (!(t.density_list == tuning_.density_list && t.force_list == tuning_.force_list &&
-          t.split == tuning_.split))
+          t.split == tuning_.split && t.cell_div == tuning_.cell_div && t.block == tuning_.block))
         tape_invalidate("the launch tuning changed");
+    const int old_div = tuning_.cell_div;
     tuning_ = t;
     graph_dirty_ = true;
+    if (t.cell_div != old_div) apply_cell_div(old_div);
+}
+
+// New cell size: every scene's device constants and bounds keys, and its
+// table capacity scaled by the change in cells per volume.
+void DeviceFluid::apply_cell_div(int old_div) {
+    const int cd = tuning_.cell_div;
+    for (size_t s = 0; s < scn_.size(); ++s) {
+        Ps_[s] = make_dev_params(hps_[s], cd);
+        for (int a = 0; a < 3; ++a) {
+            scn_[s].bkey[a] = host_key(Ps_[s].lo[a], Ps_[s].inv_cell, cd);
+            scn_[s].bkey[3 + a] = host_key(Ps_[s].hi[a], Ps_[s].inv_cell, cd);
+        }
+        const double f = std::pow((double)cd / (double)old_div, 3.0);
+        const double c = std::ceil((double)scn_[s].cap * f);
+        scn_[s].cap = (unsigned long long)std::min(std::max(c, 1.0), (double)kMaxDenseCells);
+    }
+    size_tables(nullptr);
+    push_params();
+    dens_valid_ = aux_valid_ = false;
 }
 
 std::string DeviceFluid::tuning_key() const {
@@ -581,16 +611,19 @@ std::string DeviceFluid::tuning_key() const {
     return o.str();
 }
 
-// Cache lines: "<key> density_list=<L> force_list=<L> split=<0|1>", one per key.
+// Cache lines: "<key> density_list=<L> force_list=<L> split=<0|1> cell_div=<1|2>
+// block=<128|256>", one per key (lines without the last two fields take cells
+// of h / 2 and 256-lane workgroups).
 static bool read_tuning(const std::string& path, const std::string& key, Tuning& t) {
     std::ifstream f(path);
     std::string line;
     while (std::getline(f, line)) {
         if (line.compare(0, key.size() + 1, key + " ") != 0) continue;
         Tuning c;
-        if (std::sscanf(line.c_str() + key.size(), " density_list=%d force_list=%d split=%d",
-                        &c.density_list, &c.force_list, &c.split) == 3 &&
-            tuning_valid(c)) {
+        const int got = std::sscanf(line.c_str() + key.size(),
+                                    " density_list=%d force_list=%d split=%d cell_div=%d block=%d",
+                                    &c.density_list, &c.force_list, &c.split, &c.cell_div, &c.block);
+        if ((got == 3 || got == 5) && tuning_valid(c)) {
             t = c;
             return true;
         }
@@ -608,7 +641,7 @@ static void write_tuning(const std::string& path, const std::string& key, const 
     }
     std::ostringstream o;
     o << key << " density_list=" << t.density_list << " force_list=" << t.force_list
-      << " split=" << t.split;
+      << " split=" << t.split << " cell_div=" << t.cell_div << " block=" << t.block;
     keep.push_back(o.str());
     const std::string tmp = path + ".tmp";
     {
@@ -635,20 +668,44 @@ bool DeviceFluid::autotune(float dt, uint32_t steps, const char* cache_path, dou
     if (n_ == 0 || steps == 0) return false;
     Snapshot snap;
     save_state(snap);
-    // trial k runs list length cand[k % 3] in both passes, launch form k / 3;
-    // the density and force passes are timed separately and chosen apart
-    const int cand[3] = {8, 16, 32};
-    double dms[6], fms[6];
-    for (int k = 0; k < 6; ++k) {
+    // `steps` eager steps of the saved state under tuning t: milliseconds per
+    // step {sort, density, force, whole step}
+    auto trial = [&](const Tuning& t, double out[4]) {
+        set_tuning(t);
         load_state(snap);
-        tuning_.density_list = tuning_.force_list = cand[k % 3];
-        tuning_.split = k / 3;
         profile_begin();
         for (uint32_t s = 0; s < steps; ++s) step_core(dt, own_bufs());
         double m4[4];
         const uint32_t nst = profile_end(m4);
-        dms[k] = m4[1] / nst;
-        fms[k] = m4[2] / nst;
+        for (int k = 0; k < 4; ++k) out[k] = m4[k] / nst;
+    };
+    double m4[4];
+    // 1. the cell size, with the default lists and launch form: whole steps
+    // (the sort's table and both passes' candidate counts change with it)
+    Tuning t;
+    double best = 0.0;
+    int best_div = 2;
+    for (int cd : {2, 1}) {
+        t.cell_div = cd;
+        trial(t, m4);
+        if (cd == 2 || m4[3] < best) {
+            best = m4[3];
+            best_div = cd;
+        }
+    }
+    // 2. list lengths and launch form at that cell size: trial k runs list
+    // length cand[k % 3] in both passes, launch form k / 3; the density and
+    // force passes are timed separately and chosen apart
+    const int cand[3] = {8, 16, 32};
+    double dms[6], fms[6];
+    for (int k = 0; k < 6; ++k) {
+        Tuning c;
+        c.cell_div = best_div;
+        c.density_list = c.force_list = cand[k % 3];
+        c.split = k / 3;
+        trial(c, m4);
+        dms[k] = m4[1];
+        fms[k] = m4[2];
     }
     // one launch form for both passes: the one with the smaller best sum
     int best_split = 0;
@@ -668,13 +725,23 @@ bool DeviceFluid::autotune(float dt, uint32_t steps, const char* cache_path, dou
             bf = f0;
         }
     }
-    load_state(snap);
-    sync();
-    Tuning t;
+    // 3. the gathers' workgroup size for that choice
+    t = Tuning();
+    t.cell_div = best_div;
     t.density_list = cand[bd % 3];
     t.force_list = cand[bf % 3];
     t.split = best_split;
+    double b256 = 0.0;
+    for (int bs : {256, 128}) {
+        t.block = bs;
+        trial(t, m4);
+        const double g = m4[1] + m4[2];
+        if (bs == 256) b256 = g;
+        else if (!(g < b256)) t.block = 256;
+    }
     set_tuning(t);
+    load_state(snap);
+    sync();
     dens_valid_ = aux_valid_ = false;
     if (ms)
         for (int k = 0; k < 6; ++k) {

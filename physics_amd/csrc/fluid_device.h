@@ -83,7 +83,7 @@ struct HostParams {
 
 // Derives the device constants: f64 neighbour threshold and FP32 coefficients
 // (kernels.rs:22-35 computed in f64, rounded once).
-DevParams make_dev_params(const HostParams& hp);
+DevParams make_dev_params(const HostParams& hp, int cdiv = 2);
 
 // Particle fields the device can hand back in original index order.
 enum Field : int {
@@ -191,16 +191,18 @@ class DeviceFluid {
     // Launch choices of the gather kernels (sph_kernels.h Tuning).
     const Tuning& tuning() const { return tuning_; }
     void set_tuning(const Tuning& t);
-    // Autotuner: times `steps` eager steps of the current state for each
-    // candidate (list length 8, 16, 32 in one and in two launches; density and
-    // force timed separately from HIP events), keeps the launch form with the
-    // smaller best sum and the fastest list length of each pass in it, then
-    // restores the state. rho / p / aux are undefined until the next step.
+    // Autotuner: times `steps` eager steps of the current state per
+    // candidate with HIP events: first the cell size (h / 2 or h, whole
+    // steps, default lists), then at that cell size the list lengths 8, 16,
+    // 32 in one and in two launches (density and force timed separately; the
+    // launch form with the smaller best sum and the fastest list length of
+    // each pass in it), then the gathers' workgroup size (256 or 128 lanes).
+    // Restores the state; rho / p / aux are undefined until the next step.
     // With a cache path, a stored choice for this device and workload class is
     // applied without timing (returns true), and a new choice is stored, so
     // repeated runs keep one summation order. ms[12] (optional): density
-    // milliseconds per step of the six trials (lists 8, 16, 32 combined, then
-    // split), then force.
+    // milliseconds per step of the six list trials (lists 8, 16, 32 combined,
+    // then split), then force.
     bool autotune(float dt, uint32_t steps, const char* cache_path, double* ms);
     // "gfx950 n20 s0 shear0": device architecture, log2 particles, log2 scenes,
     // non-Newtonian flag (the cache key)
@@ -262,6 +264,7 @@ class DeviceFluid {
     void set_layout(const uint32_t* counts);
     void size_tables(const float* pos3);
     void push_params();
+    void apply_cell_div(int old_div);
     SceneTable table(Domain* dom) const;
     StepBufs own_bufs();
     StepBufs slot_bufs(TapeSlot& t);

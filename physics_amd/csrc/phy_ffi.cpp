@@ -635,14 +635,18 @@ int32_t phy_fluid_set_graph(PhyWorldHandle* w, int32_t on) {
     });
 }
 
+static void to_c(const phy::Tuning& t, PhyTuning* out) {
+    out->density_list = t.density_list;
+    out->force_list = t.force_list;
+    out->split = t.split;
+    out->cell_div = t.cell_div;
+    out->block = t.block;
+}
+
 static int32_t tune(phy::DeviceFluid& d, double dt, uint32_t steps, const char* cache_path,
                     PhyTuning* out, double* ms12) {
     const bool cached = d.autotune((float)dt, steps, cache_path, ms12);
-    if (out) {
-        out->density_list = d.tuning().density_list;
-        out->force_list = d.tuning().force_list;
-        out->split = d.tuning().split;
-    }
+    if (out) to_c(d.tuning(), out);
     return cached ? 1 : 0;
 }
 
@@ -652,16 +656,15 @@ static phy::Tuning from_c(const PhyTuning* t) {
     r.density_list = t->density_list;
     r.force_list = t->force_list;
     r.split = t->split;
+    r.cell_div = t->cell_div;
+    r.block = t->block;
     return r;
 }
 
 int32_t phy_fluid_get_tuning(PhyWorldHandle* w, PhyTuning* out) {
     return guard<int32_t>(-1, [&] {
         if (!out) throw std::invalid_argument("null out");
-        const phy::Tuning& t = need_fluid(w)->world().device().tuning();
-        out->density_list = t.density_list;
-        out->force_list = t.force_list;
-        out->split = t.split;
+        to_c(need_fluid(w)->world().device().tuning(), out);
         return 0;
     });
 }
@@ -687,10 +690,7 @@ int32_t phy_fluid_autotune(PhyWorldHandle* w, double dt, uint32_t steps, const c
 int32_t phy_fluid_batch_get_tuning(PhyFluidBatch* b, PhyTuning* out) {
     return guard<int32_t>(-1, [&] {
         if (!out) throw std::invalid_argument("null out");
-        const phy::Tuning& t = need_batch(b)->device().tuning();
-        out->density_list = t.density_list;
-        out->force_list = t.force_list;
-        out->split = t.split;
+        to_c(need_batch(b)->device().tuning(), out);
         return 0;
     });
 }

@@ -289,9 +289,9 @@ __device__ __forceinline__ long long key_d(double x, double inv) {
     return 0;
 }
 
-// The dense runs of a point's 27 reference cells (as half cells, up to 36
-// z-runs) plus the overflow segment (checked by key adjacency), flattened for
-// the workgroup.
+// The dense runs of a point's 27 reference cells (as cells of the table, up
+// to 36 z-runs for half cells, 9 for cells of h) plus the overflow segment
+// (checked by key adjacency), flattened for the workgroup.
 struct Runs {
     uint32_t a[37], len[37];
     uint32_t total;
@@ -306,7 +306,8 @@ __device__ void point_runs(const D3& q, const CoupleParams& cp, const uint32_t* 
     ref_block(key_d(q.z, cp.inv_cell), d, 2, lo[2], hi[2]);
     R.nr = 0;
     R.total = 0;
-    for (int k = 0; k < 36; ++k) {
+    const int rows = block_span(d) * block_span(d);
+    for (int k = 0; k < rows; ++k) {
         uint32_t a, b;
         block_run(k, lo, hi, d, start, a, b);
         if (b > a) {
@@ -594,10 +595,10 @@ void launch_heat_source(hipStream_t st, int n, double dt, const HeatScratch& hs,
 
 // ---------------------------------------------------------------------------
 // velocity sampler: exact nearest particle (first in index order on ties) by
-// rings of half cells around the query's half cell; rings are visited until
-// the best squared distance is below ((r - 1) h / 2)^2, a full half cell of
-// margin over the smallest distance an unvisited ring can hold. The overflow
-// segment is always scanned.
+// rings of table cells (edge h / cdiv) around the query's cell; rings are
+// visited until the best squared distance is below ((r - 1) h / cdiv)^2, a
+// full cell of margin over the smallest distance an unvisited ring can hold.
+// The overflow segment is always scanned.
 
 __device__ __forceinline__ void nearest_consider(const D3& q, const float4& p, uint32_t o, double& bd,
                                                  uint32_t& bo) {
@@ -624,8 +625,9 @@ __global__ __launch_bounds__(256) void k_nearest_velocity(int nq, const double* 
     double bd = __longlong_as_double(0x7ff0000000000000ll);  // +inf
     uint32_t bo = 0xffffffffu;
     long long kq[3];
-    bool keyed = hkey(q.x * cp.inv_cell, kq[0]) & hkey(q.y * cp.inv_cell, kq[1]) &
-                 hkey(q.z * cp.inv_cell, kq[2]);
+    const double cd = (double)d.reach;
+    bool keyed = hkey(q.x * cp.inv_cell, cd, kq[0]) & hkey(q.y * cp.inv_cell, cd, kq[1]) &
+                 hkey(q.z * cp.inv_cell, cd, kq[2]);
     for (int a = 0; a < 3; ++a)  // far from the table: no ring walk (and no key overflow)
         keyed = keyed && fabs((double)kq[a] - (double)d.kmin[a]) < 1099511627776.0;
     if (isfinite(q.x) && isfinite(q.y) && isfinite(q.z) && !keyed) {
@@ -668,7 +670,7 @@ __global__ __launch_bounds__(256) void k_nearest_velocity(int nq, const double* 
                     }
                 }
             if (r >= 1) {
-                const double m = (double)(r - 1) * (0.5 * cp.h);  // half cells
+                const double m = (double)(r - 1) * (cp.h / cd);  // whole cells
                 if (bd < m * m) break;
             }
         }

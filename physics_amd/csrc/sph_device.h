@@ -23,13 +23,14 @@ __device__ __forceinline__ long long key1(float x, double inv) {
     return 0;
 }
 
-// The dense table bins particles into half cells of size h / 2: the half key
-// floor(2 fl(x (1/h))) is an exact refinement of the reference key (scaling by
-// 2 is exact, so key1 = hkey >> 1). Returns false where the reference key
-// fails (NaN, out of the i64 range): such particles live in the overflow
-// segment, which is matched by reference-key adjacency.
-__device__ __forceinline__ bool hkey(double v, long long& k) {
-    const double f = floor(v * 2.0);
+// The dense table bins particles into cells of size h / cdiv (cdiv 1 or 2):
+// the cell key floor(cdiv fl(x (1/h))) is the reference key for cdiv = 1 and
+// an exact refinement of it for cdiv = 2 (scaling by 2 is exact, so key1 =
+// hkey >> 1). Returns false where the reference key fails (NaN, out of the i64
+// range): such particles live in the overflow segment, which is matched by
+// reference-key adjacency.
+__device__ __forceinline__ bool hkey(double v, double cdiv, long long& k) {
+    const double f = floor(v * cdiv);
     if (f >= -9223372036854775808.0 && f < 9223372036854775808.0 && v >= -9223372036854775808.0 &&
         v < 9223372036854775808.0) {
         k = (long long)f;
@@ -39,12 +40,12 @@ __device__ __forceinline__ bool hkey(double v, long long& k) {
     return false;
 }
 
-// A dense particle's candidates lie within +-kReach half cells per axis: a
-// pair with r < h has |fl(x_j/h) - fl(x_i/h)| < 1 up to rounding, i.e. half
-// keys at most 2 apart (see the neighbour-search notes of the design).
-constexpr int kReach = 2;
-constexpr int kSpan = 2 * kReach + 1;  // half cells per row of the window
-constexpr int kRows = kSpan * kSpan;   // rows (x, y) of the window
+// A dense particle's candidates lie within +-reach cells per axis (reach =
+// cdiv): a pair with r < h has |fl(x_j/h) - fl(x_i/h)| < 1 up to rounding,
+// i.e. reference keys at most 1 apart and half keys at most 2 apart (see the
+// neighbour-search notes of the design). The window has span = 2 reach + 1
+// cells per row and span^2 rows (x, y) of z-runs.
+__device__ __forceinline__ int win_span(const Domain& d) { return 2 * d.reach + 1; }
 
 __device__ __forceinline__ bool finite3(float x, float y, float z) {
     return isfinite(x) && isfinite(y) && isfinite(z);
@@ -112,6 +113,25 @@ __device__ __forceinline__ TileLoc tile_loc(const SceneTable& T) {
     return L;
 }
 
+// tile_loc for workgroups of BS lanes (kTile / BS workgroups per tile; the
+// XCD remap keeps a tile's workgroups, and consecutive tiles, on one XCD).
+template <int BS>
+__device__ __forceinline__ TileLoc tile_loc_bs(const SceneTable& T) {
+    static_assert(BS == 128 || BS == 256, "workgroups of 128 or 256 lanes");
+    if (BS == kTile) return tile_loc(T);
+    constexpr int sub = kTile / BS;
+    TileLoc L;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    L.t = b / sub;
+    L.s = T.tile_scene[L.t];
+    const SceneInfo* si = T.info + L.s;
+    const uint32_t local = (uint32_t)(L.t - (int)si->tile0) * (uint32_t)kTile + (uint32_t)(b % sub) * BS +
+                           threadIdx.x;
+    L.act = local < si->n;
+    L.i = (int)(si->off + local);
+    return L;
+}
+
 __device__ __forceinline__ long long wave_min_ll(long long v) {
     for (int o = 32; o > 0; o >>= 1) {
         long long w = __shfl_xor(v, o, 64);
@@ -136,7 +156,8 @@ __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
 
 __device__ __forceinline__ uint32_t cell_of(float x, float y, float z, double inv, const Domain& d) {
     long long h0, h1, h2;
-    const bool ok = hkey((double)x * inv, h0) & hkey((double)y * inv, h1) & hkey((double)z * inv, h2);
+    const double cd = (double)d.reach;
+    const bool ok = hkey((double)x * inv, cd, h0) & hkey((double)y * inv, cd, h1) & hkey((double)z * inv, cd, h2);
     // wrapping differences: an out-of-domain key maps to a huge unsigned value
     const unsigned long long k0 = (unsigned long long)h0 - (unsigned long long)d.kmin[0];
     const unsigned long long k1 = (unsigned long long)h1 - (unsigned long long)d.kmin[1];
@@ -147,12 +168,14 @@ __device__ __forceinline__ uint32_t cell_of(float x, float y, float z, double in
     return (uint32_t)((k0 * d.dims[1] + k1) * d.dims[2] + k2);
 }
 
-// Half-key interval (table-relative, per axis) of the 27 reference cells
+// Cell-key interval (table-relative, per axis) of the 27 reference cells
 // around reference key k: the reference keys k - 1, k, k + 1 (wrapping, like
-// grid.rs:44-48) that fall inside the table, as half cells. Empty: lo > hi.
+// grid.rs:44-48) that fall inside the table, as cells of the table (a
+// reference cell is 1 or 2 table cells per axis). Empty: lo > hi.
 __device__ __forceinline__ void ref_block(long long k, const Domain& d, int a, long long& lo,
                                           long long& hi) {
-    const long long hlo = d.kmin[a] >> 1, hhi = (d.kmin[a] + d.dims[a] - 1) >> 1;
+    const int sh = d.reach - 1;  // cell key >> sh = reference key
+    const long long hlo = d.kmin[a] >> sh, hhi = (d.kmin[a] + d.dims[a] - 1) >> sh;
     const unsigned long long span = (unsigned long long)(hhi - hlo);
     long long kmn = 0, kmx = -1;
     bool any = false;
@@ -169,7 +192,8 @@ __device__ __forceinline__ void ref_block(long long k, const Domain& d, int a, l
         hi = -1;
         return;
     }
-    const long long c0 = 2 * kmn, c1 = 2 * kmx + 1;
+    const long long c0 = (long long)((unsigned long long)kmn << sh),
+                    c1 = (long long)((unsigned long long)(kmx + 1) << sh) - 1;
     lo = (c0 > d.kmin[a] ? c0 : d.kmin[a]) - d.kmin[a];
     hi = (c1 < d.kmin[a] + d.dims[a] - 1 ? c1 : d.kmin[a] + d.dims[a] - 1) - d.kmin[a];
 }
@@ -177,8 +201,8 @@ __device__ __forceinline__ void ref_block(long long k, const Domain& d, int a, l
 // ---------------------------------------------------------------------------
 // neighbour visiting
 //
-// Rows (dx, dy) of the half-cell window with the z-adjacent cells of each
-// row as one contiguous run [start[c0], start[c4 + 1]).
+// Rows (dx, dy) of the cell window with the z-adjacent cells of each row as
+// one contiguous run [start[cz - reach], start[cz + reach + 1]).
 
 struct CellCoord {
     long long cx, cy, cz;
@@ -201,31 +225,36 @@ __device__ __forceinline__ CellCoord decode(uint32_t c, const Domain& d) {
     return r;
 }
 
-// Run of row k (0..kRows-1) of the window of an in-domain particle: half
-// cells (cx + dx, cy + dy, cz - kReach .. cz + kReach), contiguous in z; empty
-// when the row leaves the domain (those cells hold no dense particles).
+// Run of row k (0..span^2-1) of the window of an in-domain particle: cells
+// (cx + dx, cy + dy, cz - reach .. cz + reach), contiguous in z; empty when
+// the row leaves the domain (those cells hold no dense particles).
 __device__ __forceinline__ void row_run(int k, const CellCoord& cc, const Domain& d,
                                         const uint32_t* __restrict__ start, uint32_t& a,
                                         uint32_t& b) {
-    long long x = cc.cx + (k / kSpan) - kReach;
-    long long y = cc.cy + (k % kSpan) - kReach;
+    const int R = d.reach, span = 2 * R + 1;
+    long long x = cc.cx + (k / span) - R;
+    long long y = cc.cy + (k % span) - R;
     if (x < 0 || x >= d.dims[0] || y < 0 || y >= d.dims[1]) {
         a = b = 0;
         return;
     }
-    long long z0 = cc.cz > kReach ? cc.cz - kReach : 0;
-    long long z1 = cc.cz + kReach < d.dims[2] ? cc.cz + kReach : d.dims[2] - 1;
+    long long z0 = cc.cz > R ? cc.cz - R : 0;
+    long long z1 = cc.cz + R < d.dims[2] ? cc.cz + R : d.dims[2] - 1;
     long long base = (x * d.dims[1] + y) * d.dims[2];
     a = start[base + z0];
     b = start[base + z1 + 1];
 }
 
-// Run of row k (0..35) of the reference block of an out-of-domain particle
-// (intervals from ref_block); empty rows have a == b.
+// Cells per axis of the reference block of an out-of-domain particle.
+__device__ __forceinline__ int block_span(const Domain& d) { return 3 * d.reach; }
+
+// Run of row k (0..block_span^2-1) of the reference block of an out-of-domain
+// particle (intervals from ref_block); empty rows have a == b.
 __device__ __forceinline__ void block_run(int k, const long long* lo, const long long* hi,
                                           const Domain& d, const uint32_t* __restrict__ start,
                                           uint32_t& a, uint32_t& b) {
-    const long long x = lo[0] + k / 6, y = lo[1] + k % 6;
+    const int bs = block_span(d);
+    const long long x = lo[0] + k / bs, y = lo[1] + k % bs;
     if (x > hi[0] || y > hi[1] || lo[2] > hi[2]) {
         a = b = 0;
         return;
@@ -254,18 +283,20 @@ __device__ __forceinline__ void for_each_candidate(uint32_t ci, float xi, float 
                                                    const float4* __restrict__ posm, F&& f) {
     if (ci < d.ncells) {
         CellCoord cc = decode(ci, d);
-        for (int k = 0; k < kRows; ++k) {
+        const int rows = win_span(d) * win_span(d);
+        for (int k = 0; k < rows; ++k) {
             uint32_t a, b;
             row_run(k, cc, d, start, a, b);
             for (uint32_t j = a; j < b; ++j) f(j);
         }
     } else {
-        // out-of-domain particle: the in-domain half cells of its 27 reference cells
+        // out-of-domain particle: the in-domain cells of its 27 reference cells
         long long lo[3], hi[3];
         ref_block(key1(xi, inv), d, 0, lo[0], hi[0]);
         ref_block(key1(yi, inv), d, 1, lo[1], hi[1]);
         ref_block(key1(zi, inv), d, 2, lo[2], hi[2]);
-        for (int k = 0; k < 36; ++k) {
+        const int rows = block_span(d) * block_span(d);
+        for (int k = 0; k < rows; ++k) {
             uint32_t a, b;
             block_run(k, lo, hi, d, start, a, b);
             for (uint32_t j = a; j < b; ++j) f(j);
@@ -298,11 +329,11 @@ __device__ __forceinline__ void for_each_candidate(uint32_t ci, float xi, float 
 struct Cursor {
     // The lane's window as rows of a rectangle of table columns [x0, x1] x
     // [y0, y1] with z cells z0..z1 (empty when z0 > z1), walked row by row:
-    // x = x0 + row / step, y = y0 + row % step, step 5 (dense particle: the 5 x
-    // 5 half-cell window) or 6 (out-of-domain particle: the in-domain half
-    // cells of its 27 reference cells). 32-bit table-relative values with
-    // wrapping arithmetic: a row left of the table wraps to a huge x and fails
-    // x <= x1.
+    // x = x0 + row / step, y = y0 + row % step, step 2 reach + 1 (dense
+    // particle: the 5 x 5 half-cell or 3 x 3 cell window) or 3 reach
+    // (out-of-domain particle: the in-domain cells of its 27 reference cells).
+    // 32-bit table-relative values with wrapping arithmetic: a row left of the
+    // table wraps to a huge x and fails x <= x1.
     int row, nrows;
     uint32_t x0, x1, y0, y1, z0, z1, mul;
     uint32_t j, jend, o0, o1;
@@ -312,9 +343,10 @@ struct Cursor {
     // Bounds [ja, jb) of row k; a row past the window (k = nrows) is empty.
     __device__ __forceinline__ void run(int k, const Domain& d, const uint32_t* __restrict__ start,
                                         uint32_t& ja, uint32_t& jb) const {
-        // k / step for k < 2^14 by multiply and shift (step 5: 0xCCCD, 6: 0xAAAB)
+        // k / step for k < 2^14 by multiply and shift (step 3: 0x15556, 5:
+        // 0xCCCD, 6: 0xAAAB)
         const uint32_t q = ((uint32_t)k * mul) >> 18;
-        const uint32_t step = mul == 0xAAABu ? 6u : 5u;
+        const uint32_t step = mul == 0xAAABu ? 6u : (mul == 0xCCCDu ? 5u : 3u);
         const uint32_t x = x0 + q, y = y0 + ((uint32_t)k - q * step);
         const bool ok = x <= x1 && y <= y1 && z0 <= z1;
         // loads from a valid address either way, then a select of values (a
@@ -347,15 +379,16 @@ struct Cursor {
         o0 = start[d.ncells];
         o1 = start[d.ncells + 1];
         const bool dense = ci < d.ncells;
-        mul = dense ? 0xCCCDu : 0xAAABu;
-        nrows = dense ? kRows : 36;
+        const bool half = d.reach == 2;
+        mul = dense ? (half ? 0xCCCDu : 0x15556u) : (half ? 0xAAABu : 0x15556u);
+        nrows = dense ? (half ? 25 : 9) : (half ? 36 : 9);
         x0 = y0 = z0 = 1u;
         x1 = y1 = z1 = 0u;  // empty
         if (!act) return;
         if (dense) {
             const CellCoord cc = decode(ci, d);
             const uint32_t cx = (uint32_t)cc.cx, cy = (uint32_t)cc.cy, cz = (uint32_t)cc.cz;
-            const uint32_t r = (uint32_t)kReach;
+            const uint32_t r = (uint32_t)d.reach;
             x0 = cx - r;
             y0 = cy - r;
             x1 = min(cx + r, (uint32_t)d.dims[0] - 1u);
@@ -427,8 +460,8 @@ enum GatherMode { kAllWaves = 0, kUniformWaves = 1, kOtherWaves = 2 };
 
 // The wave's active lanes all sit in dense half cells of one (x, y) column,
 // within kColSpan consecutive z cells [c0, c1]: their candidates are then
-// covered by one shared window (rows of z-runs cz(c0) - kReach .. cz(c1) +
-// kReach), scanned with wave-uniform loads. The window is larger than a
+// covered by one shared window (rows of z-runs cz(c0) - reach .. cz(c1) +
+// reach), scanned with wave-uniform loads. The window is larger than a
 // lane's own by at most kColSpan - 1 cells per row; every extra candidate is
 // still tested with the exact predicate.
 constexpr uint32_t kColSpan = 3;
@@ -529,42 +562,44 @@ __device__ __forceinline__ void trim_run(const float4* __restrict__ src, float z
 }
 
 // Column-local coordinates of a particle in dense cell c: (x - X, y - Y,
-// (x - X)^2 + (y - Y)^2, z), X and Y the lower corner of its half-cell column.
+// (x - X)^2 + (y - Y)^2, z), X and Y the lower corner of its cell column.
 // The uniform scan filters with them: for a candidate of row (ex, ey) and a
-// lane with local (u, v), dx = x_j - (u - ex h/2) and likewise dy, so
+// lane with local (u, v), dx = x_j - (u - ex cell) and likewise dy, so
 // dx^2 + dy^2 = n_j - 2 xr x_j - 2 yr y_j + (xr^2 + yr^2): two FMAs per
 // candidate against per-row lane constants, all terms of size h^2 (no
 // cancellation of large coordinates).
-__device__ __forceinline__ float4 col_local(const float4& p, uint32_t c, const Domain& d, double half) {
+__device__ __forceinline__ float4 col_local(const float4& p, uint32_t c, const Domain& d, double cell) {
     const CellCoord cc = decode(c, d);
-    const double X = (double)(d.kmin[0] + cc.cx) * half, Y = (double)(d.kmin[1] + cc.cy) * half;
+    const double X = (double)(d.kmin[0] + cc.cx) * cell, Y = (double)(d.kmin[1] + cc.cy) * cell;
     const float xl = (float)((double)p.x - X), yl = (float)((double)p.y - Y);
     return make_float4(xl, yl, fmaf(yl, yl, xl * xl), p.z);
 }
 
 // The column-local filter of a uniform scan (posl: column-local coordinates
 // of the sorted particles, kScanPad spare entries; li: the lane's own; hh:
-// h / 2; s_hq: the bound, NaN to pass everything). Conservative: its FP32
-// rounding is below 1e-5 h^2 while s_hq sits s_thresh 2^-16 above s_thresh.
+// the cell edge h / cdiv; s_hq: the bound, NaN to pass everything).
+// Conservative: its FP32 rounding is below 1e-5 h^2 while s_hq sits s_thresh
+// 2^-16 above s_thresh.
 struct ColLocal {
     const float4* __restrict__ posl;
     float4 li;
     float hh, s_hq;
 };
 
-// Run of row k of a column wave's shared window: z cells cz0 - kReach ..
-// cz1 + kReach of column (cx + dx, cy + dy).
-__device__ __forceinline__ void col_run(int k, const CellCoord& cc, long long cz1, const Domain& d,
+// Run of row (ex, ey) of a column wave's shared window: z cells cz0 - reach
+// .. cz1 + reach of column (cx + ex, cy + ey).
+__device__ __forceinline__ void col_run(int ex, int ey, const CellCoord& cc, long long cz1, const Domain& d,
                                         const uint32_t* __restrict__ start, uint32_t& a,
                                         uint32_t& b) {
-    long long x = cc.cx + (k / kSpan) - kReach;
-    long long y = cc.cy + (k % kSpan) - kReach;
+    const int R = d.reach;
+    long long x = cc.cx + ex;
+    long long y = cc.cy + ey;
     if (x < 0 || x >= d.dims[0] || y < 0 || y >= d.dims[1]) {
         a = b = 0;
         return;
     }
-    long long z0 = cc.cz > kReach ? cc.cz - kReach : 0;
-    long long z1 = cz1 + kReach < d.dims[2] ? cz1 + kReach : d.dims[2] - 1;
+    long long z0 = cc.cz > R ? cc.cz - R : 0;
+    long long z1 = cz1 + R < d.dims[2] ? cz1 + R : d.dims[2] - 1;
     long long base = (x * d.dims[1] + y) * d.dims[2];
     a = start[base + z0];
     b = start[base + z1 + 1];
@@ -584,25 +619,28 @@ struct DrainWidth {
 // zt: the z half-widths of the uniform scan's trimmed rows. With LOC the
 // uniform scan filters with the column-local coordinates of cl instead of
 // test (which still filters the overflow segment and the per-lane scan).
-template <int L, int MODE, int S, bool LOC, bool PAD, int R, class Test, class Load, class Proc,
-          class Flush>
+template <int L, int MODE, int S, bool LOC, bool PAD, int R, int BS = 256, class Test, class Load,
+          class Proc, class Flush>
 __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi, float zi,
                                        uint32_t self_j, const Domain& d, double inv,
                                        const ZWin zt, const ColLocal& cl,
                                        const uint32_t* __restrict__ start,
-                                       const float4* __restrict__ posm, uint32_t (*lst)[256],
+                                       const float4* __restrict__ posm, uint32_t (*lst)[BS],
                                        int tid, Test&& test, Load&& load, Proc&& proc,
                                        Flush&& flush) {
     static_assert(S == 1 || S == 2 || S == 4, "drain width");
     static_assert(L % S == 0, "whole drain rounds per list");
     static_assert(R >= 1 && R <= 8 && R <= L, "per-lane round width");
+    static_assert(BS == 128 || BS == 256, "workgroups of 128 or 256 lanes");
     // the lane's list head as a 32-bit byte offset into the list array: it
-    // advances one row (1024 bytes) per append, so an append is one select
+    // advances one row (4 BS bytes) per append, so an append is one select
     // and one add; the count is derived from it where needed
+    constexpr uint32_t kRowB = 4u * BS;
+    constexpr uint32_t kRowShift = BS == 256 ? 10u : 9u;
     char* const lbase = reinterpret_cast<char*>(&lst[0][0]);
     const uint32_t lp0 = (uint32_t)tid * 4u;
     uint32_t lp = lp0;
-    auto count = [&]() { return (int)((lp - lp0) >> 10); };
+    auto count = [&]() { return (int)((lp - lp0) >> kRowShift); };
     const uint32_t self_o = self_j << 4;
     for (int k = 0; k < L; ++k) lst[k][tid] = self_o;
     auto drain = [&](auto width) {
@@ -665,12 +703,12 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
     // slot it reads, so a slot at or past the count always names the lane.
     auto push = [&](uint32_t j, bool ok) {
         *reinterpret_cast<uint32_t*>(lbase + lp) = (PAD && !ok) ? self_o : j << 4;
-        lp += ok ? 1024u : 0u;
+        lp += ok ? kRowB : 0u;
     };
-    // list-full tests: at least L - 8 + 1, L - 4 + 1 or L entries
-    const uint32_t lp8 = lp0 + (L - 8) * 1024u;
-    const uint32_t lpR = lp0 + (L - R) * 1024u;
-    const uint32_t lpL = lp0 + L * 1024u;
+    // list-full tests: at least L - 8 + 1, L - R + 1 or L entries
+    const uint32_t lp8 = lp0 + (L - 8) * kRowB;
+    const uint32_t lpR = lp0 + (L - R) * kRowB;
+    const uint32_t lpL = lp0 + L * kRowB;
     auto ov_ok = [&](const float4& pj) {
         return keys_adjacent(xi, yi, zi, pj.x, pj.y, pj.z, inv);
     };
@@ -686,18 +724,20 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
         // the widths as plain values: a select between fields of a struct in
         // memory becomes a load from a selected address, kept on the stack
         const double zw0 = zt.w0, zw1 = zt.w1, zw2 = zt.w2;
-        // rows in mirror order (0, 24, 1, 23, ..., 11, 13, 12): a row off to
-        // one side fills the lists of the lanes on that side first, and its
-        // mirror row right after evens the counts out
-        for (int kr = 0; kr < kRows; ++kr) {
-            const int row = (kr & 1) ? kRows - 1 - (kr >> 1) : (kr >> 1);
+        // rows in mirror order (0, 24, 1, 23, ..., 11, 13, 12 for 5 x 5 rows):
+        // a row off to one side fills the lists of the lanes on that side
+        // first, and its mirror row right after evens the counts out
+        const int reach = (int)uni((uint32_t)d.reach), span = 2 * reach + 1, nrows = span * span;
+        for (int kr = 0; kr < nrows; ++kr) {
+            const int row = (kr & 1) ? nrows - 1 - (kr >> 1) : (kr >> 1);
+            const int rx = row / span;
+            const int ex = rx - reach, ey = row - rx * span - reach;
             uint32_t a, b;
-            col_run(row, cc, cz1, d, start, a, b);
+            col_run(ex, ey, cc, cz1, d, start, a, b);
             a = uni(a);
             b = uni(b);
-            const int ex = row / kSpan - kReach, ey = row % kSpan - kReach;
             if (zt.on && a < b) {
-                const int cls = (ex == kReach || ex == -kReach) + (ey == kReach || ey == -kReach);
+                const int cls = (ex == reach || ex == -reach) + (ey == reach || ey == -reach);
                 const double w = cls == 0 ? zw0 : (cls == 1 ? zw1 : zw2);
                 trim_run<LOC>(LOC ? cl.posl : posm, f32_down(zmn - w), f32_up(zmx + w), a, b);
                 a = uni(a);

@@ -1,0 +1,409 @@
+// sph_gathers.h — the density and force gather kernels (world.rs:204-347 with
+// the fused integrate / bounds / watchdog of world.rs:352-397 and
+// subsystem.rs:74-90) and their launch templates, included by sph_kernels.hip
+// (the step's instantiations) and sph_stats.hip (the statistics forms, which
+// run the same gathers and record the neighbour sets instead of outputs).
+// Template parameters: L the per-lane list length, MODE the waves a launch
+// serves, STATS the statistics form, BS the workgroup size (128 or 256 lanes:
+// 2 or 1 workgroups per 256-particle tile).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+
+#include "sph_types.h"
+#include "sph_device.h"
+#include "sph_kernels.h"
+
+namespace phy {
+
+// ---------------------------------------------------------------------------
+// neighbour-set statistics of the gathers themselves
+//
+// With STATS the density and force kernels run their own gather (the same
+// scan, filters, lists and exact tests as the step) and, instead of writing
+// the step's outputs, record for each particle the count, the sum and the xor
+// of SplitMix64-hashed scene-local original indices of every listed slot their
+// exact test accepts: the neighbour sets the physics sums over, for exact
+// comparison with the reference's sets.
+
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+struct SetAcc {
+    uint32_t c = 0;
+    unsigned long long s = 0, x = 0;
+    __device__ __forceinline__ void add(bool ok, uint32_t id) {
+        c += ok ? 1u : 0u;
+        s += ok ? (unsigned long long)id : 0ull;
+        x ^= ok ? mix64(id) : 0ull;
+    }
+    __device__ __forceinline__ void store(const GatherStats& g, uint32_t o) const {
+        g.cnt[o] = c;
+        g.sum[o] = s;
+        g.xr[o] = x;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// density / EOS (world.rs:204-246)
+//
+// The pair geometry and the sum run in f64: p = k (rho - rho0) cancels near
+// the rest density and would amplify an FP32 density error of ~2e-7 by
+// rho / (rho - rho0). MI355X executes f64 FMA at the unpacked FP32 rate, and the
+// compaction limits it to true neighbours.
+
+template <int L, int MODE, bool STATS, int BS>
+__global__ __launch_bounds__(BS) void k_density(SceneTable T, const float4* __restrict__ posm,
+                                                 const float4* __restrict__ posl,
+                                                 const uint32_t* __restrict__ cellv,
+                                                 const uint32_t* __restrict__ start_all,
+                                                 float* rho_out, float* p_out, float4* velm,
+                                                 float* mp_out, GatherStats gs) {
+    __shared__ uint32_t lst[L][BS];
+    const int tid = threadIdx.x;
+    const TileLoc TL = tile_loc_bs<BS>(T);
+    const int i = TL.i;
+    const bool act = TL.act;
+    const Domain d = T.dom[TL.s];
+    uint32_t ci = act ? cellv[i] : d.ncells;
+    if (MODE != kAllWaves && wave_uniform(act, ci, d) != (MODE == kUniformWaves)) return;
+    const DevParams P = T.params[TL.s];
+    const uint32_t* __restrict__ start = start_all + d.toff;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f);
+    constexpr bool LOC = MODE != kOtherWaves;
+    ColLocal cl{posl, make_float4(0.f, 0.f, 0.f, 0.f), (float)P.cell64, P.s_hq};
+    if (act) {
+        pi = posm[i];
+        if (LOC) cl.li = posl[i];
+    }
+    const double xi = pi.x, yi = pi.y, zi = pi.z;
+    double sum = 0.0;
+    SetAcc sa;
+    const uint32_t soff = STATS ? T.info[TL.s].off : 0u;
+    // four neighbour loads in flight: the register cost keeps this kernel's
+    // occupancy (force keeps two: four would cost it a wave per SIMD); the
+    // per-lane scan takes four candidates per round (six would cost a wave too)
+    gather<L, MODE, (MODE == kUniformWaves ? 1 : 4), LOC, false, 4, BS>(
+        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), cl, start, posm, lst, tid,
+        [&](uint32_t j, const float4& pj) { return density_maybe(P, pi.x, pi.y, pi.z, pj); },
+        [&](uint32_t o) { return at(posm, o); },
+        [&](uint32_t j, const float4& pj, bool listed) {
+            // the reference's own f64 distance decides r < h exactly (NaN fails)
+            const double dx = xi - (double)pj.x, dy = yi - (double)pj.y, dz = zi - (double)pj.z;
+            const double s = sq_exact(dx, dy, dz);
+            const bool valid = listed && s < P.s_thresh;
+            const double q = P.h2_64 - s;
+            const double t = (double)pj.w * (q * q * q);
+            sum += valid ? t : 0.0;
+            if (STATS) sa.add(valid, at(gs.id, j) - soff);
+        },
+        [&]() {});
+    if (!act) return;
+    if (STATS) {
+        sa.store(gs, gs.id[i]);
+        return;
+    }
+    double rho = P.c6_64 * sum;
+    double over = rho - P.rest64;
+    double p = over > 0.0 ? P.stiff64 * over : 0.0;
+    rho_out[i] = (float)rho;
+    p_out[i] = (float)p;
+    velm[i].w = (float)((double)pi.w / rho);             // m_j / rho_j of the force pass
+    mp_out[i] = (float)((double)pi.w * (p / (rho * rho)));  // m_j * p_j / rho_j^2
+}
+
+// ---------------------------------------------------------------------------
+// forces + integrate + bounds + watchdog (world.rs:252-397, subsystem.rs:74-90)
+//
+// Compensated summation over blocks: the contributions of one drained batch
+// (<= 2 L neighbours of this lane) are summed with plain FMAs, then each batch
+// partial is added to the running total with Kahan's algorithm.
+
+template <bool SHEAR>
+struct ForceAcc {
+    float b[10];            // batch partials: pressure xyz, viscosity xyz, XSPH xyz, shear
+    float s[10], c[10];     // compensated totals
+    __device__ __forceinline__ void zero() {
+        for (int k = 0; k < 10; ++k) b[k] = s[k] = c[k] = 0.f;
+    }
+    __device__ __forceinline__ void merge() {
+        for (int k = 0; k < (SHEAR ? 10 : 9); ++k) {
+            kahan_add(s[k], c[k], b[k]);
+            b[k] = 0.f;
+        }
+    }
+};
+
+struct NbrData {
+    float4 p, v;
+    float mp;
+};
+
+// Returns the exact force-pass decision for the slot (the pair's terms are
+// added only when it holds).
+template <bool SHEAR>
+__device__ __forceinline__ bool pair_force(ForceAcc<SHEAR>& A, const DevParams& P, uint32_t i,
+                                           uint32_t j, float4 pi, float4 vi, float Pi,
+                                           const NbrData& nb, bool listed) {
+    const float4 pj = nb.p;
+    float4 vj = nb.v;
+    float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+    float s = dx * dx + dy * dy + dz * dz;
+    const bool valid = force_valid_v(P, listed, i, j, pi.x, pi.y, pi.z, pj, s, vi, vj);
+    // distinct FP32 points closer than ~1e-19 underflow s; keep r tiny but finite.
+    // (the IEEE maximum: a NaN distance stays NaN, as in the reference)
+    s = __builtin_elementwise_maximum(s, 1.17549435e-38f);
+    float inv_r = __builtin_amdgcn_rsqf(s);
+    float r = s * inv_r;
+    float hr = P.h - r;
+    float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
+    float mr = vj.w;  // m_j / rho_j
+    float q = P.h2 - s;
+    // pressure: dir * (m_i m_j (P_i + P_j) * cs (h - r)^2), dir = d / r
+    float a = fmaf(pj.w, Pi, nb.mp) * (hr * hr) * inv_r;
+    // viscosity: (v_j - v_i) * (m_i m_j / rho_j * cv (h - r))
+    float bb = mr * hr;
+    // XSPH: (v_j - v_i) * (m_j / rho_j * poly6(r))
+    float w = mr * (q * q * q);
+    // a rejected or padding slot (the particle itself, a band pair outside
+    // r < h) is finite here and gets zero weight
+    a = valid ? a : 0.f;
+    bb = valid ? bb : 0.f;
+    w = valid ? w : 0.f;
+    A.b[0] = fmaf(a, dx, A.b[0]);
+    A.b[1] = fmaf(a, dy, A.b[1]);
+    A.b[2] = fmaf(a, dz, A.b[2]);
+    A.b[3] = fmaf(bb, dvx, A.b[3]);
+    A.b[4] = fmaf(bb, dvy, A.b[4]);
+    A.b[5] = fmaf(bb, dvz, A.b[5]);
+    A.b[6] = fmaf(w, dvx, A.b[6]);
+    A.b[7] = fmaf(w, dvy, A.b[7]);
+    A.b[8] = fmaf(w, dvz, A.b[8]);
+    if (SHEAR) {
+        // shear-rate proxy: |v_j - v_i| / (r + 1e-4) * (m_j / rho_j); only a
+        // non-Newtonian material (visc_n != 1) reads it
+        float dv = sqrtf(dvx * dvx + dvy * dvy + dvz * dvz);
+        float t = dv * __builtin_amdgcn_rcpf(r + P.r_eps) * mr;
+        A.b[9] += valid ? t : 0.f;
+    }
+    return valid;
+}
+
+template <int L, bool SHEAR, int MODE, bool STATS, int BS>
+__global__ __launch_bounds__(BS) void k_force(
+    SceneTable T, float dt, const float4* __restrict__ posm, const float4* __restrict__ posl,
+    const float4* __restrict__ velm,
+    const float* __restrict__ rho, const float* __restrict__ prs, const float* __restrict__ mp,
+    const uint32_t* __restrict__ cellv, const uint32_t* __restrict__ start_all,
+    const uint32_t* __restrict__ idv, const uint32_t* __restrict__ matv,
+    const float4* __restrict__ body, float4* posm_out, float4* vel_out, uint32_t* id_out,
+    uint32_t* mat_out, float4* acc_out, float* mu_out, float4* xsph_out, float4* fvisc_out,
+    GatherStats gs) {
+    __shared__ uint32_t lst[L][BS];
+    const int tid = threadIdx.x;
+    const TileLoc TL = tile_loc_bs<BS>(T);
+    const int i = TL.i;
+    const bool act = TL.act;
+    const Domain d = T.dom[TL.s];
+    uint32_t ci = act ? cellv[i] : d.ncells;
+    if (MODE != kAllWaves && wave_uniform(act, ci, d) != (MODE == kUniformWaves)) return;
+    const DevParams P = T.params[TL.s];
+    const uint32_t* __restrict__ start = start_all + d.toff;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi;
+    float rho_i = 1.f, Pi = 0.f;
+    if (act) {
+        pi = posm[i];
+        vi = velm[i];
+        rho_i = rho[i];
+        Pi = prs[i] / (rho_i * rho_i);
+    }
+    constexpr bool LOC = MODE != kOtherWaves;
+    ColLocal cl{posl, make_float4(0.f, 0.f, 0.f, 0.f), (float)P.cell64, P.s_hq};
+    if (LOC && act) cl.li = posl[i];
+    ForceAcc<SHEAR> A;
+    A.zero();
+    SetAcc sa;
+    const uint32_t soff = STATS ? T.info[TL.s].off : 0u;
+    // per-lane rounds of six candidates: a row of five half cells at rest
+    // spacing holds about five, so a row takes one round instead of two
+    gather<L, MODE, (MODE == kUniformWaves ? 1 : 2), LOC, true, 6, BS>(
+        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), cl, start, posm, lst, tid,
+        [&](uint32_t j, const float4& pj) { return force_maybe(P, pi.x, pi.y, pi.z, pj); },
+        [&](uint32_t o) { return NbrData{at(posm, o), at(velm, o), at(mp, o)}; },
+        // every slot at or past the lane's count names the lane itself, which
+        // no validity test accepts, so the listed flag is not needed
+        [&](uint32_t o, const NbrData& nb, bool) {
+            const bool ok = pair_force<SHEAR>(A, P, (uint32_t)i << 4, o, pi, vi, Pi, nb, true);
+            if (STATS) sa.add(ok, at(gs.id, o) - soff);
+        },
+        [&]() { A.merge(); });
+    if (!act) return;
+    if (STATS) {
+        sa.store(gs, gs.id[i]);
+        return;
+    }
+
+    // power-law effective viscosity (world.rs:319-327)
+    int m = 0;
+    if (matv) {
+        const uint32_t mt = matv[i];
+        mat_out[i] = mt;  // the new state keeps this step's cell order
+        if (P.n_mat > 1) m = mt < (uint32_t)P.n_mat ? (int)mt : 0;
+    }
+    // indexed through the table (a dynamic index into the register copy P
+    // would move that copy to scratch memory)
+    const DevParams* __restrict__ Pm = T.params + TL.s;
+    float kc = Pm->visc_k[m];
+    float mu;
+    if (!SHEAR) {
+        mu = kc;  // powf(x, 0) == 1 for every x
+    } else {
+        float sh = A.s[9];
+        float sreg = sh > P.shear_min ? sh : P.shear_min;
+        mu = kc * powf(sreg, Pm->visc_n[m] - 1.f);
+    }
+    float acc[3] = {0.f, 0.f, 0.f}, fvis[3] = {0.f, 0.f, 0.f};
+    if (rho_i > 0.f) {
+        float mcs = pi.w * P.cs, mcv = pi.w * P.cv * mu;
+        for (int a = 0; a < 3; ++a) {
+            fvis[a] = mcv * A.s[3 + a];
+            acc[a] = (mcs * A.s[a] + fvis[a]) / rho_i;
+        }
+    }
+    float xs[3] = {P.c6 * A.s[6], P.c6 * A.s[7], P.c6 * A.s[8]};
+
+    // integrate (world.rs:352-363) and enforce_bounds (366-397)
+    float bacc[3] = {0.f, 0.f, 0.f};
+    if (body) {
+        float4 b = body[i];
+        bacc[0] = b.x;
+        bacc[1] = b.y;
+        bacc[2] = b.z;
+    }
+    float x[3] = {pi.x, pi.y, pi.z}, v[3] = {vi.x, vi.y, vi.z};
+    uint32_t clamp = 0;  // bit a: axis a clamped to a wall (the adjoint's bounds mask)
+    for (int a = 0; a < 3; ++a) {
+        float aa = (acc[a] + P.g[a]) + bacc[a];
+        v[a] = v[a] + aa * dt;
+        v[a] = v[a] + xs[a] * P.xsph_eps;
+        x[a] = x[a] + v[a] * dt;
+        if (x[a] < P.lo[a]) {
+            x[a] = P.lo[a];
+            v[a] = -v[a] * P.damp;
+            clamp |= 1u << a;
+        } else if (x[a] > P.hi[a]) {
+            x[a] = P.hi[a];
+            v[a] = -v[a] * P.damp;
+            clamp |= 1u << a;
+        }
+    }
+    posm_out[i] = make_float4(x[0], x[1], x[2], pi.w);
+    vel_out[i] = make_float4(v[0], v[1], v[2], 0.f);
+    if (acc_out) {
+        acc_out[i] = make_float4(acc[0], acc[1], acc[2], 0.f);
+        mu_out[i] = mu;
+        xsph_out[i] = make_float4(xs[0], xs[1], xs[2], __uint_as_float(clamp));
+    }
+    // adjoint tape: viscous force per unit mu (m cv sum ...) and the raw
+    // shear-rate sum of the power law (0 for a Newtonian pass)
+    if (fvisc_out) {
+        const float u = rho_i > 0.f ? pi.w * P.cv : 0.f;
+        fvisc_out[i] = make_float4(u * A.s[3], u * A.s[4], u * A.s[5], SHEAR ? A.s[9] : 0.f);
+    }
+
+    // watchdog: smallest original index with a non-finite position / velocity
+    uint32_t id = idv[i];
+    id_out[i] = id;
+    uint32_t bp = finite3(x[0], x[1], x[2]) ? 0xffffffffu : id;
+    uint32_t bv = finite3(v[0], v[1], v[2]) ? 0xffffffffu : id;
+    if (bp != 0xffffffffu) atomicMin(&T.dom[TL.s].bad_pos, bp);
+    if (bv != 0xffffffffu) atomicMin(&T.dom[TL.s].bad_vel, bv);
+}
+
+
+// ---------------------------------------------------------------------------
+// launch templates
+
+template <int L, bool STATS, int BS>
+static void density_l(hipStream_t st, const SceneTable& T, bool split, const float4* posm,
+                      const float4* posl, const uint32_t* cellv, const uint32_t* start, float* rho,
+                      float* prs, float4* velm, float* mp, const GatherStats& gs) {
+    const dim3 grid(T.ntiles * (kTile / BS)), block(BS);
+    if (split) {
+        hipLaunchKernelGGL((k_density<L, kUniformWaves, STATS, BS>), grid, block, 0, st, T,
+                           posm, posl, cellv, start, rho, prs, velm, mp, gs);
+        hipLaunchKernelGGL((k_density<L, kOtherWaves, STATS, BS>), grid, block, 0, st, T,
+                           posm, posl, cellv, start, rho, prs, velm, mp, gs);
+    } else {
+        hipLaunchKernelGGL((k_density<L, kAllWaves, STATS, BS>), grid, block, 0, st, T, posm,
+                           posl, cellv, start, rho, prs, velm, mp, gs);
+    }
+}
+
+template <bool STATS>
+static void density_any(hipStream_t st, const SceneTable& T, const float4* posm, const float4* posl,
+                        const uint32_t* cellv, const uint32_t* start, float* rho, float* prs,
+                        float4* velm, float* mp, const Tuning& tu, const GatherStats& gs) {
+    if (T.ntiles == 0) return;
+    const bool sp = tu.split != 0;
+    switch (tu.density_list * 1000 + tu.block) {
+        case 8128: density_l<8, STATS, 128>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
+        case 16128: density_l<16, STATS, 128>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
+        case 32128: density_l<32, STATS, 128>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
+        case 8256: density_l<8, STATS, 256>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
+        case 16256: density_l<16, STATS, 256>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
+        default: density_l<32, STATS, 256>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
+    }
+}
+
+
+template <int L, bool SHEAR, int MODE, bool STATS, int BS>
+static void force_one(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a,
+                      const GatherStats& gs) {
+    hipLaunchKernelGGL((k_force<L, SHEAR, MODE, STATS, BS>), dim3(T.ntiles * (kTile / BS)), dim3(BS), 0, st,
+                       T, dt, a.posm, a.posl, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body,
+                       a.posm_out, a.vel_out, a.id_out, a.mat_out, a.acc_out, a.mu_out, a.xsph_out,
+                       a.fvisc_out, gs);
+}
+
+template <int L, bool SHEAR, bool STATS, int BS>
+static void force_l(hipStream_t st, const SceneTable& T, bool split, float dt, const ForceArgs& a,
+                    const GatherStats& gs) {
+    if (split) {
+        force_one<L, SHEAR, kUniformWaves, STATS, BS>(st, T, dt, a, gs);
+        force_one<L, SHEAR, kOtherWaves, STATS, BS>(st, T, dt, a, gs);
+    } else {
+        force_one<L, SHEAR, kAllWaves, STATS, BS>(st, T, dt, a, gs);
+    }
+}
+
+template <bool SHEAR, bool STATS, int BS>
+static void force_list(hipStream_t st, const SceneTable& T, bool split, float dt, const ForceArgs& a,
+                       int list, const GatherStats& gs) {
+    switch (list) {
+        case 8: force_l<8, SHEAR, STATS, BS>(st, T, split, dt, a, gs); break;
+        case 16: force_l<16, SHEAR, STATS, BS>(st, T, split, dt, a, gs); break;
+        default: force_l<32, SHEAR, STATS, BS>(st, T, split, dt, a, gs); break;
+    }
+}
+
+template <bool STATS>
+static void force_any(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a, bool shear,
+                      const Tuning& tu, const GatherStats& gs) {
+    if (T.ntiles == 0) return;
+    const bool sp = tu.split != 0;
+    if (tu.block == 128) {
+        if (shear) force_list<true, STATS, 128>(st, T, sp, dt, a, tu.force_list, gs);
+        else force_list<false, STATS, 128>(st, T, sp, dt, a, tu.force_list, gs);
+    } else {
+        if (shear) force_list<true, STATS, 256>(st, T, sp, dt, a, tu.force_list, gs);
+        else force_list<false, STATS, 256>(st, T, sp, dt, a, tu.force_list, gs);
+    }
+}
+
+}  // namespace phy

@@ -15,11 +15,15 @@ constexpr int kForceList = 32;     // default per-lane LDS neighbour list length
 constexpr int kDensityList = 16;   // default per-lane LDS neighbour list length of k_density
 constexpr int kAdjList = 16;       // per-lane list length of the adjoint gathers
 
-// Launch choices of the gather kernels, picked by the autotuner. The list
-// length sets the LDS per workgroup (L x 1 KiB, hence occupancy) against the
-// drain count. The force pass's compensated sums are blocked per drain, so its
-// list length is part of the summation order; the density pass's order does
-// not depend on it.
+// Launch choices of the step, picked by the autotuner. The list length sets
+// the LDS per workgroup (L x 4 block bytes, hence occupancy) against the drain
+// count. The force pass's compensated sums are blocked per drain, so its list
+// length is part of the summation order; the density pass's order does not
+// depend on it. The cell size sets the neighbour search's candidate order
+// (both passes' summation order) and its candidate count: cells of h / 2 give
+// a 5 x 5 x 5-cell window (15.6 h^3 of candidate volume), cells of h the
+// reference's 27 cells (27 h^3) in 9 rows instead of 25. The block size is
+// the gathers' workgroup size; it changes no result.
 struct Tuning {
     int density_list = kDensityList;  // 8, 16 or 32
     int force_list = kForceList;      // 8, 16 or 32
@@ -27,6 +31,8 @@ struct Tuning {
     // cell (compiled without the per-lane scan, so with fewer registers and
     // more waves in flight) and one for the others; 0: one launch for all.
     int split = 0;
+    int cell_div = 2;                 // cells per h along each axis: 2 (h / 2) or 1 (h)
+    int block = 256;                  // gather workgroup: 128 or 256 lanes
 };
 bool tuning_valid(const Tuning& t);
 

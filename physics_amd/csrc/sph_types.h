@@ -33,11 +33,14 @@ struct DevParams {
     double s_thresh;     // smallest f64 s with sqrt(s) >= h: r < h  <=>  s < s_thresh
     double h2_64, c6_64;  // f64 h*h and poly6 coefficient (density pass)
     double rest64, stiff64;
-    // Largest |dz| of a pair with r < h whose x / y half-cell columns are 0, 1
-    // or 2 apart in both (zw[0]), apart by 2 in one (zw[1]) or in both axes
-    // (zw[2]); rounded up. The uniform scan trims each z-run to the lanes'
-    // z-range widened by these.
+    // Largest |dz| of a pair with r < h whose x / y cell columns are less than
+    // cdiv apart in both (zw[0]), cdiv apart in one (zw[1]) or in both axes
+    // (zw[2]); rounded up (all three equal h for cells of h). The uniform scan
+    // trims each z-run to the lanes' z-range widened by these.
     double zw[3];
+    double cell64;       // edge of a dense-table cell: h / cdiv
+    int32_t cdiv;        // cells per support radius along each axis: 1 (cells of h) or 2 (h / 2)
+    int32_t pad0;
     float s_lo, s_hi;    // FP32 filter: s32 < s_lo => inside; s32 > s_hi => outside
     float s_hq;          // s_thresh (1 + 2^-16) rounded up: bound of the column-local filter
     float h, h2;         // FP32 support radius and its square
@@ -61,19 +64,23 @@ struct SceneInfo {
     long long bkey[6];       // cell keys of the scene's bounds box: lo xyz, hi xyz
 };
 
-// Dense grid domain of one scene for one step. Keys are half-cell keys
-// floor(2 fl(x (1/h))), an exact refinement of the reference's i64 cell keys
-// (grid.rs:26-33: reference key = half key >> 1); the table covers kmin ..
-// kmin + dims - 1 per axis with z the fastest-varying axis, so the
-// z-adjacent half cells of a neighbour row are one contiguous run of sorted
-// particles. Local cell index ncells is the scene's overflow segment. The
-// scene's cells 0 .. ncells sit at toff in the concatenated cell table of the
-// step.
+// Dense grid domain of one scene for one step. Keys are cell keys
+// floor(cdiv fl(x (1/h))) with cdiv = 1 (the reference's i64 cell keys,
+// grid.rs:26-33) or 2 (half cells, an exact refinement: scaling by 2 is exact,
+// so reference key = half key >> 1); the table covers kmin .. kmin + dims - 1
+// per axis with z the fastest-varying axis, so the z-adjacent cells of a
+// neighbour row are one contiguous run of sorted particles. Local cell index
+// ncells is the scene's overflow segment. The scene's cells 0 .. ncells sit at
+// toff in the concatenated cell table of the step.
 struct Domain {
     long long kmin[3];
     long long dims[3];
     uint32_t ncells;
     uint32_t toff;
+    // cells per support radius (cdiv of the scene's parameters): a dense
+    // particle's candidates lie within +-reach cells per axis
+    int32_t reach;
+    int32_t pad0;
     // AABB of the finite keys of the current positions (filled by k_aabb).
     long long amin[3];
     long long amax[3];

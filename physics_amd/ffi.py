@@ -136,10 +136,16 @@ class PhyHeatGrid(C.Structure):
 
 
 class PhyTuning(C.Structure):
-    """Per-lane LDS neighbour-list lengths of the density and force passes and
-    the launch form (split: uniform-cell waves and the others in two launches)."""
+    """Per-lane LDS neighbour-list lengths of the density and force passes, the
+    launch form (split: uniform-cell waves and the others in two launches), the
+    grid's cells per h (cell_div 2: h / 2, 1: h) and the gathers' workgroup size."""
 
-    _fields_ = [("density_list", C.c_int32), ("force_list", C.c_int32), ("split", C.c_int32)]
+    _fields_ = [("density_list", C.c_int32), ("force_list", C.c_int32), ("split", C.c_int32),
+                ("cell_div", C.c_int32), ("block", C.c_int32)]
+
+    def as_tuple(self):
+        return (int(self.density_list), int(self.force_list), int(self.split), int(self.cell_div),
+                int(self.block))
 
 
 _lib = None
@@ -521,18 +527,21 @@ class FluidWorld(World):
             raise PhyError(last_error())
 
     def tuning(self):
+        """(density_list, force_list, split, cell_div, block)"""
         t = PhyTuning()
         if lib().phy_fluid_get_tuning(self._h, C.byref(t)) != 0:
             raise PhyError(last_error())
-        return int(t.density_list), int(t.force_list), int(t.split)
+        return t.as_tuple()
 
-    def set_tuning(self, density_list: int, force_list: int, split: int = 0) -> None:
-        t = PhyTuning(density_list, force_list, split)
+    def set_tuning(self, density_list: int, force_list: int, split: int = 0, cell_div: int = 2,
+                   block: int = 256) -> None:
+        t = PhyTuning(density_list, force_list, split, cell_div, block)
         if lib().phy_fluid_set_tuning(self._h, C.byref(t)) != 0:
             raise PhyError(last_error())
 
     def autotune(self, dt: float, steps: int = 5, cache_path: str | None = None):
-        """-> (cached?, (density_list, force_list, split), ms[12] of the timed candidates)."""
+        """-> (cached?, (density_list, force_list, split, cell_div, block), ms[12]
+        of the timed list candidates)."""
         return _autotune(lib().phy_fluid_autotune, self._h, dt, steps, cache_path)
 
     def flat_export(self):
@@ -641,7 +650,7 @@ def _autotune(fn, h, dt, steps, cache_path):
     rc = fn(h, dt, int(steps), path, C.byref(t), _dp(ms))
     if rc < 0:
         raise PhyError(last_error())
-    return bool(rc == 1), (int(t.density_list), int(t.force_list), int(t.split)), ms
+    return bool(rc == 1), t.as_tuple(), ms
 
 
 def device_mem_info():
@@ -746,10 +755,11 @@ class FluidBatch:
     def tuning(self):
         t = PhyTuning()
         self._ok(lib().phy_fluid_batch_get_tuning(self._h, C.byref(t)))
-        return int(t.density_list), int(t.force_list), int(t.split)
+        return t.as_tuple()
 
-    def set_tuning(self, density_list: int, force_list: int, split: int = 0) -> None:
-        t = PhyTuning(density_list, force_list, split)
+    def set_tuning(self, density_list: int, force_list: int, split: int = 0, cell_div: int = 2,
+                   block: int = 256) -> None:
+        t = PhyTuning(density_list, force_list, split, cell_div, block)
         self._ok(lib().phy_fluid_batch_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, dt: float, steps: int = 5, cache_path: str | None = None):

@@ -86,18 +86,34 @@ def test_gather_sets_column_after_60_steps():
     _assert_sets(g, _state_pair(g, op), "12^3 column, step 60")
 
 
+@pytest.mark.parametrize("block", [256, 128])
+@pytest.mark.parametrize("cell_div", [2, 1])
 @pytest.mark.parametrize("dl", [8, 16, 32])
 @pytest.mark.parametrize("fl", [8, 16, 32])
 @pytest.mark.parametrize("split", [0, 1])
-def test_gather_sets_under_every_tuning(dl, fl, split):
+def test_gather_sets_under_every_tuning(dl, fl, split, cell_div, block):
     # a compressed 20^3 column (column waves with trimmed rows, per-lane
-    # waves at its edges) under every list length and launch form
+    # waves at its edges) under every list length, launch form, cell size and
+    # workgroup size
     sc = scenes.dam_break((20, 20, 20), seed=2)
     g, o = P.make_pair(sc.pos, mass=sc.mass, bounds=(sc.bounds_min, sc.bounds_max))
     g.step(P.DT32, 120)
-    g.set_tuning(dl, fl, split)
+    g.set_tuning(dl, fl, split, cell_div, block)
     _, op = P.params_pair(bounds=(sc.bounds_min, sc.bounds_max))
-    _assert_sets(g, _state_pair(g, op), f"20^3 step 120 lists {dl}/{fl} split {split}")
+    _assert_sets(g, _state_pair(g, op), f"20^3 step 120 lists {dl}/{fl} split {split} cells h/{cell_div} "
+                 f"block {block}")
+
+
+@pytest.mark.parametrize("cell_div", [2, 1])
+def test_gather_sets_with_cells_of_h_and_h2_at_rest_and_collapsed(cell_div):
+    sc = scenes.dam_break((40, 40, 40), seed=3)
+    gp, op = P.params_pair(bounds=(sc.bounds_min, sc.bounds_max))
+    g = ffi.FluidWorld(gp)
+    g.set_particles_f32(sc.pos, None, np.full(len(sc.pos), sc.mass, np.float32))
+    g.set_tuning(16, 32, 0, cell_div, 256)
+    _assert_sets(g, _state_pair(g, op), f"40^3 at rest, cells h/{cell_div}")
+    g.step(P.DT32, 150)
+    _assert_sets(g, _state_pair(g, op), f"40^3 step 150, cells h/{cell_div}")
 
 
 def test_gather_sets_of_the_shear_and_recorded_variants():

@@ -1,9 +1,10 @@
-"""Autotuner and launch choices of the gather kernels.
+"""Autotuner and launch choices of the step.
 
-Every list length must keep one-step parity with the f64 oracle; the density
-pass's summation order does not depend on its list length (bit-identical rho);
-the autotuner must leave the state untouched and its cached choice must be
-reapplied without timing.
+Every list length, launch form, cell size (h / 2, h) and workgroup size
+(256, 128) must keep one-step parity with the f64 oracle; the density pass's
+summation order does not depend on its list length (bit-identical rho) and no
+result depends on the workgroup size; the autotuner must leave the state
+untouched and its cached choice must be reapplied without timing.
 """
 import os
 
@@ -24,18 +25,52 @@ def _compressed_block(seed=21):
     return pos, vel
 
 
+@pytest.mark.parametrize("block", (256, 128))
+@pytest.mark.parametrize("cell_div", (2, 1))
 @pytest.mark.parametrize("split", (0, 1))
 @pytest.mark.parametrize("dl", LISTS)
 @pytest.mark.parametrize("fl", LISTS)
-def test_every_launch_choice_keeps_one_step_parity(dl, fl, split):
+def test_every_launch_choice_keeps_one_step_parity(dl, fl, split, cell_div, block):
     pos, vel = _compressed_block()
     g, o = make_pair(pos, vel, mass=0.9)
-    g.set_tuning(dl, fl, split)
-    assert g.tuning() == (dl, fl, split)
+    g.set_tuning(dl, fl, split, cell_div, block)
+    assert g.tuning() == (dl, fl, split, cell_div, block)
     x0 = step_both(g, o)
     err = compare_step(g, o, x0)
-    print(f"density_list {dl} force_list {fl} split {split}: {err}", flush=True)
+    print(f"density_list {dl} force_list {fl} split {split} cell_div {cell_div} block {block}: {err}",
+          flush=True)
     assert max(err.values()) < TOL_STEP
+
+
+@pytest.mark.parametrize("cell_div", (2, 1))
+def test_workgroup_size_does_not_change_results(cell_div):
+    """128- and 256-lane workgroups walk the same lanes through the same scans."""
+    sc = scenes.dam_break((20, 20, 20), seed=9)
+    p = scenes.apply_scene(ffi.PhySphParams.defaults(), sc)
+    outs = []
+    for block in (256, 128):
+        g = ffi.FluidWorld(p)
+        g.set_particles_f32(sc.pos, None, np.full(len(sc.pos), sc.mass, np.float32))
+        g.set_tuning(16, 32, 0, cell_div, block)
+        g.step(DT32, 120)
+        outs.append((g.get("pos", f32=True), g.get("vel", f32=True)))
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
+def test_cell_size_switches_keep_per_step_parity_through_a_collapse():
+    """Cells of h and of h / 2 alternate along a collapse (tables resized each
+    time); every step keeps one-step parity with the oracle fed the GPU state."""
+    from tests.parity import sync_oracle_to_gpu
+    sc = scenes.dam_break((14, 14, 14), seed=5)
+    g, o = make_pair(sc.pos, mass=sc.mass, bounds=(sc.bounds_min, sc.bounds_max))
+    worst = 0.0
+    for k in range(80):
+        g.set_tuning(16, 32, 0, 1 if (k // 10) % 2 else 2, 256)
+        sync_oracle_to_gpu(g, o)
+        x0 = step_both(g, o)
+        worst = max(worst, max(compare_step(g, o, x0).values()))
+    print(f"worst one-step error over 80 steps switching cell sizes: {worst:.2e}", flush=True)
+    assert worst < TOL_STEP
 
 
 @pytest.mark.parametrize("fl", LISTS)
@@ -76,6 +111,7 @@ def test_autotune_restores_state_and_caches(tmp_path):
     cached, choice, ms = g.autotune(DT32, 4, cache)
     print("choice", choice, "ms", ms, flush=True)
     assert not cached and choice[0] in LISTS and choice[1] in LISTS and choice[2] in (0, 1)
+    assert choice[3] in (1, 2) and choice[4] in (128, 256)
     assert np.isfinite(ms).all()
     assert np.array_equal(g.get("pos", f32=True), x) and np.array_equal(g.get("vel", f32=True), v)
     assert os.path.exists(cache)
