@@ -353,9 +353,9 @@ int32_t phy_fluid_set_graph(PhyWorldHandle *w, int32_t on);
  * density and of the force pass (8, 16 or 32; default 16 / 32), the launch
  * form (split = 1: each pass is one launch for the waves whose lanes share a
  * grid cell plus one for the others; 0: one launch), the neighbour grid's
- * cell size (cell_div = 2: cells of h / 2, a 5 x 5 x 5-cell window; 1: cells
- * of h, the reference's 27 cells) and the gathers' workgroup size (block =
- * 128 or 256 lanes). The force pass's compensated sums are blocked per list
+ * cell size (cell_div = 2: cells of h / 2, a 5 x 5 x 5-cell window, the
+ * default; 1: cells of h, the reference's 27 cells) and the gathers'
+ * workgroup size (block = 128 lanes, the default, or 256). The force pass's compensated sums are blocked per list
  * drain, and the cell size orders the candidates, so force_list and cell_div
  * are part of the summation order: keep them fixed (for example through the
  * autotuner's cache) when runs must repeat bit for bit. The block size

@@ -612,8 +612,8 @@ std::string DeviceFluid::tuning_key() const {
 }
 
 // Cache lines: "<key> density_list=<L> force_list=<L> split=<0|1> cell_div=<1|2>
-// block=<128|256>", one per key (lines without the last two fields take cells
-// of h / 2 and 256-lane workgroups).
+// block=<128|256>", one per key (lines without the last two fields take the
+// defaults: cells of h / 2 and 128-lane workgroups).
 static bool read_tuning(const std::string& path, const std::string& key, Tuning& t) {
     std::ifstream f(path);
     std::string line;

@@ -337,37 +337,31 @@ struct Cursor {
     int row, nrows;
     uint32_t x0, x1, y0, y1, z0, z1, mul;
     uint32_t j, jend, o0, o1;
+    uint32_t pa, pb;  // bounds of row `row`, loaded ahead
 
     __device__ __forceinline__ bool in_overflow() const { return row > nrows; }
 
-    // Bounds [ja, jb) of row k; a row past the window (k = nrows) is empty.
-    __device__ __forceinline__ void run(int k, const Domain& d, const uint32_t* __restrict__ start,
-                                        uint32_t& ja, uint32_t& jb) const {
-        // k / step for k < 2^14 by multiply and shift (step 3: 0x15556, 5:
-        // 0xCCCD, 6: 0xAAAB)
+    // Issues the loads of row k's bounds into pa, pb (k >= nrows reads as an
+    // empty row: both loads from one entry).
+    __device__ __forceinline__ void issue(int k, const Domain& d, const uint32_t* __restrict__ start) {
         const uint32_t q = ((uint32_t)k * mul) >> 18;
         const uint32_t step = mul == 0xAAABu ? 6u : (mul == 0xCCCDu ? 5u : 3u);
         const uint32_t x = x0 + q, y = y0 + ((uint32_t)k - q * step);
-        const bool ok = x <= x1 && y <= y1 && z0 <= z1;
-        // loads from a valid address either way, then a select of values (a
-        // select of addresses would put a zero on the stack)
+        const bool ok = k < nrows && x <= x1 && y <= y1 && z0 <= z1;
         const uint32_t base = ok ? (x * (uint32_t)d.dims[1] + y) * (uint32_t)d.dims[2] : 0u;
-        const uint32_t sa = start[base + (ok ? z0 : 0u)], sb = start[base + (ok ? z1 + 1u : 0u)];
-        ja = ok ? sa : 0u;
-        jb = ok ? sb : 0u;
+        pa = start[base + (ok ? z0 : 0u)];
+        pb = start[base + (ok ? z1 + 1u : 0u)];
     }
 
     // Advances to the next non-empty run: the window's rows, then the
-    // overflow segment. Branch-free per iteration (row nrows reads as empty
-    // and is replaced by the overflow bounds).
+    // overflow segment. The row after the one taken is loaded ahead.
     __device__ __forceinline__ void next_run(const Domain& d, const uint32_t* __restrict__ start) {
         while (j >= jend && row <= nrows) {
-            uint32_t ja, jb;
-            run(row, d, start, ja, jb);
             const bool ovr = row == nrows;
-            j = ovr ? o0 : ja;
-            jend = ovr ? o1 : jb;
+            j = ovr ? o0 : pa;
+            jend = ovr ? o1 : pb;
             ++row;
+            issue(row, d, start);
         }
     }
 
@@ -409,6 +403,7 @@ struct Cursor {
                 z1 = (uint32_t)hz;
             }
         }
+        issue(0, d, start);
         next_run(d, start);
     }
 };

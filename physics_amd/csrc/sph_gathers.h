@@ -162,19 +162,20 @@ __device__ __forceinline__ bool pair_force(ForceAcc<SHEAR>& A, const DevParams& 
     float r = s * inv_r;
     float hr = P.h - r;
     float dvx = vj.x - vi.x, dvy = vj.y - vi.y, dvz = vj.z - vi.z;
-    float mr = vj.w;  // m_j / rho_j
+    // m_j / rho_j, zero for a rejected or padding slot (the particle itself, a
+    // band pair outside r < h, a coincident point): its viscosity, XSPH and
+    // shear weights vanish with it (h - r, q and the velocity differences are
+    // finite there), so they need no selects of their own
+    const float mr = valid ? vj.w : 0.f;
     float q = P.h2 - s;
-    // pressure: dir * (m_i m_j (P_i + P_j) * cs (h - r)^2), dir = d / r
+    // pressure: dir * (m_i m_j (P_i + P_j) * cs (h - r)^2), dir = d / r; selected
+    // away for a rejected slot, whose m p / rho^2 may be NaN (rho = 0)
     float a = fmaf(pj.w, Pi, nb.mp) * (hr * hr) * inv_r;
+    a = valid ? a : 0.f;
     // viscosity: (v_j - v_i) * (m_i m_j / rho_j * cv (h - r))
     float bb = mr * hr;
     // XSPH: (v_j - v_i) * (m_j / rho_j * poly6(r))
     float w = mr * (q * q * q);
-    // a rejected or padding slot (the particle itself, a band pair outside
-    // r < h) is finite here and gets zero weight
-    a = valid ? a : 0.f;
-    bb = valid ? bb : 0.f;
-    w = valid ? w : 0.f;
     A.b[0] = fmaf(a, dx, A.b[0]);
     A.b[1] = fmaf(a, dy, A.b[1]);
     A.b[2] = fmaf(a, dz, A.b[2]);
@@ -189,7 +190,7 @@ __device__ __forceinline__ bool pair_force(ForceAcc<SHEAR>& A, const DevParams& 
         // non-Newtonian material (visc_n != 1) reads it
         float dv = sqrtf(dvx * dvx + dvy * dvy + dvz * dvz);
         float t = dv * __builtin_amdgcn_rcpf(r + P.r_eps) * mr;
-        A.b[9] += valid ? t : 0.f;
+        A.b[9] += t;
     }
     return valid;
 }

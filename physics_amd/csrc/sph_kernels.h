@@ -32,7 +32,7 @@ struct Tuning {
     // more waves in flight) and one for the others; 0: one launch for all.
     int split = 0;
     int cell_div = 2;                 // cells per h along each axis: 2 (h / 2) or 1 (h)
-    int block = 256;                  // gather workgroup: 128 or 256 lanes
+    int block = 128;                  // gather workgroup: 128 or 256 lanes
 };
 bool tuning_valid(const Tuning& t);
 

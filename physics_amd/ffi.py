@@ -534,7 +534,7 @@ class FluidWorld(World):
         return t.as_tuple()
 
     def set_tuning(self, density_list: int, force_list: int, split: int = 0, cell_div: int = 2,
-                   block: int = 256) -> None:
+                   block: int = 128) -> None:
         t = PhyTuning(density_list, force_list, split, cell_div, block)
         if lib().phy_fluid_set_tuning(self._h, C.byref(t)) != 0:
             raise PhyError(last_error())
@@ -758,7 +758,7 @@ class FluidBatch:
         return t.as_tuple()
 
     def set_tuning(self, density_list: int, force_list: int, split: int = 0, cell_div: int = 2,
-                   block: int = 256) -> None:
+                   block: int = 128) -> None:
         t = PhyTuning(density_list, force_list, split, cell_div, block)
         self._ok(lib().phy_fluid_batch_set_tuning(self._h, C.byref(t)))
 
