@@ -813,13 +813,18 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 bool t[R];
 #pragma unroll
                 for (int u = 0; u < R; ++u) t[u] = (j + (uint32_t)u <= last) & test(j + u, pq[u]);
-                if (cur.in_overflow()) {
+                // The count is at most L - R here, so the writes stay inside
+                // the list. Each side of the wave-wide overflow test appends
+                // its own flags: flags that crossed the merge would be carried
+                // as bytes in vector registers and unpacked again.
+                if (wave_any(cur.in_overflow())) {
+                    const bool inov = cur.in_overflow();
 #pragma unroll
-                    for (int u = 0; u < R; ++u) t[u] = t[u] && ov_ok(pq[u]);
+                    for (int u = 0; u < R; ++u) push(j + u, t[u] & (!inov | ov_ok(pq[u])));
+                } else {
+#pragma unroll
+                    for (int u = 0; u < R; ++u) push(j + u, t[u]);
                 }
-                // the count is at most L - R here, so the writes stay inside the list
-#pragma unroll
-                for (int u = 0; u < R; ++u) push(j + u, t[u]);
                 cur.j = min(j + (uint32_t)R, cur.jend);
                 if (cur.j >= cur.jend) cur.next_run(d, start);
             }

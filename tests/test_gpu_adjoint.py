@@ -290,8 +290,9 @@ def test_rollout_block_gradients_match_fd():
 def test_inverse_design_rollout_gradients_match_fd():
     """The batched inverse-design configuration: 16 rollouts of 64x64x32
     particles with per-rollout viscosity, 200 FP32 steps, checkpoint every 20,
-    differentiated together in one batch. For rollouts 0, 7 and 15: dL/d
-    stiffness, dL/d visc_k[0] and three seeded v0 directions are checked
+    differentiated together in one batch. For rollouts 0 and 15 (the first
+    and the last of the batch): dL/d stiffness, dL/d visc_k[0] and two seeded
+    v0 directions are checked
     against central FD (eps = 1e-4, refined as in _direction_ok) of the f64
     oracle on that rollout alone: the parameter derivatives to 1e-3 of their
     own value, the v0 directions by _direction_ok's bar."""
@@ -320,7 +321,7 @@ def test_inverse_design_rollout_gradients_match_fd():
     _, v0b, dk, dmu = b.tape_backward(xbar, None, rhobar)
     b.tape_end()
 
-    for s in (0, 7, 15):
+    for s in (0, 15):
         sc, mu0 = ro[s]
         n = counts[s]
         op = scenes.apply_scene(O.Params.defaults(), sc)
@@ -345,7 +346,7 @@ def test_inverse_design_rollout_gradients_match_fd():
         vb = v0b[off[s]:off[s + 1]].astype(np.float64)
         cases = [("stiffness", None, 1.0, 0.0, float(dk[s])),
                  ("visc_k[0]", None, 0.0, 1.0, float(dmu[s]))]
-        for t in range(3):
+        for t in range(2):
             dv = np.random.default_rng(77 + 100 * s + t).normal(size=(n, 3))
             cases.append((f"v0 direction {t}", dv, 0.0, 0.0, float((vb * dv).sum())))
         res = []
