@@ -2,7 +2,7 @@
 // (crates/phy-fluid/src/sph/world.rs:183-192):
 //
 //   k_aabb / k_domain_finalize   dense-grid domain from the finite cell keys
-//   k_bin, k_scan_*, k_scatter   counting sort of particles into cells of size h
+//   k_bin, k_scan_*, k_scatter   counting sort of particles into cells of size h / 2 (or h)
 //   k_reorder                    deterministic within-cell rank + gather to cell order
 //   k_density                    rho / p gather (world.rs:204-246)
 //   k_force                      forces (world.rs:252-347) fused with integrate
@@ -254,9 +254,10 @@ __global__ __launch_bounds__(256) void k_bin(SceneTable T, const float4* __restr
     }
 }
 
-// Three-phase exclusive scan over the concatenated cell table (every scene's
+// Two-phase exclusive scan over the concatenated cell table (every scene's
 // cells and overflow cell) with a fixed grid; the table size is read on the
-// device. Phase 3 also clears counts for the next step.
+// device. Phase 1 sums each chunk; phase 2 scans the chunks and also clears
+// counts for the next step.
 __device__ __forceinline__ void chunk_of(uint32_t total, int b, int nb, uint32_t& lo, uint32_t& hi) {
     uint32_t per = (total + nb - 1) / nb;
     lo = min(total, per * (uint32_t)b);
@@ -287,25 +288,9 @@ __global__ __launch_bounds__(256) void k_scan_partial(const uint32_t* __restrict
     if (threadIdx.x == 0) partial[blockIdx.x] = v;
 }
 
-// single block of 1024 threads; nb <= 1024
-__global__ __launch_bounds__(1024) void k_scan_top(uint32_t* partial, int nb,
-                                                   const uint32_t* __restrict__ table_total,
-                                                   uint32_t* start) {
-    __shared__ uint32_t sh[1024];
-    int t = threadIdx.x;
-    uint32_t v = t < nb ? partial[t] : 0;
-    sh[t] = v;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        uint32_t a = t >= o ? sh[t - o] : 0;
-        __syncthreads();
-        sh[t] += a;
-        __syncthreads();
-    }
-    if (t < nb) partial[t] = sh[t] - v;  // exclusive
-    if (t == 1023) start[*table_total] = sh[1023];
-}
-
+// Phase 2: workgroup b scans chunk b from the sum of the chunk totals before
+// it (at most kScanBlocks - 1 of them, summed by the workgroup itself), and
+// the last workgroup writes the grand total after the table.
 __global__ __launch_bounds__(256) void k_scan_final(uint32_t* counts,
                                                     const uint32_t* __restrict__ table_total,
                                                     const uint32_t* __restrict__ partial,
@@ -314,8 +299,10 @@ __global__ __launch_bounds__(256) void k_scan_final(uint32_t* counts,
     uint32_t total = *table_total;
     uint32_t lo, hi;
     chunk_of(total, blockIdx.x, gridDim.x, lo, hi);
-    uint32_t carry = partial[blockIdx.x];
     int t = threadIdx.x;
+    uint32_t v0 = 0;
+    for (int k = t; k < (int)blockIdx.x; k += 256) v0 += partial[k];
+    uint32_t carry = block_sum_u32(v0, sh);
     for (uint32_t base = lo; base < hi; base += 256) {
         uint32_t c = base + t;
         uint32_t v = c < hi ? counts[c] : 0;
@@ -340,6 +327,7 @@ __global__ __launch_bounds__(256) void k_scan_final(uint32_t* counts,
         carry += tot;
         __syncthreads();
     }
+    if (blockIdx.x == gridDim.x - 1 && t == 0) start[total] = carry;
 }
 
 // Within-cell order key of a dense-cell member: z as order-preserving bits
@@ -617,8 +605,6 @@ void launch_sort(hipStream_t st, const SceneTable& T, const SortBuffers& b) {
                            b.counts);
     hipLaunchKernelGGL(k_scan_partial, dim3(kScanBlocks), dim3(256), 0, st, b.counts, T.table_total,
                        b.partial);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, b.partial, kScanBlocks, T.table_total,
-                       b.start);
     hipLaunchKernelGGL(k_scan_final, dim3(kScanBlocks), dim3(256), 0, st, b.counts, T.table_total,
                        b.partial, b.start);
     if (T.n > 0)
@@ -633,7 +619,6 @@ void launch_sort(hipStream_t st, const SceneTable& T, const SortBuffers& b) {
 void launch_exclusive_scan(hipStream_t st, uint32_t* counts, const uint32_t* total, uint32_t* partial,
                            uint32_t* start) {
     hipLaunchKernelGGL(k_scan_partial, dim3(kScanBlocks), dim3(256), 0, st, counts, total, partial);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(1024), 0, st, partial, kScanBlocks, total, start);
     hipLaunchKernelGGL(k_scan_final, dim3(kScanBlocks), dim3(256), 0, st, counts, total, partial, start);
 }
 
