@@ -13,9 +13,13 @@ kernel it prints the launches per step and the mean duration (kernel trace)
 or the mean counter values per launch (counter collection); dispatches outside
 the 1M world (the other legs) are summarised per kernel and grid size.
 
-usage: python -m physics_amd.trace_windows CSV [--warmup 5] [--steps 20]
+usage: python -m physics_amd.trace_windows CSV [CSV ...] [--warmup 5] [--steps 20]
            [--collapsed-start 300] [--run-steps 1000] [--particles 1000000]
-           [--lines-for KERNEL]   (bench's counter-summary lines for KERNEL)
+           [--lines-for KERNEL [--fetch-scale B --write-scale B]]
+Several CSVs (one per rocprofv3 run of the same command, e.g. one counter
+each) are merged per window and kernel. --lines-for prints the counter-summary
+lines bench.py reads, with HBM bytes per launch from FETCH_SIZE and WRITE_SIZE
+scaled by a plain copy's bytes per counter unit.
 """
 from __future__ import annotations
 
@@ -76,12 +80,14 @@ def windows(a):
     return w
 
 
-STATS_FORM = re.compile(r"^(k_density<\d+, \d, true|k_force<\d+, (true|false), \d, true)")
+# not part of a step: the gathers' statistics forms and the readbacks and
+# copies bench.py makes between a window's chunks (outside its timed brackets)
+NOT_STEP = re.compile(r"^(k_density<\d+, \d, true|k_force<\d+, (true|false), \d, true|k_unsort|__amd_rocclr)")
 
 
 def split(disps, force, a):
     """{window: [dispatches]} for the first world's windows and the full run
-    (the statistics forms of the gathers, run between windows, left out)."""
+    (dispatches that are not part of a step left out)."""
     first = a.collapsed_start + a.steps if len(windows(a)) > 1 else a.warmup + a.steps
     out = collections.OrderedDict()
 
@@ -95,7 +101,7 @@ def split(disps, force, a):
             prev = force[f0 - 1]["end"] if f0 > 0 else -1
             lo = min(d["start"] for d in disps if d["start"] > prev and d["name"] == "k_domain_reset") - 1
         hi = force[f0 + s1 - 1]["end"]
-        out[tag] = [d for d in disps if d["start"] > lo and d["end"] <= hi and not STATS_FORM.match(d["name"])]
+        out[tag] = [d for d in disps if d["start"] > lo and d["end"] <= hi and not NOT_STEP.match(d["name"])]
 
     for tag, f0, s0, s1 in windows(a):
         take(tag, f0, s0, s1)
