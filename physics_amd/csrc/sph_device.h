@@ -8,6 +8,8 @@
 #include <stdint.h>
 #include <math.h>
 
+#include <type_traits>
+
 #include "sph_types.h"
 
 namespace phy {
@@ -324,89 +326,10 @@ __device__ __forceinline__ void for_each_candidate(uint32_t ci, float xi, float 
 //    common case once the column has collapsed, ~100 particles per half cell). The
 //    candidate sequence is then wave-uniform: loop control runs on the scalar
 //    unit and candidate loads are uniform.
-//  * per-lane: each lane walks its own rows with a cursor, one candidate per round.
+//  * per-lane: each lane has its own window of rows; the wave walks the rows
+//    together, each lane loading its own z-run of the row, six candidates of
+//    it per round.
 
-struct Cursor {
-    // The lane's window as rows of a rectangle of table columns [x0, x1] x
-    // [y0, y1] with z cells z0..z1 (empty when z0 > z1), walked row by row:
-    // x = x0 + row / step, y = y0 + row % step, step 2 reach + 1 (dense
-    // particle: the 5 x 5 half-cell or 3 x 3 cell window) or 3 reach
-    // (out-of-domain particle: the in-domain cells of its 27 reference cells).
-    // 32-bit table-relative values with wrapping arithmetic: a row left of the
-    // table wraps to a huge x and fails x <= x1.
-    int row, nrows;
-    uint32_t x0, x1, y0, y1, z0, z1, mul;
-    uint32_t j, jend, o0, o1;
-    uint32_t pa, pb;  // bounds of row `row`, loaded ahead
-
-    __device__ __forceinline__ bool in_overflow() const { return row > nrows; }
-
-    // Issues the loads of row k's bounds into pa, pb (k >= nrows reads as an
-    // empty row: both loads from one entry).
-    __device__ __forceinline__ void issue(int k, const Domain& d, const uint32_t* __restrict__ start) {
-        const uint32_t q = ((uint32_t)k * mul) >> 18;
-        const uint32_t step = mul == 0xAAABu ? 6u : (mul == 0xCCCDu ? 5u : 3u);
-        const uint32_t x = x0 + q, y = y0 + ((uint32_t)k - q * step);
-        const bool ok = k < nrows && x <= x1 && y <= y1 && z0 <= z1;
-        const uint32_t base = ok ? (x * (uint32_t)d.dims[1] + y) * (uint32_t)d.dims[2] : 0u;
-        pa = start[base + (ok ? z0 : 0u)];
-        pb = start[base + (ok ? z1 + 1u : 0u)];
-    }
-
-    // Advances to the next non-empty run: the window's rows, then the
-    // overflow segment. The row after the one taken is loaded ahead.
-    __device__ __forceinline__ void next_run(const Domain& d, const uint32_t* __restrict__ start) {
-        while (j >= jend && row <= nrows) {
-            const bool ovr = row == nrows;
-            j = ovr ? o0 : pa;
-            jend = ovr ? o1 : pb;
-            ++row;
-            issue(row, d, start);
-        }
-    }
-
-    __device__ __forceinline__ void init(bool act, uint32_t ci, float xi, float yi, float zi,
-                                         const Domain& d, double inv,
-                                         const uint32_t* __restrict__ start) {
-        row = 0;
-        j = jend = 0;
-        o0 = start[d.ncells];
-        o1 = start[d.ncells + 1];
-        const bool dense = ci < d.ncells;
-        const bool half = d.reach == 2;
-        mul = dense ? (half ? 0xCCCDu : 0x15556u) : (half ? 0xAAABu : 0x15556u);
-        nrows = dense ? (half ? 25 : 9) : (half ? 36 : 9);
-        x0 = y0 = z0 = 1u;
-        x1 = y1 = z1 = 0u;  // empty
-        if (!act) return;
-        if (dense) {
-            const CellCoord cc = decode(ci, d);
-            const uint32_t cx = (uint32_t)cc.cx, cy = (uint32_t)cc.cy, cz = (uint32_t)cc.cz;
-            const uint32_t r = (uint32_t)d.reach;
-            x0 = cx - r;
-            y0 = cy - r;
-            x1 = min(cx + r, (uint32_t)d.dims[0] - 1u);
-            y1 = min(cy + r, (uint32_t)d.dims[1] - 1u);
-            z0 = cz > r ? cz - r : 0u;
-            z1 = min(cz + r, (uint32_t)d.dims[2] - 1u);
-        } else {
-            long long lx, hx, ly, hy, lz, hz;
-            ref_block(key1(xi, inv), d, 0, lx, hx);
-            ref_block(key1(yi, inv), d, 1, ly, hy);
-            ref_block(key1(zi, inv), d, 2, lz, hz);
-            if (lx <= hx && ly <= hy && lz <= hz) {
-                x0 = (uint32_t)lx;
-                x1 = (uint32_t)hx;
-                y0 = (uint32_t)ly;
-                y1 = (uint32_t)hy;
-                z0 = (uint32_t)lz;
-                z1 = (uint32_t)hz;
-            }
-        }
-        issue(0, d, start);
-        next_run(d, start);
-    }
-};
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
 
@@ -449,8 +372,8 @@ __device__ __forceinline__ T at(const T* __restrict__ base, uint32_t o) {
 // with PAD each drained slot is reset to self_j.
 // Which waves a gather launch serves: every wave, only waves whose active
 // lanes all sit in one dense cell (uniform scan), or only the others (per-lane
-// scan). The forward passes launch the last two separately, so the uniform
-// kernel does not carry the per-lane cursor's registers.
+// scan). The two-launch tuning launches the last two separately, so the
+// uniform kernel does not carry the per-lane scan's registers.
 enum GatherMode { kAllWaves = 0, kUniformWaves = 1, kOtherWaves = 2 };
 
 // The wave's active lanes all sit in dense half cells of one (x, y) column,
@@ -609,7 +532,7 @@ struct DrainWidth {
 
 // S: list slots drained per round (1, 2 or 4: more keeps more neighbour
 // loads in flight, fewer needs fewer registers) in the uniform scan; the
-// per-lane scan, whose cursor stays live across its drains, drains at most
+// per-lane scan, whose row state stays live across its drains, drains at most
 // two.
 // zt: the z half-widths of the uniform scan's trimmed rows. With LOC the
 // uniform scan filters with the column-local coordinates of cl instead of
@@ -791,44 +714,95 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
             if (wave_any(lp == lpL)) drain(DrainWidth<S>{});
         }
     } else if (MODE != kUniformWaves) {
-        Cursor cur;
-        cur.init(act, ci, xi, yi, zi, d, inv, start);
-        // up to R consecutive candidates of the lane's current run per round:
-        // R independent loads in flight instead of one dependent load per
-        // candidate
-        while (wave_any(act && cur.j < cur.jend)) {
-            if (act && cur.j < cur.jend) {
-                const uint32_t j = cur.j, last = cur.jend - 1;
-                // R consecutive entries from one base offset: j + R - 1 <=
-                // last + 7 stays inside the array (kScanPad spare entries),
-                // and an entry past the run gets a false flag (an append
-                // without PAD writes it unlisted, and the drain selects its
-                // term away)
-                const char* pb = reinterpret_cast<const char*>(posm) + (size_t)(j << 4);
-                float4 pq[R];
-#pragma unroll
-                for (int u = 0; u < R; ++u) pq[u] = *reinterpret_cast<const float4*>(pb + 16 * u);
-                // the distance filters without branches; the reference key
-                // adjacency only where the lane walks the overflow segment
-                bool t[R];
-#pragma unroll
-                for (int u = 0; u < R; ++u) t[u] = (j + (uint32_t)u <= last) & test(j + u, pq[u]);
-                // The count is at most L - R here, so the writes stay inside
-                // the list. Each side of the wave-wide overflow test appends
-                // its own flags: flags that crossed the merge would be carried
-                // as bytes in vector registers and unpacked again.
-                if (wave_any(cur.in_overflow())) {
-                    const bool inov = cur.in_overflow();
-#pragma unroll
-                    for (int u = 0; u < R; ++u) push(j + u, t[u] & (!inov | ov_ok(pq[u])));
-                } else {
-#pragma unroll
-                    for (int u = 0; u < R; ++u) push(j + u, t[u]);
+        // The per-lane scan. Each lane's window is a rectangle of table columns
+        // [x0, x1] x [y0, y1] with z cells z0..z1 (empty when z0 > z1), as
+        // 32-bit table-relative values with wrapping arithmetic (a row left of
+        // the table wraps to a huge x and fails x <= x1): for a dense particle
+        // its 5 x 5 half-cell (3 x 3 cell) window, for an out-of-domain one
+        // the in-domain cells of its 27 reference cells (6 x 6 half cells).
+        // The lanes walk the rows together: a wave-uniform loop over span x
+        // span rows (x-major), span the widest window of the wave, each lane
+        // loading the bounds of its own z-run of the row (a row outside its
+        // rectangle is empty); then the overflow segment. A lane's rows come
+        // in its own x-major order either way. The bounds of the next row are
+        // loaded while the current row is tested.
+        uint32_t x0 = 1u, x1 = 0u, y0 = 1u, y1 = 0u, z0 = 1u, z1 = 0u, step = 0u;
+        if (act) {
+            const uint32_t r = (uint32_t)d.reach;
+            if (ci < d.ncells) {
+                const CellCoord cc = decode(ci, d);
+                const uint32_t cx = (uint32_t)cc.cx, cy = (uint32_t)cc.cy, cz = (uint32_t)cc.cz;
+                x0 = cx - r;
+                y0 = cy - r;
+                x1 = min(cx + r, (uint32_t)d.dims[0] - 1u);
+                y1 = min(cy + r, (uint32_t)d.dims[1] - 1u);
+                z0 = cz > r ? cz - r : 0u;
+                z1 = min(cz + r, (uint32_t)d.dims[2] - 1u);
+                step = 2u * r + 1u;
+            } else {
+                long long lx, hx, ly, hy, lz, hz;
+                ref_block(key1(xi, inv), d, 0, lx, hx);
+                ref_block(key1(yi, inv), d, 1, ly, hy);
+                ref_block(key1(zi, inv), d, 2, lz, hz);
+                if (lx <= hx && ly <= hy && lz <= hz) {
+                    x0 = (uint32_t)lx;
+                    x1 = (uint32_t)hx;
+                    y0 = (uint32_t)ly;
+                    y1 = (uint32_t)hy;
+                    z0 = (uint32_t)lz;
+                    z1 = (uint32_t)hz;
                 }
-                cur.j = min(j + (uint32_t)R, cur.jend);
-                if (cur.j >= cur.jend) cur.next_run(d, start);
+                step = 3u * r;
             }
-            if (wave_any(lp > lpR)) drain(DrainWidth<(S < 2 ? S : 2)>{});
+        }
+        const bool zok = act && z0 <= z1;
+        const uint32_t D1 = (uint32_t)d.dims[1], D2 = (uint32_t)d.dims[2];
+        const uint32_t span = uni(wave_max_u32(step)), nrows = span * span;
+        auto bounds = [&](uint32_t k, uint32_t& ja, uint32_t& jb) {
+            const uint32_t ra = k / span, rb = k - ra * span;  // wave-uniform
+            const uint32_t x = x0 + ra, y = y0 + rb;
+            const bool ok = zok && x <= x1 && y <= y1;
+            const uint32_t base = ok ? (x * D1 + y) * D2 : 0u;
+            ja = start[base + (ok ? z0 : 0u)];
+            jb = start[base + (ok ? z1 + 1u : 0u)];
+        };
+        // R consecutive entries of the lane's run from one base offset (j + R
+        // - 1 <= last + 7 stays inside the array's kScanPad spare entries; an
+        // entry past the run gets a false flag); OV: the overflow segment's
+        // reference key adjacency. The count is at most L - R before a round.
+        auto round = [&](uint32_t& j, uint32_t jend, auto ov) {
+            const uint32_t last = jend - 1u;
+            const char* pb = reinterpret_cast<const char*>(posm) + (size_t)(j << 4);
+            float4 pq[R];
+#pragma unroll
+            for (int u = 0; u < R; ++u) pq[u] = *reinterpret_cast<const float4*>(pb + 16 * u);
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                bool t = (j + (uint32_t)u <= last) & test(j + u, pq[u]);
+                if (decltype(ov)::value) t = t & ov_ok(pq[u]);
+                push(j + u, t);
+            }
+            j = min(j + (uint32_t)R, jend);
+        };
+        uint32_t pa = 0u, pb = 0u;
+        if (nrows > 0u) bounds(0u, pa, pb);
+        for (uint32_t k = 0; k < nrows; ++k) {
+            uint32_t j = pa;
+            const uint32_t jend = pb;
+            if (k + 1u < nrows) bounds(k + 1u, pa, pb);
+            while (wave_any(j < jend)) {
+                if (j < jend) round(j, jend, std::false_type{});
+                if (wave_any(lp > lpR)) drain(DrainWidth<(S < 2 ? S : 2)>{});
+            }
+        }
+        {
+            const uint32_t o0 = uni(start[d.ncells]), o1 = uni(start[d.ncells + 1]);
+            uint32_t j = act ? o0 : 0u;
+            const uint32_t jend = act ? o1 : 0u;
+            while (wave_any(j < jend)) {
+                if (j < jend) round(j, jend, std::true_type{});
+                if (wave_any(lp > lpR)) drain(DrainWidth<(S < 2 ? S : 2)>{});
+            }
         }
     }
     if (uniform) drain(DrainWidth<S>{});
