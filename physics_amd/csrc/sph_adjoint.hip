@@ -181,7 +181,7 @@ __device__ __forceinline__ void adj_force_body(
         },
         // a slot at or past the lane's count names the lane itself, which the
         // validity test rejects, so the listed flag is not needed
-        [&](uint32_t o, const AdjNbr& nb, bool) {
+        [&](uint32_t o, const AdjNbr& nb, bool, auto) {
             const float Dx = nb.p.x - pa.x, Dy = nb.p.y - pa.y, Dz = nb.p.z - pa.z;
             float s = Dx * Dx + Dy * Dy + Dz * Dz;
             const bool valid = force_valid(P, true, (uint32_t)a << 4, o, pa.x, pa.y, pa.z, nb.p, s);
@@ -254,7 +254,7 @@ __device__ __forceinline__ void adj_force_body(
                 A.C3 += w * nb.ms.y * adv * ire;
             }
         },
-        [&]() {});
+        [&](auto) {}, [] { return false; });
     double kb = 0.0;
     if (act) {
         const float4 xl = xb_loc[a], vl = vb_loc[a];
@@ -345,7 +345,7 @@ __global__ __launch_bounds__(256) void k_adj_density(SceneTable T, AdjSlot S,
         act, ca, pa.x, pa.y, pa.z, act ? (uint32_t)a : 0u, d, P.inv_cell, zwin(P, d), ColLocal{}, start, S.posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return density_maybe(P, pa.x, pa.y, pa.z, pj); },
         [&](uint32_t o) { return DN{at(S.posm, o), at(rhob, o)}; },
-        [&](uint32_t o, const DN& nb, bool listed) {
+        [&](uint32_t o, const DN& nb, bool listed, auto) {
             const float Dx = nb.p.x - pa.x, Dy = nb.p.y - pa.y, Dz = nb.p.z - pa.z;
             const float s = Dx * Dx + Dy * Dy + Dz * Dz;
             const bool valid = density_valid(P, listed, pa.x, pa.y, pa.z, nb.p, s);
@@ -359,7 +359,7 @@ __global__ __launch_bounds__(256) void k_adj_density(SceneTable T, AdjSlot S,
             B2[1] += c2 * Dy;
             B2[2] += c2 * Dz;
         },
-        [&]() {});
+        [&](auto) {}, [] { return false; });
     if (!act) return;
     // rho_a = sum_b m_b W(r_ab): x_bar_a -= W'/r (rho_bar_a m_b + rho_bar_b m_a) D
     const float ra = rhob[a], ma = pa.w;

@@ -356,10 +356,12 @@ __device__ __forceinline__ T at(const T* __restrict__ base, uint32_t o) {
 }
 
 // test(j, pj) -> bool, a conservative FP32 filter (every true neighbour
-// passes; the exact decision is proc's); load(o) -> T and proc(o, T, listed)
+// passes; the exact decision is proc's); load(o) -> T and proc(o, T, listed, scan)
 // take the byte offset o = 16 j of candidate j (proc accumulates; a slot that
 // is not listed, or that proc's exact test rejects, must contribute exactly
-// nothing); flush() after every drain. self_j is a valid index: every list
+// nothing); flush(scan) after every drain; scan is std::true_type in the
+// uniform scan's drains and std::false_type in the per-lane scan's, so a proc
+// can keep state that only one of them uses. self_j is a valid index: every list
 // slot starts as its offset, and every slot written later holds a candidate
 // or an entry at most seven past a run's end (inside the array's kScanPad
 // spare entries), so drain slots beyond a lane's count read in-bounds records. With
@@ -369,7 +371,9 @@ __device__ __forceinline__ T at(const T* __restrict__ base, uint32_t o) {
 // candidates additionally need reference key adjacency. Appends are
 // branch-free: every candidate writes the list head (with PAD a failing one
 // writes self_j's offset) and the head advances only when the filter passes;
-// with PAD each drained slot is reset to self_j.
+// with PAD each drained slot is reset to self_j. again() -> bool, called once
+// a uniform scan has drained its last slot (wave-uniform result): true scans
+// the shared window again from the start (the caller resets its sums).
 // Which waves a gather launch serves: every wave, only waves whose active
 // lanes all sit in one dense cell (uniform scan), or only the others (per-lane
 // scan). The two-launch tuning launches the last two separately, so the
@@ -538,14 +542,14 @@ struct DrainWidth {
 // uniform scan filters with the column-local coordinates of cl instead of
 // test (which still filters the overflow segment and the per-lane scan).
 template <int L, int MODE, int S, bool LOC, bool PAD, int R, int BS = 256, class Test, class Load,
-          class Proc, class Flush>
+          class Proc, class Flush, class Again>
 __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi, float zi,
                                        uint32_t self_j, const Domain& d, double inv,
                                        const ZWin zt, const ColLocal& cl,
                                        const uint32_t* __restrict__ start,
                                        const float4* __restrict__ posm, uint32_t (*lst)[BS],
                                        int tid, Test&& test, Load&& load, Proc&& proc,
-                                       Flush&& flush) {
+                                       Flush&& flush, Again&& again) {
     static_assert(S == 1 || S == 2 || S == 4, "drain width");
     static_assert(L % S == 0, "whole drain rounds per list");
     static_assert(R >= 1 && R <= 8 && R <= L, "per-lane round width");
@@ -561,13 +565,13 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
     auto count = [&]() { return (int)((lp - lp0) >> kRowShift); };
     const uint32_t self_o = self_j << 4;
     for (int k = 0; k < L; ++k) lst[k][tid] = self_o;
-    auto drain = [&](auto width) {
+    auto drain = [&](auto width, auto scan) {
         constexpr int W = decltype(width)::value;
         const int cnt = count();
         // do-while: the loop test sits at the bottom, so the accumulators
         // stay in place across iterations
         if (!wave_any(0 < cnt)) {
-            flush();
+            flush(scan);
             return;
         }
         int k = 0;
@@ -586,10 +590,10 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 auto t1 = load(o1);
                 auto t2 = load(o2);
                 auto t3 = load(o3);
-                proc(o0, t0, v0);
-                proc(o1, t1, v1);
-                proc(o2, t2, v2);
-                proc(o3, t3, v3);
+                proc(o0, t0, v0, scan);
+                proc(o1, t1, v1, scan);
+                proc(o2, t2, v2, scan);
+                proc(o3, t3, v3, scan);
             } else if (W == 2) {
                 const bool v0 = k < cnt, v1 = k + 1 < cnt;
                 const uint32_t o0 = lst[k][tid], o1 = lst[k + 1][tid];
@@ -599,19 +603,19 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 }
                 auto t0 = load(o0);
                 auto t1 = load(o1);
-                proc(o0, t0, v0);
-                proc(o1, t1, v1);
+                proc(o0, t0, v0, scan);
+                proc(o1, t1, v1, scan);
             } else {
                 const bool v0 = k < cnt;
                 const uint32_t o0 = lst[k][tid];
                 if (PAD) lst[k][tid] = self_o;
                 auto t0 = load(o0);
-                proc(o0, t0, v0);
+                proc(o0, t0, v0, scan);
             }
             k += W;
         } while (wave_any(k < cnt));
         lp = lp0;
-        flush();
+        flush(scan);
     };
     // The slot at the head is written whatever ok says, and the head
     // advances only when ok (no branch). Without PAD it gets j (which must
@@ -646,73 +650,76 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
         // a row off to one side fills the lists of the lanes on that side
         // first, and its mirror row right after evens the counts out
         const int reach = (int)uni((uint32_t)d.reach), span = 2 * reach + 1, nrows = span * span;
-        for (int kr = 0; kr < nrows; ++kr) {
-            const int row = (kr & 1) ? nrows - 1 - (kr >> 1) : (kr >> 1);
-            const int rx = row / span;
-            const int ex = rx - reach, ey = row - rx * span - reach;
-            uint32_t a, b;
-            col_run(ex, ey, cc, cz1, d, start, a, b);
-            a = uni(a);
-            b = uni(b);
-            if (zt.on && a < b) {
-                const int cls = (ex == reach || ex == -reach) + (ey == reach || ey == -reach);
-                const double w = cls == 0 ? zw0 : (cls == 1 ? zw1 : zw2);
-                trim_run<LOC>(LOC ? cl.posl : posm, f32_down(zmn - w), f32_up(zmx + w), a, b);
+        do {
+            for (int kr = 0; kr < nrows; ++kr) {
+                const int row = (kr & 1) ? nrows - 1 - (kr >> 1) : (kr >> 1);
+                const int rx = row / span;
+                const int ex = rx - reach, ey = row - rx * span - reach;
+                uint32_t a, b;
+                col_run(ex, ey, cc, cz1, d, start, a, b);
                 a = uni(a);
                 b = uni(b);
+                if (zt.on && a < b) {
+                    const int cls = (ex == reach || ex == -reach) + (ey == reach || ey == -reach);
+                    const double w = cls == 0 ? zw0 : (cls == 1 ? zw1 : zw2);
+                    trim_run<LOC>(LOC ? cl.posl : posm, f32_down(zmn - w), f32_up(zmx + w), a, b);
+                    a = uni(a);
+                    b = uni(b);
+                }
+                // the lane's position relative to the row's column corner
+                const float xr = cl.li.x - (float)ex * cl.hh, yr = cl.li.y - (float)ey * cl.hh;
+                const float mx = -2.f * xr, my = -2.f * yr;
+                const float thr = cl.s_hq - fmaf(yr, yr, xr * xr);
+                auto loc_maybe = [&](const float4& q) {
+                    const float dz = q.w - zi;
+                    const float t = fmaf(q.y, my, fmaf(q.x, mx, fmaf(dz, dz, q.z)));
+                    return !(t >= thr);  // a NaN bound passes everything
+                };
+                const float4* __restrict__ src = LOC ? cl.posl : posm;
+                auto maybe = [&](uint32_t j, const float4& q) { return LOC ? loc_maybe(q) : test(j, q); };
+                // Both ends of the z-sorted run at once: four candidates from the
+                // front and four from the back per round. Lanes sit at different z,
+                // so a front-to-back walk would fill the low lanes' lists first and
+                // the high lanes' last; taking both ends keeps the counts closer and
+                // the drains' lanes busier. Each round is two 64-byte wave-uniform
+                // loads; cnt <= L - 8 before it, so its eight writes stay inside
+                // the list (non-short-circuit flags: no branches).
+                uint32_t lo = a, hi = b;
+                while (hi - lo >= 8u) {
+                    const Quad4 qa = *reinterpret_cast<const Quad4*>(src + lo);
+                    const Quad4 qb = *reinterpret_cast<const Quad4*>(src + hi - 4u);
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) push(lo + u, maybe(lo + u, qa.p[u]) & act);
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) push(hi - 4u + u, maybe(hi - 4u + u, qb.p[u]) & act);
+                    lo += 4u;
+                    hi -= 4u;
+                    if (wave_any(lp > lp8)) drain(DrainWidth<S>{}, std::true_type{});
+                }
+                if (lo < hi) {
+                    // the last 1..7 candidates: one round from lo (the arrays have
+                    // kScanPad spare entries, so every index of the round is in
+                    // bounds); a candidate past the run gets a false flag
+                    const uint32_t last = hi - 1u;
+                    const Quad4 qa = *reinterpret_cast<const Quad4*>(src + lo);
+                    const Quad4 qb = *reinterpret_cast<const Quad4*>(src + lo + 4u);
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u)
+                        push(lo + u, maybe(lo + u, qa.p[u]) & (act & (lo + u <= last)));
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u)
+                        push(lo + 4u + u, maybe(lo + 4u + u, qb.p[u]) & (act & (lo + 4u + u <= last)));
+                    if (wave_any(lp > lp8)) drain(DrainWidth<S>{}, std::true_type{});
+                }
             }
-            // the lane's position relative to the row's column corner
-            const float xr = cl.li.x - (float)ex * cl.hh, yr = cl.li.y - (float)ey * cl.hh;
-            const float mx = -2.f * xr, my = -2.f * yr;
-            const float thr = cl.s_hq - fmaf(yr, yr, xr * xr);
-            auto loc_maybe = [&](const float4& q) {
-                const float dz = q.w - zi;
-                const float t = fmaf(q.y, my, fmaf(q.x, mx, fmaf(dz, dz, q.z)));
-                return !(t >= thr);  // a NaN bound passes everything
-            };
-            const float4* __restrict__ src = LOC ? cl.posl : posm;
-            auto maybe = [&](uint32_t j, const float4& q) { return LOC ? loc_maybe(q) : test(j, q); };
-            // Both ends of the z-sorted run at once: four candidates from the
-            // front and four from the back per round. Lanes sit at different z,
-            // so a front-to-back walk would fill the low lanes' lists first and
-            // the high lanes' last; taking both ends keeps the counts closer and
-            // the drains' lanes busier. Each round is two 64-byte wave-uniform
-            // loads; cnt <= L - 8 before it, so its eight writes stay inside
-            // the list (non-short-circuit flags: no branches).
-            uint32_t lo = a, hi = b;
-            while (hi - lo >= 8u) {
-                const Quad4 qa = *reinterpret_cast<const Quad4*>(src + lo);
-                const Quad4 qb = *reinterpret_cast<const Quad4*>(src + hi - 4u);
-#pragma unroll
-                for (uint32_t u = 0; u < 4; ++u) push(lo + u, maybe(lo + u, qa.p[u]) & act);
-#pragma unroll
-                for (uint32_t u = 0; u < 4; ++u) push(hi - 4u + u, maybe(hi - 4u + u, qb.p[u]) & act);
-                lo += 4u;
-                hi -= 4u;
-                if (wave_any(lp > lp8)) drain(DrainWidth<S>{});
+            const uint32_t o0 = uni(start[d.ncells]), o1 = uni(start[d.ncells + 1]);
+            for (uint32_t j = o0; j < o1; ++j) {
+                const float4 pj = posm[j];
+                push(j, act && ov_ok(pj) && test(j, pj));
+                if (wave_any(lp == lpL)) drain(DrainWidth<S>{}, std::true_type{});
             }
-            if (lo < hi) {
-                // the last 1..7 candidates: one round from lo (the arrays have
-                // kScanPad spare entries, so every index of the round is in
-                // bounds); a candidate past the run gets a false flag
-                const uint32_t last = hi - 1u;
-                const Quad4 qa = *reinterpret_cast<const Quad4*>(src + lo);
-                const Quad4 qb = *reinterpret_cast<const Quad4*>(src + lo + 4u);
-#pragma unroll
-                for (uint32_t u = 0; u < 4; ++u)
-                    push(lo + u, maybe(lo + u, qa.p[u]) & (act & (lo + u <= last)));
-#pragma unroll
-                for (uint32_t u = 0; u < 4; ++u)
-                    push(lo + 4u + u, maybe(lo + 4u + u, qb.p[u]) & (act & (lo + 4u + u <= last)));
-                if (wave_any(lp > lp8)) drain(DrainWidth<S>{});
-            }
-        }
-        const uint32_t o0 = uni(start[d.ncells]), o1 = uni(start[d.ncells + 1]);
-        for (uint32_t j = o0; j < o1; ++j) {
-            const float4 pj = posm[j];
-            push(j, act && ov_ok(pj) && test(j, pj));
-            if (wave_any(lp == lpL)) drain(DrainWidth<S>{});
-        }
+            drain(DrainWidth<S>{}, std::true_type{});
+        } while (again());
     } else if (MODE != kUniformWaves) {
         // The per-lane scan. Each lane's window is a rectangle of table columns
         // [x0, x1] x [y0, y1] with z cells z0..z1 (empty when z0 > z1), as
@@ -792,7 +799,7 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
             if (k + 1u < nrows) bounds(k + 1u, pa, pb);
             while (wave_any(j < jend)) {
                 if (j < jend) round(j, jend, std::false_type{});
-                if (wave_any(lp > lpR)) drain(DrainWidth<(S < 2 ? S : 2)>{});
+                if (wave_any(lp > lpR)) drain(DrainWidth<(S < 2 ? S : 2)>{}, std::false_type{});
             }
         }
         {
@@ -801,12 +808,11 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
             const uint32_t jend = act ? o1 : 0u;
             while (wave_any(j < jend)) {
                 if (j < jend) round(j, jend, std::true_type{});
-                if (wave_any(lp > lpR)) drain(DrainWidth<(S < 2 ? S : 2)>{});
+                if (wave_any(lp > lpR)) drain(DrainWidth<(S < 2 ? S : 2)>{}, std::false_type{});
             }
         }
+        drain(DrainWidth<(S < 2 ? S : 2)>{}, std::false_type{});
     }
-    if (uniform) drain(DrainWidth<S>{});
-    else drain(DrainWidth<(S < 2 ? S : 2)>{});
 }
 
 // Candidate filters of the compaction. Both are conservative: |s32 - s64| <=

@@ -52,10 +52,27 @@ struct SetAcc {
 // ---------------------------------------------------------------------------
 // density / EOS (world.rs:204-246)
 //
-// The pair geometry and the sum run in f64: p = k (rho - rho0) cancels near
-// the rest density and would amplify an FP32 density error of ~2e-7 by
+// Near the rest density the pair geometry and the sum run in f64: p = k (rho -
+// rho0) cancels there and would amplify an FP32 density error of ~2e-6 by
 // rho / (rho - rho0). MI355X executes f64 FMA at the unpacked FP32 rate, and the
 // compaction limits it to true neighbours.
+//
+// Compressed waves take FP32 pair terms. A column wave (64 particles within
+// three half cells of one column: about 20x the rest density or more) first
+// sums T = sum m_j q^3, q = h^2 - s, with s from FP32 differences, in FP32
+// partials per drain folded into an f64 total; the neighbour sets are the same
+// (FP32 test outside the band, the exact f64 predicate inside it). Error, with
+// u = 2^-24: |s32 - s| <= 5u s, so |q32 - q| <= 7u h^2 (h^2 rounded, s <= h^2),
+// a term's error is at most 21u h^2 m q^2 + 3u m q^3, and a partial of at most
+// 16 positive terms adds 15u. With E = sum m q^2 summed alongside (in FP32: an
+// error below 2^-11 relative for up to 2^12 terms), the relative error of T is
+// at most (21 R + 18) u, R = h^2 E / T (1.5 for neighbours spread evenly over
+// the ball). A lane accepts its FP32 density when R <= 1.99 (60u),
+// rho - rho0 >= 0.75 rho (p = k (rho - rho0) then within 80u = 5e-6 relative)
+// and 2^-100 <= T <= 2^100 (no FP32 overflow, underflow negligible); when any
+// lane of the wave does not, the wave scans again and sums in f64.
+constexpr double kLowpShare = 0.75;
+constexpr double kLowpRatio = 1.99;
 
 template <int L, int MODE, bool STATS, int BS>
 __global__ __launch_bounds__(BS) void k_density(SceneTable T, const float4* __restrict__ posm,
@@ -83,8 +100,12 @@ __global__ __launch_bounds__(BS) void k_density(SceneTable T, const float4* __re
     }
     const double xi = pi.x, yi = pi.y, zi = pi.z;
     double sum = 0.0;
+    float part = 0.f, esum = 0.f;
     SetAcc sa;
     const uint32_t soff = STATS ? T.info[TL.s].off : 0u;
+    // FP32 terms first for a column wave (wave-uniform; the gather takes its
+    // uniform scan for exactly these waves)
+    bool lowp = MODE == kUniformWaves || (MODE == kAllWaves && wave_uniform(act, ci, d));
     // four neighbour loads in flight per drain round (force keeps two: four
     // would cost it a wave per SIMD); the per-lane scan takes six candidates
     // per round: a particle's run of a row holds about five at rest density
@@ -92,20 +113,56 @@ __global__ __launch_bounds__(BS) void k_density(SceneTable T, const float4* __re
     // at rest: 0.168 -> 0.136 ms per density pass, for five waves per SIMD
     // instead of six)
     gather<L, MODE, (MODE == kUniformWaves ? 1 : 4), LOC, false, 6, BS>(
-        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), cl, start, posm, lst, tid,
-        [&](uint32_t j, const float4& pj) { return density_maybe(P, pi.x, pi.y, pi.z, pj); },
+        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), cl, start, posm, lst,
+        tid, [&](uint32_t j, const float4& pj) { return density_maybe(P, pi.x, pi.y, pi.z, pj); },
         [&](uint32_t o) { return at(posm, o); },
-        [&](uint32_t j, const float4& pj, bool listed) {
-            // the reference's own f64 distance decides r < h exactly (NaN fails)
-            const double dx = xi - (double)pj.x, dy = yi - (double)pj.y, dz = zi - (double)pj.z;
-            const double s = sq_exact(dx, dy, dz);
-            const bool valid = listed && s < P.s_thresh;
-            const double q = P.h2_64 - s;
-            const double t = (double)pj.w * (q * q * q);
-            sum += valid ? t : 0.0;
+        [&](uint32_t j, const float4& pj, bool listed, auto scan) {
+            bool valid;
+            if (decltype(scan)::value && lowp) {
+                const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                const float s = dx * dx + dy * dy + dz * dz;
+                valid = listed && s < P.s_lo;
+                // the f64 band (and NaN): the reference's own distance, inline
+                // (a call here would cost the kernel registers); rarely entered
+                if (listed && !(s < P.s_lo)) {
+                    const double ex = xi - (double)pj.x, ey = yi - (double)pj.y, ez = zi - (double)pj.z;
+                    valid = sq_exact(ex, ey, ez) < P.s_thresh;
+                }
+                // both factors selected: a slot past the lane's count may name
+                // a particle with a NaN position or mass
+                const float q = valid ? P.h2 - s : 0.f;
+                const float mq2 = valid ? pj.w * (q * q) : 0.f;
+                part = fmaf(mq2, q, part);
+                esum += mq2;
+            } else {
+                // the reference's own f64 distance decides r < h exactly (NaN fails)
+                const double dx = xi - (double)pj.x, dy = yi - (double)pj.y, dz = zi - (double)pj.z;
+                const double s = sq_exact(dx, dy, dz);
+                valid = listed && s < P.s_thresh;
+                const double q = P.h2_64 - s;
+                const double t = (double)pj.w * (q * q * q);
+                sum += valid ? t : 0.0;
+            }
             if (STATS) sa.add(valid, at(gs.id, j) - soff);
         },
-        [&]() {});
+        [&](auto scan) {
+            if (decltype(scan)::value && lowp) {
+                sum += (double)part;
+                part = 0.f;
+            }
+        },
+        // after an FP32 pass: keep it, or scan again in f64
+        [&]() {
+            if (!lowp) return false;
+            const double rho = P.c6_64 * sum;
+            const bool keep = !act || (sum >= 0x1p-100 && sum <= 0x1p100 &&
+                                       P.h2_64 * (double)esum <= kLowpRatio * sum && rho - P.rest64 >= kLowpShare * rho);
+            if (!wave_any(!keep)) return false;
+            lowp = false;
+            sum = 0.0;
+            sa = SetAcc{};
+            return true;
+        });
     if (!act) return;
     if (STATS) {
         sa.store(gs, gs.id[i]);
@@ -241,11 +298,11 @@ __global__ __launch_bounds__(BS) void k_force(
         [&](uint32_t o) { return NbrData{at(posm, o), at(velm, o), at(mp, o)}; },
         // every slot at or past the lane's count names the lane itself, which
         // no validity test accepts, so the listed flag is not needed
-        [&](uint32_t o, const NbrData& nb, bool) {
+        [&](uint32_t o, const NbrData& nb, bool, auto) {
             const bool ok = pair_force<SHEAR>(A, P, (uint32_t)i << 4, o, pi, vi, Pi, nb, true);
             if (STATS) sa.add(ok, at(gs.id, o) - soff);
         },
-        [&]() { A.merge(); });
+        [&](auto) { A.merge(); }, [] { return false; });
     if (!act) return;
     if (STATS) {
         sa.store(gs, gs.id[i]);
