@@ -32,12 +32,13 @@ def normwise(a, b):
 
 
 def params_pair(h=scenes.H32, bounds=((-5, -5, -5), (5, 5, 5)), gravity=-9.81, visc_k=(3.5,),
-                visc_n=(1.0,), stiffness=250.0):
+                visc_n=(1.0,), stiffness=250.0, rest_density=1000.0):
     gp = ffi.PhySphParams.defaults()
     op = O.Params.defaults()
     for p in (gp, op):
         p.h = h
         p.stiffness = stiffness
+        p.rest_density = rest_density
         p.gravity[1] = gravity
         p.set_visc(visc_k, visc_n)
         for a in range(3):

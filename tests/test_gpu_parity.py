@@ -82,6 +82,33 @@ def test_one_step_parity_moving_compressed_multimaterial():
         assert v <= P.TOL_STEP, (k, v)
 
 
+@pytest.mark.parametrize("rest", [1000.0, 6000.0, 1e6])
+def test_compressed_column_waves_parity_and_sets(rest):
+    # 27 particles per half cell (spacing h/6), so whole waves sit within three
+    # half cells of one column and the density pass tries FP32 terms first:
+    # rho ~ 27 rho0 keeps them at rho0 = 1000; at 6000 lanes near the block's
+    # faces fall below rho - rho0 >= 0.75 rho and their waves sum again in f64;
+    # at 1e6 every wave does (p = 0), so rho carries only the f64 sum's final
+    # rounding to FP32
+    pos = _block(12, 5, jit=0.05, s=float(np.float32(scenes.H32 / 6)), origin=(0.31, 0.47, 0.23))
+    n = len(pos)
+    vel = (0.2 * (scenes.rng_f64(6, 3 * n).reshape(n, 3) - 0.5)).astype(np.float32)
+    g, o = P.make_pair(pos, vel, mass=0.99, rest_density=rest)
+    for which, ofn in (("density", o.neighbor_stats), ("force", o.force_neighbor_stats)):
+        c, s, x = g.gather_stats(which)
+        oc, os_, ox = ofn()
+        assert np.array_equal(c, oc) and np.array_equal(s, os_) and np.array_equal(x, ox), which
+    x0 = P.step_both(g, o)
+    err = P.compare_step(g, o, x0)
+    _print(f"compressed rest={rest:g}", err)
+    for k, v in err.items():
+        assert v <= P.TOL_STEP, (k, v)
+    if rest == 1e6:
+        rg, ro = g.get("rho"), o.get("rho")
+        rel = np.abs(rg - ro) / np.abs(ro)
+        assert rel.max() <= 2.0 ** -24 * (1 + 1e-6), rel.max()
+
+
 def test_per_step_parity_over_a_collapse():
     # the oracle is re-fed the GPU state every step: the per-step bar holds along
     # a trajectory that compresses the column against the floor
