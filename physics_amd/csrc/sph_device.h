@@ -369,8 +369,9 @@ __device__ __forceinline__ T at(const T* __restrict__ base, uint32_t o) {
 // into the neighbour's data needs finite data there (a stale slot may name a
 // NaN particle); a proc that selects its terms does not. Overflow-segment
 // candidates additionally need reference key adjacency. Appends are
-// branch-free: every candidate writes the list head (with PAD a failing one
-// writes self_j's offset) and the head advances only when the filter passes;
+// branch-free: every candidate writes the list head and the head advances
+// only when the filter passes (with PAD a drain first writes self_j's offset
+// at the head of a list that is not full);
 // with PAD each drained slot is reset to self_j. again() -> bool, called once
 // a uniform scan has drained its last slot (wave-uniform result): true scans
 // the shared window again from the start (the caller resets its sums).
@@ -527,6 +528,7 @@ __device__ __forceinline__ void col_run(int ex, int ey, const CellCoord& cc, lon
     b = start[base + z1 + 1];
 }
 
+
 // Slots per drain round as a type, so one drain can be instantiated at two
 // widths.
 template <int N>
@@ -565,8 +567,15 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
     auto count = [&]() { return (int)((lp - lp0) >> kRowShift); };
     const uint32_t self_o = self_j << 4;
     for (int k = 0; k < L; ++k) lst[k][tid] = self_o;
+    // list-full tests: at least L - 8 + 1, L - R + 1 or L entries
+    const uint32_t lp8 = lp0 + (L - 8) * kRowB;
+    const uint32_t lpR = lp0 + (L - R) * kRowB;
+    const uint32_t lpL = lp0 + L * kRowB;
     auto drain = [&](auto width, auto scan) {
         constexpr int W = decltype(width)::value;
+        // the head slot may hold the last failing candidate: with PAD it
+        // names the lane again (a full list has no head slot)
+        if (PAD && lp < lpL) *reinterpret_cast<uint32_t*>(lbase + lp) = self_o;
         const int cnt = count();
         // do-while: the loop test sits at the bottom, so the accumulators
         // stay in place across iterations
@@ -617,20 +626,16 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
         lp = lp0;
         flush(scan);
     };
-    // The slot at the head is written whatever ok says, and the head
-    // advances only when ok (no branch). Without PAD it gets j (which must
-    // then be a valid index: it stays in the list until overwritten, and
-    // drains read every slot). With PAD a failing candidate writes the lane's
-    // own offset instead, and the drain puts that offset back into every
-    // slot it reads, so a slot at or past the count always names the lane.
+    // The slot at the head gets j whatever ok says, and the head advances
+    // only when ok (no branch); j must be a valid index: it stays in the list
+    // until overwritten, and drains read every slot. Only the head slot can
+    // hold a failing candidate. With PAD a drain first writes the lane's own
+    // offset there and puts it back into every slot it reads, so a slot at or
+    // past the count always names the lane.
     auto push = [&](uint32_t j, bool ok) {
-        *reinterpret_cast<uint32_t*>(lbase + lp) = (PAD && !ok) ? self_o : j << 4;
+        *reinterpret_cast<uint32_t*>(lbase + lp) = j << 4;
         lp += ok ? kRowB : 0u;
     };
-    // list-full tests: at least L - 8 + 1, L - R + 1 or L entries
-    const uint32_t lp8 = lp0 + (L - 8) * kRowB;
-    const uint32_t lpR = lp0 + (L - R) * kRowB;
-    const uint32_t lpL = lp0 + L * kRowB;
     auto ov_ok = [&](const float4& pj) {
         return keys_adjacent(xi, yi, zi, pj.x, pj.y, pj.z, inv);
     };

@@ -180,20 +180,30 @@ __global__ __launch_bounds__(BS) void k_density(SceneTable T, const float4* __re
 // ---------------------------------------------------------------------------
 // forces + integrate + bounds + watchdog (world.rs:252-397, subsystem.rs:74-90)
 //
-// Compensated summation over blocks: the contributions of one drained batch
-// (<= 2 L neighbours of this lane) are summed with plain FMAs, then each batch
-// partial is added to the running total with Kahan's algorithm.
+// Summation over blocks: the contributions of one drained batch (at most L
+// neighbours of this lane) are summed with plain FMAs, then each batch partial
+// is added to the running total, with Kahan's algorithm for the pressure
+// force (its terms point every way and cancel in the interior) and plainly for
+// the viscosity, XSPH and shear sums (a 1M collapsed sheet: acc, xsph and v
+// within 1.4e-7, 3.4e-7 and 9e-8 of the oracle; full Kahan 1.8e-7, 3.4e-7,
+// 7.5e-8). Ten fewer registers than Kahan on all nine sums, and with the
+// occupancy hint of five waves per SIMD (a few values spill outside the
+// loops) the kernel runs five waves instead of four: 1M at rest 0.393 ->
+// 0.379 ms per step, collapsed 2.83 -> 2.78 ms.
 
 template <bool SHEAR>
 struct ForceAcc {
-    float b[10];            // batch partials: pressure xyz, viscosity xyz, XSPH xyz, shear
-    float s[10], c[10];     // compensated totals
+    float b[10];   // batch partials: pressure xyz, viscosity xyz, XSPH xyz, shear
+    float s[10];   // totals
+    float c[3];    // compensation of the pressure totals
     __device__ __forceinline__ void zero() {
-        for (int k = 0; k < 10; ++k) b[k] = s[k] = c[k] = 0.f;
+        for (int k = 0; k < 10; ++k) b[k] = s[k] = 0.f;
+        for (int k = 0; k < 3; ++k) c[k] = 0.f;
     }
     __device__ __forceinline__ void merge() {
         for (int k = 0; k < (SHEAR ? 10 : 9); ++k) {
-            kahan_add(s[k], c[k], b[k]);
+            if (k < 3) kahan_add(s[k], c[k], b[k]);
+            else s[k] += b[k];
             b[k] = 0.f;
         }
     }
@@ -256,7 +266,7 @@ __device__ __forceinline__ bool pair_force(ForceAcc<SHEAR>& A, const DevParams& 
 }
 
 template <int L, bool SHEAR, int MODE, bool STATS, int BS>
-__global__ __launch_bounds__(BS) void k_force(
+__global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_force(
     SceneTable T, float dt, const float4* __restrict__ posm, const float4* __restrict__ posl,
     const float4* __restrict__ velm,
     const float* __restrict__ rho, const float* __restrict__ prs, const float* __restrict__ mp,
