@@ -57,22 +57,103 @@ struct SetAcc {
 // rho / (rho - rho0). MI355X executes f64 FMA at the unpacked FP32 rate, and the
 // compaction limits it to true neighbours.
 //
-// Compressed waves take FP32 pair terms. A column wave (64 particles within
-// three half cells of one column: about 20x the rest density or more) first
-// sums T = sum m_j q^3, q = h^2 - s, with s from FP32 differences, in FP32
-// partials per drain folded into an f64 total; the neighbour sets are the same
-// (FP32 test outside the band, the exact f64 predicate inside it). Error, with
-// u = 2^-24: |s32 - s| <= 5u s, so |q32 - q| <= 7u h^2 (h^2 rounded, s <= h^2),
-// a term's error is at most 21u h^2 m q^2 + 3u m q^3, and a partial of at most
-// 16 positive terms adds 15u. With E = sum m q^2 summed alongside (in FP32: an
-// error below 2^-11 relative for up to 2^12 terms), the relative error of T is
-// at most (21 R + 18) u, R = h^2 E / T (1.5 for neighbours spread evenly over
-// the ball). A lane accepts its FP32 density when R <= 1.99 (60u),
-// rho - rho0 >= 0.75 rho (p = k (rho - rho0) then within 80u = 5e-6 relative)
-// and 2^-100 <= T <= 2^100 (no FP32 overflow, underflow negligible); when any
-// lane of the wave does not, the wave scans again and sums in f64.
+// Compressed waves take FP32 pair terms, candidate-major. A column wave (64
+// particles within three half cells of one column: about 20x the rest density
+// or more) walks its shared window (the uniform scan's rows, trimmed to the
+// lanes' z-range, and the overflow segment) four candidates at a time with
+// wave-uniform loads, and every lane evaluates every candidate: s from FP32
+// differences of the sorted positions, the FP32 test outside the band and the
+// exact f64 predicate inside it (the same sets as the list path), and sums
+// T = sum m_j q^3 (q = h^2 - s) in FP32 partials of 16 candidates folded into
+// an f64 total. No lists, no drains: with about half of the window's
+// candidates being neighbours, this costs fewer operations than filtering,
+// listing and draining them. Error, with u = 2^-24: |s32 - s| <= 5u s, so
+// |q32 - q| <= 7u h^2 (h^2 rounded, s <= h^2), a term's error is at most 21u
+// h^2 m q^2 + 3u m q^3, and a partial of at most 16 positive terms adds 15u.
+// With E = sum m q^2 summed alongside (in FP32: an error below 2^-11 relative
+// for up to 2^12 terms), the relative error of T is at most (21 R + 18) u, R =
+// h^2 E / T (1.5 for neighbours spread evenly over the ball). A lane accepts
+// its FP32 density when R <= 1.99 (60u), rho - rho0 >= 0.75 rho (p = k (rho -
+// rho0) then within 80u = 5e-6 relative) and 2^-100 <= T <= 2^100 (no FP32
+// overflow, underflow negligible); when any lane of the wave does not, the
+// wave sums again in f64 through the list path.
 constexpr double kLowpShare = 0.75;
 constexpr double kLowpRatio = 1.99;
+
+// The FP32 candidate-major density sums of a column wave (see above) into sum
+// (T, f64) and esum (E); with STATS the accepted sets into sa as well.
+template <bool STATS>
+__device__ __forceinline__ void density_column(const WaveCol& wc, bool act, const float4& pi,
+                                               const DevParams& P, const Domain& d, const ZWin& zt,
+                                               const uint32_t* __restrict__ start,
+                                               const float4* __restrict__ posm, const GatherStats& gs,
+                                               uint32_t soff, double& sum, float& esum, SetAcc& sa) {
+    float part = 0.f;
+    // in: the candidate exists (wave-uniform in the rows, the key adjacency
+    // of the overflow segment); SEL: its data may be non-finite (overflow)
+    auto term = [&](uint32_t j, const float4& q, bool in, auto sel) {
+        const float dx = q.x - pi.x, dy = q.y - pi.y, dz = q.z - pi.z;
+        const float s = dx * dx + dy * dy + dz * dz;
+        bool valid = s < P.s_lo;
+        // the f64 band: the reference's own distance, inline (a call here
+        // would cost the kernel registers); rarely entered; NaN fails both
+        if (!valid && s < P.s_hi) {
+            const double ex = (double)pi.x - (double)q.x, ey = (double)pi.y - (double)q.y,
+                         ez = (double)pi.z - (double)q.z;
+            valid = sq_exact(ex, ey, ez) < P.s_thresh;
+        }
+        valid = valid && in;
+        float qq = P.h2 - s;
+        if (decltype(sel)::value) qq = valid ? qq : 0.f;
+        const float mq2 = valid ? q.w * (qq * qq) : 0.f;
+        part = fmaf(mq2, qq, part);
+        esum += mq2;
+        if (STATS) sa.add(valid, gs.id[j] - soff);
+    };
+    const CellCoord cc = decode(wc.c0, d);
+    const long long cz1 = cc.cz + (long long)(wc.c1 - wc.c0);
+    const double zmn = (double)wave_min_f(act ? pi.z : INFINITY);
+    const double zmx = (double)wave_max_f(act ? pi.z : -INFINITY);
+    const double zw0 = zt.w0, zw1 = zt.w1, zw2 = zt.w2;
+    const int reach = (int)uni((uint32_t)d.reach);
+    for (int ex = -reach; ex <= reach; ++ex)
+        for (int ey = -reach; ey <= reach; ++ey) {
+            uint32_t a, b;
+            col_run(ex, ey, cc, cz1, d, start, a, b);
+            a = uni(a);
+            b = uni(b);
+            if (zt.on && a < b) {
+                const int cls = (ex == reach || ex == -reach) + (ey == reach || ey == -reach);
+                const double w = cls == 0 ? zw0 : (cls == 1 ? zw1 : zw2);
+                trim_run<false>(posm, f32_down(zmn - w), f32_up(zmx + w), a, b);
+                a = uni(a);
+                b = uni(b);
+            }
+            // four candidates per round from one 64-byte wave-uniform load (the
+            // array has kScanPad spare entries); a 16-candidate partial is
+            // folded into the f64 total
+            int round = 0;
+            for (uint32_t j = a; j < b; j += 4u) {
+                const Quad4 qm = *reinterpret_cast<const Quad4*>(posm + j);
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) term(j + u, qm.p[u], j + u < b, std::false_type{});
+                if (++round == 4) {
+                    sum += (double)part;
+                    part = 0.f;
+                    round = 0;
+                }
+            }
+            sum += (double)part;
+            part = 0.f;
+        }
+    const uint32_t o0 = uni(start[d.ncells]), o1 = uni(start[d.ncells + 1]);
+    for (uint32_t j = o0; j < o1; ++j) {
+        const float4 q = posm[j];
+        term(j, q, keys_adjacent(pi.x, pi.y, pi.z, q.x, q.y, q.z, P.inv_cell), std::true_type{});
+        sum += (double)part;
+        part = 0.f;
+    }
+}
 
 template <int L, int MODE, bool STATS, int BS>
 __global__ __launch_bounds__(BS) void k_density(SceneTable T, const float4* __restrict__ posm,
@@ -98,71 +179,49 @@ __global__ __launch_bounds__(BS) void k_density(SceneTable T, const float4* __re
         pi = posm[i];
         if (LOC) cl.li = posl[i];
     }
-    const double xi = pi.x, yi = pi.y, zi = pi.z;
     double sum = 0.0;
-    float part = 0.f, esum = 0.f;
     SetAcc sa;
     const uint32_t soff = STATS ? T.info[TL.s].off : 0u;
-    // FP32 terms first for a column wave (wave-uniform; the gather takes its
-    // uniform scan for exactly these waves)
-    bool lowp = MODE == kUniformWaves || (MODE == kAllWaves && wave_uniform(act, ci, d));
-    // four neighbour loads in flight per drain round (force keeps two: four
-    // would cost it a wave per SIMD); the per-lane scan takes six candidates
-    // per round: a particle's run of a row holds about five at rest density
-    // with half cells, so most runs take one round instead of two (1M column
-    // at rest: 0.168 -> 0.136 ms per density pass, for five waves per SIMD
-    // instead of six)
-    gather<L, MODE, (MODE == kUniformWaves ? 1 : 4), LOC, false, 6, BS>(
-        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), cl, start, posm, lst,
-        tid, [&](uint32_t j, const float4& pj) { return density_maybe(P, pi.x, pi.y, pi.z, pj); },
-        [&](uint32_t o) { return at(posm, o); },
-        [&](uint32_t j, const float4& pj, bool listed, auto scan) {
-            bool valid;
-            if (decltype(scan)::value && lowp) {
-                const float dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                const float s = dx * dx + dy * dy + dz * dz;
-                valid = listed && s < P.s_lo;
-                // the f64 band (and NaN): the reference's own distance, inline
-                // (a call here would cost the kernel registers); rarely entered
-                if (listed && !(s < P.s_lo)) {
-                    const double ex = xi - (double)pj.x, ey = yi - (double)pj.y, ez = zi - (double)pj.z;
-                    valid = sq_exact(ex, ey, ez) < P.s_thresh;
-                }
-                // both factors selected: a slot past the lane's count may name
-                // a particle with a NaN position or mass
-                const float q = valid ? P.h2 - s : 0.f;
-                const float mq2 = valid ? pj.w * (q * q) : 0.f;
-                part = fmaf(mq2, q, part);
-                esum += mq2;
-            } else {
-                // the reference's own f64 distance decides r < h exactly (NaN fails)
-                const double dx = xi - (double)pj.x, dy = yi - (double)pj.y, dz = zi - (double)pj.z;
-                const double s = sq_exact(dx, dy, dz);
-                valid = listed && s < P.s_thresh;
-                const double q = P.h2_64 - s;
-                const double t = (double)pj.w * (q * q * q);
-                sum += valid ? t : 0.0;
-            }
-            if (STATS) sa.add(valid, at(gs.id, j) - soff);
-        },
-        [&](auto scan) {
-            if (decltype(scan)::value && lowp) {
-                sum += (double)part;
-                part = 0.f;
-            }
-        },
-        // after an FP32 pass: keep it, or scan again in f64
-        [&]() {
-            if (!lowp) return false;
+    bool done = false;
+    if (MODE != kOtherWaves) {
+        const WaveCol wc = wave_column(act, ci, d);
+        if (wc.ok) {
+            float esum = 0.f;
+            density_column<STATS>(wc, act, pi, P, d, zwin(P, d), start, posm, gs, soff, sum, esum, sa);
             const double rho = P.c6_64 * sum;
             const bool keep = !act || (sum >= 0x1p-100 && sum <= 0x1p100 &&
                                        P.h2_64 * (double)esum <= kLowpRatio * sum && rho - P.rest64 >= kLowpShare * rho);
-            if (!wave_any(!keep)) return false;
-            lowp = false;
-            sum = 0.0;
-            sa = SetAcc{};
-            return true;
-        });
+            done = !wave_any(!keep);
+            if (!done) {
+                sum = 0.0;
+                sa = SetAcc{};
+            }
+        }
+    }
+    if (!done) {
+        const double xi = pi.x, yi = pi.y, zi = pi.z;
+        // four neighbour loads in flight per drain round (force keeps two: four
+        // would cost it a wave per SIMD); the per-lane scan takes six candidates
+        // per round: a particle's run of a row holds about five at rest density
+        // with half cells, so most runs take one round instead of two (1M column
+        // at rest: 0.168 -> 0.136 ms per density pass, for five waves per SIMD
+        // instead of six)
+        gather<L, MODE, (MODE == kUniformWaves ? 1 : 4), LOC, false, 6, BS>(
+            act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), cl, start, posm, lst,
+            tid, [&](uint32_t j, const float4& pj) { return density_maybe(P, pi.x, pi.y, pi.z, pj); },
+            [&](uint32_t o) { return at(posm, o); },
+            [&](uint32_t j, const float4& pj, bool listed, auto) {
+                // the reference's own f64 distance decides r < h exactly (NaN fails)
+                const double dx = xi - (double)pj.x, dy = yi - (double)pj.y, dz = zi - (double)pj.z;
+                const double s = sq_exact(dx, dy, dz);
+                const bool valid = listed && s < P.s_thresh;
+                const double q = P.h2_64 - s;
+                const double t = (double)pj.w * (q * q * q);
+                sum += valid ? t : 0.0;
+                if (STATS) sa.add(valid, at(gs.id, j) - soff);
+            },
+            [&](auto) {}, [] { return false; });
+    }
     if (!act) return;
     if (STATS) {
         sa.store(gs, gs.id[i]);
