@@ -11,6 +11,8 @@
 #include <stdint.h>
 #include <math.h>
 
+#include <type_traits>
+
 #include "sph_types.h"
 #include "sph_device.h"
 #include "sph_kernels.h"
@@ -361,28 +363,37 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
     const uint32_t soff = STATS ? T.info[TL.s].off : 0u;
     // per-lane rounds of six candidates: a row of five half cells at rest
     // spacing holds about five, so a row takes one round instead of two
+    // the slot as a byte offset of 16- and 4-byte records: the stores after
+    // the gather use it with the arrays' scalar bases (no 64-bit per-lane
+    // address stays live across the gather)
+    const uint32_t self_j = act ? (uint32_t)i : 0u;
     gather<L, MODE, (MODE == kUniformWaves ? 1 : 2), LOC, true, 6, BS>(
-        act, ci, pi.x, pi.y, pi.z, act ? (uint32_t)i : 0u, d, P.inv_cell, zwin(P, d), cl, start, posm, lst, tid,
+        act, ci, pi.x, pi.y, pi.z, self_j, d, P.inv_cell, zwin(P, d), cl, start, posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_maybe(P, pi.x, pi.y, pi.z, pj); },
         [&](uint32_t o) { return NbrData{at(posm, o), at(velm, o), at(mp, o)}; },
         // every slot at or past the lane's count names the lane itself, which
         // no validity test accepts, so the listed flag is not needed
         [&](uint32_t o, const NbrData& nb, bool, auto) {
-            const bool ok = pair_force<SHEAR>(A, P, (uint32_t)i << 4, o, pi, vi, Pi, nb, true);
+            const bool ok = pair_force<SHEAR>(A, P, self_j << 4, o, pi, vi, Pi, nb, true);
             if (STATS) sa.add(ok, at(gs.id, o) - soff);
         },
         [&](auto) { A.merge(); }, [] { return false; });
     if (!act) return;
+    const uint32_t o16 = self_j << 4;
     if (STATS) {
-        sa.store(gs, gs.id[i]);
+        sa.store(gs, at(gs.id, o16));
         return;
     }
+    auto put = [&](auto* base, auto v) {
+        using E = std::remove_reference_t<decltype(*base)>;
+        *reinterpret_cast<E*>(reinterpret_cast<char*>(base) + (o16 >> (sizeof(E) == 16 ? 0 : 2))) = v;
+    };
 
     // power-law effective viscosity (world.rs:319-327)
     int m = 0;
     if (matv) {
-        const uint32_t mt = matv[i];
-        mat_out[i] = mt;  // the new state keeps this step's cell order
+        const uint32_t mt = at(matv, o16);
+        put(mat_out, mt);  // the new state keeps this step's cell order
         if (P.n_mat > 1) m = mt < (uint32_t)P.n_mat ? (int)mt : 0;
     }
     // indexed through the table (a dynamic index into the register copy P
@@ -397,6 +408,12 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
         float sreg = sh > P.shear_min ? sh : P.shear_min;
         mu = kc * powf(sreg, Pm->visc_n[m] - 1.f);
     }
+    // rho_i again from memory: kept in a register across the gather it is
+    // the one value the five-wave occupancy hint would spill (an opaque copy
+    // of the offset keeps the compiler from reusing the first load)
+    uint32_t o_rho = o16;
+    asm volatile("" : "+v"(o_rho));
+    rho_i = at(rho, o_rho);
     float acc[3] = {0.f, 0.f, 0.f}, fvis[3] = {0.f, 0.f, 0.f};
     if (rho_i > 0.f) {
         float mcs = pi.w * P.cs, mcv = pi.w * P.cv * mu;
@@ -410,7 +427,7 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
     // integrate (world.rs:352-363) and enforce_bounds (366-397)
     float bacc[3] = {0.f, 0.f, 0.f};
     if (body) {
-        float4 b = body[i];
+        float4 b = at(body, o16);
         bacc[0] = b.x;
         bacc[1] = b.y;
         bacc[2] = b.z;
@@ -432,23 +449,23 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
             clamp |= 1u << a;
         }
     }
-    posm_out[i] = make_float4(x[0], x[1], x[2], pi.w);
-    vel_out[i] = make_float4(v[0], v[1], v[2], 0.f);
+    put(posm_out, make_float4(x[0], x[1], x[2], pi.w));
+    put(vel_out, make_float4(v[0], v[1], v[2], 0.f));
     if (acc_out) {
-        acc_out[i] = make_float4(acc[0], acc[1], acc[2], 0.f);
-        mu_out[i] = mu;
-        xsph_out[i] = make_float4(xs[0], xs[1], xs[2], __uint_as_float(clamp));
+        put(acc_out, make_float4(acc[0], acc[1], acc[2], 0.f));
+        put(mu_out, mu);
+        put(xsph_out, make_float4(xs[0], xs[1], xs[2], __uint_as_float(clamp)));
     }
     // adjoint tape: viscous force per unit mu (m cv sum ...) and the raw
     // shear-rate sum of the power law (0 for a Newtonian pass)
     if (fvisc_out) {
         const float u = rho_i > 0.f ? pi.w * P.cv : 0.f;
-        fvisc_out[i] = make_float4(u * A.s[3], u * A.s[4], u * A.s[5], SHEAR ? A.s[9] : 0.f);
+        put(fvisc_out, make_float4(u * A.s[3], u * A.s[4], u * A.s[5], SHEAR ? A.s[9] : 0.f));
     }
 
     // watchdog: smallest original index with a non-finite position / velocity
-    uint32_t id = idv[i];
-    id_out[i] = id;
+    uint32_t id = at(idv, o16);
+    put(id_out, id);
     uint32_t bp = finite3(x[0], x[1], x[2]) ? 0xffffffffu : id;
     uint32_t bv = finite3(v[0], v[1], v[2]) ? 0xffffffffu : id;
     if (bp != 0xffffffffu) atomicMin(&T.dom[TL.s].bad_pos, bp);
