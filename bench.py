@@ -230,7 +230,7 @@ def cpu_sample(sc, n, dt, snaps, budget, threads):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+        print(json.dumps({"error": "this benchmark runs on one GPU; --gpus must be 1",
                           "n_gpus": a.gpus}), flush=True)
         sys.exit(2)
     from physics_amd import ffi, scenes

@@ -24,7 +24,8 @@ SOURCES = ["sph_kernels.hip", "sph_stats.hip", "sph_adjoint.hip", "sph_couple.hi
 # the coupling's f64 arithmetic rounds operation by operation like the reference
 EXTRA = {"sph_couple.hip": ["-ffp-contract=off"]}
 
-# -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 generation.
+# -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 generation (some of their forms
+# have given wrong results on gfx950 without any error).
 COMMON = [
     f"--offload-arch={ARCH}",
     "-O3",

@@ -368,7 +368,7 @@ void FluidSubsystem::step(double dt) { world_->step(dt); }
 
 void FluidSubsystem::couple(World& world, double dt) {
     // subsystem.rs:92-143 looks up a HeatField (only when thermal_expansion or
-    // heat_gain > 0) and a RigidSubsystem. This build hosts fluid subsystems only,
+    // heat_gain > 0) and a RigidSubsystem. This library hosts fluid subsystems only,
     // so neither lookup can succeed: the couple phase leaves the fluid untouched.
     (void)world;
     (void)dt;

@@ -36,7 +36,7 @@ def _readme_world():
     return g
 
 
-# ---- README dam break ------------------------------------
+# ---- README dam break (343 particles, 200 steps) -----------------------------
 
 def _closed_form_y(y0, steps, dt=0.01, g=-9.81, lo=-5.0, damp=0.4):
     """Semi-implicit Euler free fall with the floor reflection, in f64."""
@@ -100,7 +100,7 @@ def test_readme_dam_break_on_the_gpu():
     assert checked >= 110
 
 
-# ---- the 1M column ------------------------------------------
+# ---- the 1M column (100^3 particles) -------------------------------------------
 
 def test_full_size_column_first_20_steps_against_the_oracle():
     sc = scenes.dam_break((100, 100, 100), seed=1)
@@ -136,7 +136,7 @@ def test_full_size_column_one_step_parity_in_the_collapsed_sheet():
         assert v <= P.TOL_STEP, (f, v)
 
 
-# ---- 256 batched scenes at full size -------------------------
+# ---- 256 batched scenes at full size (4.6M particles) ---------------------------
 
 @pytest.mark.timeout(600)
 def test_full_size_batch_per_scene_parity_and_sets():

@@ -33,9 +33,10 @@ from physics_amd import scenes
 H = 0.2
 
 
-def test_kernel_coefficients_match_survey_values():
+def test_kernel_coefficients_match_closed_forms():
     c, _ = O.kernels(0.2)
-    # SURVEY derived facts: c6, cs, cv at f64 h = 0.2 and at h = (double)0.2f
+    # c6 = 315 / (64 pi h^9), cs = -45 / (pi h^6), cv = 45 / (pi h^6) at f64 h = 0.2
+    # and at h = (double)0.2f
     assert c[0] == pytest.approx(3059924.7481657104, rel=1e-15)
     assert c[1] == pytest.approx(-223811.63872297775, rel=1e-15)
     assert c[2] == pytest.approx(223811.63872297775, rel=1e-15)
