@@ -80,6 +80,14 @@ DevParams make_dev_params(const HostParams& hp, int cdiv) {
     P.s_hq = (float)(s * (1.0 + std::ldexp(1.0, -16)));
     if ((double)P.s_hq <= s * (1.0 + std::ldexp(1.0, -16))) P.s_hq = std::nextafter(P.s_hq, INFINITY);
     if (!(h >= 1e-15 && h <= 1e15)) P.s_hq = NAN;
+    // Band of the FP32 q = h^2 - s of the column density (an FMA chain: error
+    // below 10u h^2 for s <= 2 h^2, u = 2^-24, and |q| > h^2 / 2 beyond):
+    // outside +-q_band its sign is the reference's r < h. Same range as the
+    // column-local filter; elsewhere infinite, so the exact predicate decides
+    // every pair.
+    P.q_band = (float)(s * std::ldexp(1.0, -19));
+    if ((double)P.q_band <= s * std::ldexp(1.0, -19)) P.q_band = std::nextafter(P.q_band, INFINITY);
+    if (!(h >= 1e-15 && h <= 1e15)) P.q_band = INFINITY;
     if (!(h > 0.0)) {  // degenerate support radius: only the exact path decides
         P.s_lo = -INFINITY;
         P.s_hi = INFINITY;

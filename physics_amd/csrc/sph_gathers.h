@@ -63,22 +63,26 @@ struct SetAcc {
 // particles within three half cells of one column: about 20x the rest density
 // or more) walks its shared window (the uniform scan's rows, trimmed to the
 // lanes' z-range, and the overflow segment) four candidates at a time with
-// wave-uniform loads, and every lane evaluates every candidate: s from FP32
-// differences of the sorted positions, the FP32 test outside the band and the
-// exact f64 predicate inside it (the same sets as the list path), and sums
-// T = sum m_j q^3 (q = h^2 - s) in FP32 partials of 16 candidates folded into
-// an f64 total. No lists, no drains: with about half of the window's
-// candidates being neighbours, this costs fewer operations than filtering,
-// listing and draining them. Error, with u = 2^-24: |s32 - s| <= 5u s, so
-// |q32 - q| <= 7u h^2 (h^2 rounded, s <= h^2), a term's error is at most 21u
-// h^2 m q^2 + 3u m q^3, and a partial of at most 16 positive terms adds 15u.
-// With E = sum m q^2 summed alongside (in FP32: an error below 2^-11 relative
-// for up to 2^12 terms), the relative error of T is at most (21 R + 18) u, R =
-// h^2 E / T (1.5 for neighbours spread evenly over the ball). A lane accepts
-// its FP32 density when R <= 1.99 (60u), rho - rho0 >= 0.75 rho (p = k (rho -
-// rho0) then within 80u = 5e-6 relative) and 2^-100 <= T <= 2^100 (no FP32
-// overflow, underflow negligible); when any lane of the wave does not, the
-// wave sums again in f64 through the list path.
+// wave-uniform loads, and every lane evaluates every candidate: q = h^2 - s by
+// an FMA chain over FP32 differences of the sorted positions, decided by its
+// sign outside a band of +-2^-19 h^2 and by the exact f64 predicate inside it
+// (the same sets as the list path), and sums T = sum m_j q^3 in FP32 partials
+// of 16 candidates folded into an f64 total. No lists, no drains: with about
+// half of the window's candidates being neighbours, this costs fewer
+// operations than filtering, listing and draining them. Error, with u =
+// 2^-24, for a neighbour (s <= h^2, so every step of the chain lies in [0,
+// h^2]): h^2 rounded (u h^2), the differences squared (2u s) and three
+// roundings (2u h^2 + u q) give |q32 - q| <= 5u h^2; a term's error is at most
+// 15u h^2 m q^2 + 3u m q^3, and a partial of at most 16 positive terms adds
+// 15u. With E = sum m q^2 summed alongside (in FP32: an error below 2^-11
+// relative for up to 2^12 terms), the relative error of T is at most (15 R +
+// 18) u, R = h^2 E / T (1.5 for neighbours spread evenly over the ball). A lane
+// accepts its FP32 density when R <= 1.99 (48u), rho - rho0 >= 0.75 rho (p =
+// k (rho - rho0) then within 64u = 4e-6 relative) and 2^-100 <= T <= 2^100 (no
+// FP32 overflow, underflow negligible); when any lane of the wave does not,
+// the wave sums again in f64 through the list path. A neighbour inside the
+// band whose q32 rounds to 0 or below adds 0 (its true term is below m (5u
+// h^2)^3).
 constexpr double kLowpShare = 0.75;
 constexpr double kLowpRatio = 1.99;
 
@@ -91,27 +95,22 @@ __device__ __forceinline__ void density_column(const WaveCol& wc, bool act, cons
                                                const float4* __restrict__ posm, const GatherStats& gs,
                                                uint32_t soff, double& sum, float& esum, SetAcc& sa) {
     float part = 0.f;
-    // in: the candidate exists (wave-uniform in the rows, the key adjacency
-    // of the overflow segment); SEL: its data may be non-finite (overflow)
-    auto term = [&](uint32_t j, const float4& q, bool in, auto sel) {
+    // q = h^2 - s in FP32: an FMA chain over the sorted positions' differences
+    auto qfp = [&](const float4& q) {
         const float dx = q.x - pi.x, dy = q.y - pi.y, dz = q.z - pi.z;
-        const float s = dx * dx + dy * dy + dz * dz;
-        bool valid = s < P.s_lo;
-        // the f64 band: the reference's own distance, inline (a call here
-        // would cost the kernel registers); rarely entered; NaN fails both
-        if (!valid && s < P.s_hi) {
-            const double ex = (double)pi.x - (double)q.x, ey = (double)pi.y - (double)q.y,
-                         ez = (double)pi.z - (double)q.z;
-            valid = sq_exact(ex, ey, ez) < P.s_thresh;
-        }
-        valid = valid && in;
-        float qq = P.h2 - s;
-        if (decltype(sel)::value) qq = valid ? qq : 0.f;
-        const float mq2 = valid ? q.w * (qq * qq) : 0.f;
-        part = fmaf(mq2, qq, part);
-        esum += mq2;
-        if (STATS) sa.add(valid, gs.id[j] - soff);
+        return fmaf(-dx, dx, fmaf(-dy, dy, fmaf(-dz, dz, P.h2)));
     };
+    // the reference's own f64 distance, inline (a call here would cost the
+    // kernel registers); NaN fails
+    auto exact_in = [&](const float4& q) {
+        const double ex = (double)pi.x - (double)q.x, ey = (double)pi.y - (double)q.y,
+                     ez = (double)pi.z - (double)q.z;
+        return sq_exact(ex, ey, ez) < P.s_thresh;
+    };
+    const float qb = P.q_band;
+    // a candidate's decision: |q| > q_band decides by the sign of q, the band
+    // between by the exact predicate (NaN fails both)
+    auto decide = [&](float qv, const float4& q) { return fabsf(qv) > qb ? qv > 0.f : exact_in(q); };
     const CellCoord cc = decode(wc.c0, d);
     const long long cz1 = cc.cz + (long long)(wc.c1 - wc.c0);
     const double zmn = (double)wave_min_f(act ? pi.z : INFINITY);
@@ -132,26 +131,67 @@ __device__ __forceinline__ void density_column(const WaveCol& wc, bool act, cons
                 b = uni(b);
             }
             // four candidates per round from one 64-byte wave-uniform load (the
-            // array has kScanPad spare entries); a 16-candidate partial is
-            // folded into the f64 total
+            // array has kScanPad spare entries). Each lane takes max(q, 0), the
+            // term of every candidate outside the band; a round that finds one
+            // of its lanes inside the band (|q| <= q_band, from a min over the
+            // four |q|) lets those lanes zero the candidates the exact predicate
+            // rejects. Full rounds first, then the run's last partial round with
+            // the masses past its end zeroed. A 16-candidate partial is folded
+            // into the f64 total.
             int round = 0;
-            for (uint32_t j = a; j < b; j += 4u) {
+            auto group = [&](uint32_t j, auto tail) {
                 const Quad4 qm = *reinterpret_cast<const Quad4*>(posm + j);
+                float qe[4];
+                float band = INFINITY;
 #pragma unroll
-                for (uint32_t u = 0; u < 4; ++u) term(j + u, qm.p[u], j + u < b, std::false_type{});
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const float qv = qfp(qm.p[u]);
+                    band = fminf(band, fabsf(qv));
+                    qe[u] = fmaxf(qv, 0.f);  // NaN: 0
+                }
+                bool vb[4];
+                if (STATS) {
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u)
+                        vb[u] = (!decltype(tail)::value || j + u < b) && decide(qfp(qm.p[u]), qm.p[u]);
+                }
+                if (wave_any(band <= qb)) {
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u)
+                        if (fabsf(qfp(qm.p[u])) <= qb && !exact_in(qm.p[u])) qe[u] = 0.f;
+                }
+#pragma unroll
+                for (uint32_t u = 0; u < 4; ++u) {
+                    const float m = !decltype(tail)::value || j + u < b ? qm.p[u].w : 0.f;
+                    const float mq2 = m * (qe[u] * qe[u]);
+                    part = fmaf(mq2, qe[u], part);
+                    esum += mq2;
+                    if (STATS) sa.add(vb[u], gs.id[j + u] - soff);
+                }
                 if (++round == 4) {
                     sum += (double)part;
                     part = 0.f;
                     round = 0;
                 }
-            }
+            };
+            uint32_t j = a;
+            for (; j + 4u <= b; j += 4u) group(j, std::false_type{});
+            if (j < b) group(j, std::true_type{});
             sum += (double)part;
             part = 0.f;
         }
     const uint32_t o0 = uni(start[d.ncells]), o1 = uni(start[d.ncells + 1]);
     for (uint32_t j = o0; j < o1; ++j) {
+        // the overflow segment: per-lane membership (key adjacency), and data
+        // that may be non-finite, so both factors are selected
         const float4 q = posm[j];
-        term(j, q, keys_adjacent(pi.x, pi.y, pi.z, q.x, q.y, q.z, P.inv_cell), std::true_type{});
+        const float qv = qfp(q);
+        const bool valid = keys_adjacent(pi.x, pi.y, pi.z, q.x, q.y, q.z, P.inv_cell) && decide(qv, q);
+        const float qe = valid ? fmaxf(qv, 0.f) : 0.f;
+        const float mq2 = valid ? q.w * (qe * qe) : 0.f;
+        part = fmaf(mq2, qe, part);
+        esum += mq2;
+        if (STATS) sa.add(valid, gs.id[j] - soff);
         sum += (double)part;
         part = 0.f;
     }
