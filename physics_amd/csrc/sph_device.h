@@ -674,13 +674,16 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                 // the lane's position relative to the row's column corner
                 const float xr = cl.li.x - (float)ex * cl.hh, yr = cl.li.y - (float)ey * cl.hh;
                 const float mx = -2.f * xr, my = -2.f * yr;
-                const float thr = cl.s_hq - fmaf(yr, yr, xr * xr);
+                // an inactive lane's bound is -inf: it takes no candidate (the
+                // scan's data are finite), with no per-candidate lane mask
+                const float thr = act ? cl.s_hq - fmaf(yr, yr, xr * xr) : -INFINITY;
                 auto loc_maybe = [&](const float4& q) {
                     const float dz = q.w - zi;
                     const float t = fmaf(q.y, my, fmaf(q.x, mx, fmaf(dz, dz, q.z)));
                     return !(t >= thr);  // a NaN bound passes everything
                 };
                 const float4* __restrict__ src = LOC ? cl.posl : posm;
+                const bool la = LOC || act;
                 auto maybe = [&](uint32_t j, const float4& q) { return LOC ? loc_maybe(q) : test(j, q); };
                 // Both ends of the z-sorted run at once: four candidates from the
                 // front and four from the back per round. Lanes sit at different z,
@@ -694,9 +697,9 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                     const Quad4 qa = *reinterpret_cast<const Quad4*>(src + lo);
                     const Quad4 qb = *reinterpret_cast<const Quad4*>(src + hi - 4u);
 #pragma unroll
-                    for (uint32_t u = 0; u < 4; ++u) push(lo + u, maybe(lo + u, qa.p[u]) & act);
+                    for (uint32_t u = 0; u < 4; ++u) push(lo + u, maybe(lo + u, qa.p[u]) & la);
 #pragma unroll
-                    for (uint32_t u = 0; u < 4; ++u) push(hi - 4u + u, maybe(hi - 4u + u, qb.p[u]) & act);
+                    for (uint32_t u = 0; u < 4; ++u) push(hi - 4u + u, maybe(hi - 4u + u, qb.p[u]) & la);
                     lo += 4u;
                     hi -= 4u;
                     if (wave_any(lp > lp8)) drain(DrainWidth<S>{}, std::true_type{});
@@ -710,10 +713,10 @@ __device__ __forceinline__ void gather(bool act, uint32_t ci, float xi, float yi
                     const Quad4 qb = *reinterpret_cast<const Quad4*>(src + lo + 4u);
 #pragma unroll
                     for (uint32_t u = 0; u < 4; ++u)
-                        push(lo + u, maybe(lo + u, qa.p[u]) & (act & (lo + u <= last)));
+                        push(lo + u, maybe(lo + u, qa.p[u]) & (la & (lo + u <= last)));
 #pragma unroll
                     for (uint32_t u = 0; u < 4; ++u)
-                        push(lo + 4u + u, maybe(lo + 4u + u, qb.p[u]) & (act & (lo + 4u + u <= last)));
+                        push(lo + 4u + u, maybe(lo + 4u + u, qb.p[u]) & (la & (lo + 4u + u <= last)));
                     if (wave_any(lp > lp8)) drain(DrainWidth<S>{}, std::true_type{});
                 }
             }
@@ -847,35 +850,48 @@ __device__ __forceinline__ double sq_exact(double dx, double dy, double dz) {
 
 // Exact density-pass decision (r < h, self included) for a listed slot with
 // FP32 squared distance s32; the f64 band and NaN take the exact predicate.
+// The vote is one compare (an unlisted slot's key is 0, inside), and the f64
+// call runs for every lane behind it: the common path changes no execution
+// mask.
 __device__ __forceinline__ bool density_valid(const DevParams& P, bool listed, float xi, float yi,
                                               float zi, const float4& pj, float s32) {
-    const bool fast = s32 < P.s_lo;
-    const bool need = listed && !fast;
+    const float key = listed ? s32 : 0.f;
+    const bool fast = key < P.s_lo;
     bool ex = false;
-    // a divergent branch around the out-of-line f64 call: the wave skips it
-    // when no lane needs it
-    if (need) ex = exact_s64(xi, yi, zi, pj.x, pj.y, pj.z) < P.s_thresh;
+    if (wave_any(!(key < P.s_lo))) ex = !fast && exact_s64(xi, yi, zi, pj.x, pj.y, pj.z) < P.s_thresh;
     return listed && (fast || ex);
+}
+
+// The FP32 side of the force-pass decision in integer form. s32 is +0,
+// positive or NaN, so with t = bits(s32) - 1 (+0 wraps to the top) 0 < s32 <
+// s_lo is t < fast_bound (0 when s_lo is not positive: nothing is clear). A
+// slot with t >= fast_bound needs the exact predicate unless it names the
+// particle itself; its key is t, the particle's own slots' key 0, so the
+// wave's vote is one compare.
+__device__ __forceinline__ uint32_t fast_bound(const DevParams& P) {
+    return P.s_lo > 0.f ? __float_as_uint(P.s_lo) - 1u : 0u;
 }
 
 // Exact force-pass decision for a listed slot j of particle i (world.rs:
 // 292-299): skip j == i, r >= h and r <= 0; a NaN distance is kept. i and j
 // are compared for equality only (indices or list offsets alike). s32 is the
 // slot's FP32 squared distance. Clear cases are decided in FP32; the f64 band,
-// NaN and coincident distinct points take the exact f64 predicate, entered only
-// when some lane of the wave needs it.
+// NaN and coincident distinct points take the exact f64 predicate, which runs
+// for every lane when some lane needs it (the common path changes no
+// execution mask).
 __device__ __forceinline__ bool force_valid(const DevParams& P, bool listed, uint32_t i, uint32_t j,
                                             float xi, float yi, float zi, const float4& pj,
                                             float s32) {
-    const bool fast = s32 < P.s_lo && s32 > 0.f;  // s32 > 0: distinct points, so j != i
-    const bool need = listed && j != i && !fast;
+    const uint32_t tb = fast_bound(P), t = __float_as_uint(s32) - 1u;
+    const bool fast = listed && t < tb;
+    const uint32_t key = listed && j != i ? t : 0u;
     bool ex = false;
-    if (need) {
+    if (wave_any(key >= tb)) {
         const double e = exact_s64(xi, yi, zi, pj.x, pj.y, pj.z);
         const bool nz = xi != pj.x || yi != pj.y || zi != pj.z;
-        ex = (e < P.s_thresh && nz) || e != e;
+        ex = key >= tb && ((e < P.s_thresh && nz) || e != e);
     }
-    return listed && (fast || ex);
+    return fast || ex;
 }
 
 // force_valid for a proc that multiplies its zero weights into the
@@ -887,16 +903,18 @@ __device__ __forceinline__ bool force_valid_v(const DevParams& P, bool listed, u
                                               uint32_t j, float xi, float yi, float zi,
                                               const float4& pj, float s32, const float4& vi,
                                               float4& vj) {
-    const bool fast = s32 < P.s_lo && s32 > 0.f;
-    const bool need = listed && j != i && !fast;
+    const uint32_t tb = fast_bound(P), t = __float_as_uint(s32) - 1u;
+    const bool fast = listed && t < tb;
+    const uint32_t key = listed && j != i ? t : 0u;
     bool ex = false;
-    if (need) {
+    if (wave_any(key >= tb)) {
         const double e = exact_s64(xi, yi, zi, pj.x, pj.y, pj.z);
         const bool nz = xi != pj.x || yi != pj.y || zi != pj.z;
-        ex = (e < P.s_thresh && nz) || e != e;
-        if (!ex) vj = vi;
+        const bool need = key >= tb;
+        ex = need && ((e < P.s_thresh && nz) || e != e);
+        if (need && !ex) vj = vi;
     }
-    return listed && (fast || ex);
+    return fast || ex;
 }
 
 }  // namespace phy
