@@ -176,6 +176,7 @@ DeviceFluid::DeviceFluid(const std::vector<HostParams>& scenes) {
     hip_check(hipStreamCreateWithFlags(&st_, hipStreamNonBlocking), "hipStreamCreate");
     hps_ = scenes;
     Ps_ = ps;
+    m_uni_.assign(scenes.size(), NAN);
     const size_t S = scenes.size();
     scn_.assign(S, SceneInfo{});
     for (size_t s = 0; s < S; ++s) {
@@ -289,6 +290,7 @@ void DeviceFluid::push_params() {
     const size_t S = scn_.size();
     any_shear_ = false;
     for (const DevParams& P : Ps_) any_shear_ = any_shear_ || !P.newtonian;
+    for (size_t s = 0; s < S; ++s) Ps_[s].m_uni = m_uni_[s];
     d_params_.ensure(S);
     d_info_.ensure(S);
     hip_check(hipMemcpyAsync(d_params_.get(), Ps_.data(), S * sizeof(DevParams), hipMemcpyHostToDevice, st_), "params");
@@ -364,6 +366,21 @@ void DeviceFluid::upload_scenes(const uint32_t* counts, const float* pos3, const
     set_layout(counts);
     ensure_capacity(n_);
     size_tables(pos3);
+    // per scene the common mass (bit-equal masses), which the column density
+    // factors out of its terms; NaN otherwise
+    for (size_t s = 0; s < scn_.size(); ++s) {
+        const float m0 = (float)hps_[s].mass;
+        float mu = NAN;
+        for (uint32_t k = 0; k < scn_[s].n; ++k) {
+            const float m = mass ? mass[(size_t)scn_[s].off + k] : m0;
+            if (k == 0) mu = m;
+            else if (__builtin_memcmp(&m, &mu, sizeof(float)) != 0) {
+                mu = NAN;
+                break;
+            }
+        }
+        m_uni_[s] = mu;
+    }
     push_params();
     const size_t n = n_;
     std::vector<float4> p4(n), v4(n);

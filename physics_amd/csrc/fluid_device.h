@@ -291,6 +291,7 @@ class DeviceFluid {
 
     std::vector<HostParams> hps_;
     std::vector<DevParams> Ps_;
+    std::vector<float> m_uni_;  // per scene: the common particle mass, NaN when masses differ
     std::vector<SceneInfo> scn_;
     std::vector<uint32_t> tile_scene_;
     int ntiles_ = 0;

@@ -117,69 +117,90 @@ __device__ __forceinline__ void density_column(const WaveCol& wc, bool act, cons
     const double zmx = (double)wave_max_f(act ? pi.z : -INFINITY);
     const double zw0 = zt.w0, zw1 = zt.w1, zw2 = zt.w2;
     const int reach = (int)uni((uint32_t)d.reach);
-    for (int ex = -reach; ex <= reach; ++ex)
-        for (int ey = -reach; ey <= reach; ++ey) {
-            uint32_t a, b;
-            col_run(ex, ey, cc, cz1, d, start, a, b);
-            a = uni(a);
-            b = uni(b);
-            if (zt.on && a < b) {
-                const int cls = (ex == reach || ex == -reach) + (ey == reach || ey == -reach);
-                const double w = cls == 0 ? zw0 : (cls == 1 ? zw1 : zw2);
-                trim_run<false>(posm, f32_down(zmn - w), f32_up(zmx + w), a, b);
+    // UM: every particle of the scene has the mass m_uni, factored out of the
+    // rows' terms (T and E scaled once after the rows)
+    auto rows = [&](auto um) {
+        constexpr bool UM = decltype(um)::value;
+        for (int ex = -reach; ex <= reach; ++ex)
+            for (int ey = -reach; ey <= reach; ++ey) {
+                uint32_t a, b;
+                col_run(ex, ey, cc, cz1, d, start, a, b);
                 a = uni(a);
                 b = uni(b);
+                if (zt.on && a < b) {
+                    const int cls = (ex == reach || ex == -reach) + (ey == reach || ey == -reach);
+                    const double w = cls == 0 ? zw0 : (cls == 1 ? zw1 : zw2);
+                    trim_run<false>(posm, f32_down(zmn - w), f32_up(zmx + w), a, b);
+                    a = uni(a);
+                    b = uni(b);
+                }
+                // four candidates per round from one 64-byte wave-uniform load
+                // (the array has kScanPad spare entries). Each lane takes max(q,
+                // 0), the term of every candidate outside the band; a round that
+                // finds one of its lanes inside the band (|q| <= q_band, from a
+                // min over the four |q|) lets those lanes zero the candidates the
+                // exact predicate rejects. Full rounds first, then the run's last
+                // partial round with the candidates past its end zeroed. A
+                // 16-candidate partial is folded into the f64 total.
+                int round = 0;
+                auto group = [&](uint32_t j, auto tail) {
+                    constexpr bool TAIL = decltype(tail)::value;
+                    const Quad4 qm = *reinterpret_cast<const Quad4*>(posm + j);
+                    float qe[4];
+                    float band = INFINITY;
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) {
+                        const float qv = qfp(qm.p[u]);
+                        band = fminf(band, fabsf(qv));
+                        qe[u] = fmaxf(qv, 0.f);  // NaN: 0
+                    }
+                    bool vb[4];
+                    if (STATS) {
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; ++u)
+                            vb[u] = (!TAIL || j + u < b) && decide(qfp(qm.p[u]), qm.p[u]);
+                    }
+                    if (wave_any(band <= qb)) {
+#pragma unroll
+                        for (uint32_t u = 0; u < 4; ++u)
+                            if (fabsf(qfp(qm.p[u])) <= qb && !exact_in(qm.p[u])) qe[u] = 0.f;
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < 4; ++u) {
+                        const bool in = !TAIL || j + u < b;
+                        float mq2;
+                        if (UM) {
+                            const float q = in ? qe[u] : 0.f;
+                            mq2 = q * q;
+                            qe[u] = q;
+                        } else {
+                            mq2 = (in ? qm.p[u].w : 0.f) * (qe[u] * qe[u]);
+                        }
+                        part = fmaf(mq2, qe[u], part);
+                        esum += mq2;
+                        if (STATS) sa.add(vb[u], gs.id[j + u] - soff);
+                    }
+                    if (++round == 4) {
+                        sum += (double)part;
+                        part = 0.f;
+                        round = 0;
+                    }
+                };
+                uint32_t j = a;
+                for (; j + 4u <= b; j += 4u) group(j, std::false_type{});
+                if (j < b) group(j, std::true_type{});
+                sum += (double)part;
+                part = 0.f;
             }
-            // four candidates per round from one 64-byte wave-uniform load (the
-            // array has kScanPad spare entries). Each lane takes max(q, 0), the
-            // term of every candidate outside the band; a round that finds one
-            // of its lanes inside the band (|q| <= q_band, from a min over the
-            // four |q|) lets those lanes zero the candidates the exact predicate
-            // rejects. Full rounds first, then the run's last partial round with
-            // the masses past its end zeroed. A 16-candidate partial is folded
-            // into the f64 total.
-            int round = 0;
-            auto group = [&](uint32_t j, auto tail) {
-                const Quad4 qm = *reinterpret_cast<const Quad4*>(posm + j);
-                float qe[4];
-                float band = INFINITY;
-#pragma unroll
-                for (uint32_t u = 0; u < 4; ++u) {
-                    const float qv = qfp(qm.p[u]);
-                    band = fminf(band, fabsf(qv));
-                    qe[u] = fmaxf(qv, 0.f);  // NaN: 0
-                }
-                bool vb[4];
-                if (STATS) {
-#pragma unroll
-                    for (uint32_t u = 0; u < 4; ++u)
-                        vb[u] = (!decltype(tail)::value || j + u < b) && decide(qfp(qm.p[u]), qm.p[u]);
-                }
-                if (wave_any(band <= qb)) {
-#pragma unroll
-                    for (uint32_t u = 0; u < 4; ++u)
-                        if (fabsf(qfp(qm.p[u])) <= qb && !exact_in(qm.p[u])) qe[u] = 0.f;
-                }
-#pragma unroll
-                for (uint32_t u = 0; u < 4; ++u) {
-                    const float m = !decltype(tail)::value || j + u < b ? qm.p[u].w : 0.f;
-                    const float mq2 = m * (qe[u] * qe[u]);
-                    part = fmaf(mq2, qe[u], part);
-                    esum += mq2;
-                    if (STATS) sa.add(vb[u], gs.id[j + u] - soff);
-                }
-                if (++round == 4) {
-                    sum += (double)part;
-                    part = 0.f;
-                    round = 0;
-                }
-            };
-            uint32_t j = a;
-            for (; j + 4u <= b; j += 4u) group(j, std::false_type{});
-            if (j < b) group(j, std::true_type{});
-            sum += (double)part;
-            part = 0.f;
-        }
+    };
+    const float mu = P.m_uni;
+    if (mu == mu) {
+        rows(std::true_type{});
+        sum *= (double)mu;
+        esum *= mu;
+    } else {
+        rows(std::false_type{});
+    }
     const uint32_t o0 = uni(start[d.ncells]), o1 = uni(start[d.ncells + 1]);
     for (uint32_t j = o0; j < o1; ++j) {
         // the overflow segment: per-lane membership (key adjacency), and data

@@ -44,6 +44,7 @@ struct DevParams {
     float s_lo, s_hi;    // FP32 filter: s32 < s_lo => inside; s32 > s_hi => outside
     float s_hq;          // s_thresh (1 + 2^-16) rounded up: bound of the column-local filter
     float q_band;        // s_thresh 2^-19 rounded up: |q32| > q_band decides h^2 - s in FP32
+    float m_uni;         // the scene's common particle mass, NaN when the masses differ
     float h, h2;         // FP32 support radius and its square
     float c6, cs, cv;    // poly6, spiky-gradient, viscosity-laplacian coefficients
     float rest, stiff, shear_min, r_eps;

@@ -82,25 +82,28 @@ def test_one_step_parity_moving_compressed_multimaterial():
         assert v <= P.TOL_STEP, (k, v)
 
 
-@pytest.mark.parametrize("rest", [1000.0, 6000.0, 1e6])
-def test_compressed_column_waves_parity_and_sets(rest):
+@pytest.mark.parametrize("rest,varied", [(1000.0, False), (6000.0, False), (1e6, False), (1000.0, True),
+                                         (6000.0, True)])
+def test_compressed_column_waves_parity_and_sets(rest, varied):
     # 27 particles per half cell (spacing h/6), so whole waves sit within three
     # half cells of one column and the density pass tries FP32 terms first:
     # rho ~ 27 rho0 keeps them at rho0 = 1000; at 6000 lanes near the block's
     # faces fall below rho - rho0 >= 0.75 rho and their waves sum again in f64;
     # at 1e6 every wave does (p = 0), so rho carries only the f64 sum's final
-    # rounding to FP32
+    # rounding to FP32. Equal masses take the terms with the common mass
+    # factored out, varied masses (0.8 .. 1.2 x 0.99) the per-candidate masses.
     pos = _block(12, 5, jit=0.05, s=float(np.float32(scenes.H32 / 6)), origin=(0.31, 0.47, 0.23))
     n = len(pos)
     vel = (0.2 * (scenes.rng_f64(6, 3 * n).reshape(n, 3) - 0.5)).astype(np.float32)
-    g, o = P.make_pair(pos, vel, mass=0.99, rest_density=rest)
+    mass = (0.99 * (0.8 + 0.4 * scenes.rng_f64(8, n))).astype(np.float32) if varied else 0.99
+    g, o = P.make_pair(pos, vel, mass=mass, rest_density=rest)
     for which, ofn in (("density", o.neighbor_stats), ("force", o.force_neighbor_stats)):
         c, s, x = g.gather_stats(which)
         oc, os_, ox = ofn()
         assert np.array_equal(c, oc) and np.array_equal(s, os_) and np.array_equal(x, ox), which
     x0 = P.step_both(g, o)
     err = P.compare_step(g, o, x0)
-    _print(f"compressed rest={rest:g}", err)
+    _print(f"compressed rest={rest:g} varied={varied}", err)
     for k, v in err.items():
         assert v <= P.TOL_STEP, (k, v)
     if rest == 1e6:
