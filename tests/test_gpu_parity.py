@@ -112,6 +112,35 @@ def test_compressed_column_waves_parity_and_sets(rest, varied):
         assert rel.max() <= 2.0 ** -24 * (1 + 1e-6), rel.max()
 
 
+
+def test_compressed_column_waves_at_the_support_boundary():
+    # the compressed block of the test above (column waves: candidate-major
+    # density, uniform force scan) with partners placed within a few ulps of
+    # r = h of 200 block particles, so the density's band vote and exact
+    # predicate and the force pass's pair test decide those pairs
+    h = scenes.H32
+    pos = _block(12, 5, jit=0.05, s=float(np.float32(h / 6)), origin=(0.31, 0.47, 0.23))
+    rng = np.random.default_rng(11)
+    idx = rng.choice(len(pos), 200, replace=False)
+    d = rng.normal(size=(200, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    eps = rng.integers(-4, 5, 200) * 1e-8
+    other = (pos[idx].astype(np.float64) + d * (h + eps)[:, None]).astype(np.float32)
+    allp = np.concatenate([pos, other])
+    g, o = P.make_pair(allp, mass=0.99)
+    for which, ofn in (("density", o.neighbor_stats), ("force", o.force_neighbor_stats)):
+        c, s, x = g.gather_stats(which)
+        oc, os_, ox = ofn()
+        assert np.array_equal(c, oc) and np.array_equal(s, os_) and np.array_equal(x, ox), which
+    # the band is populated: some placed pairs are neighbours, some are not
+    r = np.linalg.norm(other.astype(np.float64) - pos[idx].astype(np.float64), axis=1)
+    assert (r < h).any() and (r >= h).any()
+    x0 = P.step_both(g, o)
+    err = P.compare_step(g, o, x0)
+    _print("compressed boundary", err)
+    for k, v in err.items():
+        assert v <= P.TOL_STEP, (k, v)
+
 def test_per_step_parity_over_a_collapse():
     # the oracle is re-fed the GPU state every step: the per-step bar holds along
     # a trajectory that compresses the column against the floor
