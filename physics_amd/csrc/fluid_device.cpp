@@ -374,7 +374,7 @@ void DeviceFluid::upload_scenes(const uint32_t* counts, const float* pos3, const
         for (uint32_t k = 0; k < scn_[s].n; ++k) {
             const float m = mass ? mass[(size_t)scn_[s].off + k] : m0;
             if (k == 0) mu = m;
-            else if (__builtin_memcmp(&m, &mu, sizeof(float)) != 0) {
+            else if (memcmp(&m, &mu, sizeof(float)) != 0) {
                 mu = NAN;
                 break;
             }
