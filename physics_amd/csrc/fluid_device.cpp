@@ -290,7 +290,11 @@ void DeviceFluid::push_params() {
     const size_t S = scn_.size();
     any_shear_ = false;
     for (const DevParams& P : Ps_) any_shear_ = any_shear_ || !P.newtonian;
-    for (size_t s = 0; s < S; ++s) Ps_[s].m_uni = m_uni_[s];
+    all_um_ = S > 0;
+    for (size_t s = 0; s < S; ++s) {
+        Ps_[s].m_uni = m_uni_[s];
+        all_um_ = all_um_ && m_uni_[s] == m_uni_[s];
+    }
     d_params_.ensure(S);
     d_info_.ensure(S);
     hip_check(hipMemcpyAsync(d_params_.get(), Ps_.data(), S * sizeof(DevParams), hipMemcpyHostToDevice, st_), "params");
@@ -327,6 +331,7 @@ void DeviceFluid::ensure_capacity(size_t n) {
         // the uniform scan loads candidates eight at a time
         posm_s_.ensure(c + kScanPad); velm_s_.ensure(c); body_s_.ensure(c);
         posl_.ensure(c + kScanPad);
+        frec_.ensure(c);
         id_s_.ensure(c); mat_s_.ensure(c); cell_s_.ensure(c);
         rho_.ensure(c); prs_.ensure(c); mp_.ensure(c);
         cell_.ensure(c); rank_.ensure(c); tmp_.ensure(c); ctmp_.ensure(c); keytmp_.ensure(c);
@@ -524,7 +529,12 @@ void DeviceFluid::step_core(float dt, const StepBufs& s) {
     run_domain_and_sort(s);
     if (prof_) ev();
     const SceneTable T = table(s.dom);
-    launch_density(st_, T, s.posm_s, s.posl, s.cell_s, s.start, s.rho, s.prs, s.velm_s, s.mp,
+    // with every scene's masses equal the density pass writes each slot's (x,
+    // y, z, m p / rho^2) and the force pass reads its neighbours from that
+    // and velm (same values, two loads instead of three); one buffer serves
+    // every buffer set, like posl
+    float4* const frec = all_um_ ? frec_.get() : nullptr;
+    launch_density(st_, T, s.posm_s, s.posl, s.cell_s, s.start, s.rho, s.prs, s.velm_s, s.mp, frec,
                    tuning_);
     ForceArgs a;
     a.posm = s.posm_s;
@@ -533,6 +543,7 @@ void DeviceFluid::step_core(float dt, const StepBufs& s) {
     a.rho = s.rho;
     a.prs = s.prs;
     a.mp = s.mp;
+    a.frec = frec;
     a.cell = s.cell_s;
     a.start = s.start;
     a.id = s.id_s;

@@ -292,6 +292,7 @@ class DeviceFluid {
     std::vector<HostParams> hps_;
     std::vector<DevParams> Ps_;
     std::vector<float> m_uni_;  // per scene: the common particle mass, NaN when masses differ
+    bool all_um_ = false;       // every scene has a common mass: the force pass reads frec_
     std::vector<SceneInfo> scn_;
     std::vector<uint32_t> tile_scene_;
     int ntiles_ = 0;
@@ -314,6 +315,7 @@ class DeviceFluid {
     // sorted copies for the step
     DevBuf<float4> posm_s_, velm_s_, body_s_;
     DevBuf<float4> posl_;
+    DevBuf<float4> frec_;  // (x, y, z, m p / rho^2) per sorted slot, consumed within a step
     DevBuf<uint32_t> id_s_, mat_s_, cell_s_;
     DevBuf<float> rho_, prs_, mp_;
     // sort scratch

@@ -224,7 +224,7 @@ __global__ __launch_bounds__(BS) void k_density(SceneTable T, const float4* __re
                                                  const uint32_t* __restrict__ cellv,
                                                  const uint32_t* __restrict__ start_all,
                                                  float* rho_out, float* p_out, float4* velm,
-                                                 float* mp_out, GatherStats gs) {
+                                                 float* mp_out, float4* frec, GatherStats gs) {
     __shared__ uint32_t lst[L][BS];
     const int tid = threadIdx.x;
     const TileLoc TL = tile_loc_bs<BS>(T);
@@ -295,8 +295,10 @@ __global__ __launch_bounds__(BS) void k_density(SceneTable T, const float4* __re
     double p = over > 0.0 ? P.stiff64 * over : 0.0;
     rho_out[i] = (float)rho;
     p_out[i] = (float)p;
-    velm[i].w = (float)((double)pi.w / rho);             // m_j / rho_j of the force pass
-    mp_out[i] = (float)((double)pi.w * (p / (rho * rho)));  // m_j * p_j / rho_j^2
+    velm[i].w = (float)((double)pi.w / rho);  // m_j / rho_j of the force pass
+    const float mpj = (float)((double)pi.w * (p / (rho * rho)));  // m_j * p_j / rho_j^2
+    mp_out[i] = mpj;
+    if (frec) frec[i] = make_float4(pi.x, pi.y, pi.z, mpj);
 }
 
 // ---------------------------------------------------------------------------
@@ -338,7 +340,9 @@ struct NbrData {
 
 // Returns the exact force-pass decision for the slot (the pair's terms are
 // added only when it holds).
-template <bool SHEAR>
+// UM: the scene's masses are all m_uni, and nb.p carries m p / rho^2 in w
+// (the density pass's record) instead of the mass.
+template <bool SHEAR, bool UM = false>
 __device__ __forceinline__ bool pair_force(ForceAcc<SHEAR>& A, const DevParams& P, uint32_t i,
                                            uint32_t j, float4 pi, float4 vi, float Pi,
                                            const NbrData& nb, bool listed) {
@@ -362,7 +366,7 @@ __device__ __forceinline__ bool pair_force(ForceAcc<SHEAR>& A, const DevParams& 
     float q = P.h2 - s;
     // pressure: dir * (m_i m_j (P_i + P_j) * cs (h - r)^2), dir = d / r; selected
     // away for a rejected slot, whose m p / rho^2 may be NaN (rho = 0)
-    float a = fmaf(pj.w, Pi, nb.mp) * (hr * hr) * inv_r;
+    float a = (UM ? fmaf(P.m_uni, Pi, pj.w) : fmaf(pj.w, Pi, nb.mp)) * (hr * hr) * inv_r;
     a = valid ? a : 0.f;
     // viscosity: (v_j - v_i) * (m_i m_j / rho_j * cv (h - r))
     float bb = mr * hr;
@@ -387,11 +391,12 @@ __device__ __forceinline__ bool pair_force(ForceAcc<SHEAR>& A, const DevParams& 
     return valid;
 }
 
-template <int L, bool SHEAR, int MODE, bool STATS, int BS>
+template <int L, bool SHEAR, int MODE, bool STATS, int BS, bool UM>
 __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_force(
     SceneTable T, float dt, const float4* __restrict__ posm, const float4* __restrict__ posl,
     const float4* __restrict__ velm,
     const float* __restrict__ rho, const float* __restrict__ prs, const float* __restrict__ mp,
+    const float4* __restrict__ frec,
     const uint32_t* __restrict__ cellv, const uint32_t* __restrict__ start_all,
     const uint32_t* __restrict__ idv, const uint32_t* __restrict__ matv,
     const float4* __restrict__ body, float4* posm_out, float4* vel_out, uint32_t* id_out,
@@ -431,11 +436,15 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
     gather<L, MODE, (MODE == kUniformWaves ? 1 : 2), LOC, true, 6, BS>(
         act, ci, pi.x, pi.y, pi.z, self_j, d, P.inv_cell, zwin(P, d), cl, start, posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_maybe(P, pi.x, pi.y, pi.z, pj); },
-        [&](uint32_t o) { return NbrData{at(posm, o), at(velm, o), at(mp, o)}; },
+        // UM: two loads per neighbour (the density pass's record and velm)
+        [&](uint32_t o) {
+            if (UM) return NbrData{at(frec, o), at(velm, o), 0.f};
+            return NbrData{at(posm, o), at(velm, o), at(mp, o)};
+        },
         // every slot at or past the lane's count names the lane itself, which
         // no validity test accepts, so the listed flag is not needed
         [&](uint32_t o, const NbrData& nb, bool, auto) {
-            const bool ok = pair_force<SHEAR>(A, P, self_j << 4, o, pi, vi, Pi, nb, true);
+            const bool ok = pair_force<SHEAR, UM>(A, P, self_j << 4, o, pi, vi, Pi, nb, true);
             if (STATS) sa.add(ok, at(gs.id, o) - soff);
         },
         [&](auto) { A.merge(); }, [] { return false; });
@@ -540,63 +549,73 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
 template <int L, bool STATS, int BS>
 static void density_l(hipStream_t st, const SceneTable& T, bool split, const float4* posm,
                       const float4* posl, const uint32_t* cellv, const uint32_t* start, float* rho,
-                      float* prs, float4* velm, float* mp, const GatherStats& gs) {
+                      float* prs, float4* velm, float* mp, float4* frec, const GatherStats& gs) {
     const dim3 grid(T.ntiles * (kTile / BS)), block(BS);
     if (split) {
         hipLaunchKernelGGL((k_density<L, kUniformWaves, STATS, BS>), grid, block, 0, st, T,
-                           posm, posl, cellv, start, rho, prs, velm, mp, gs);
+                           posm, posl, cellv, start, rho, prs, velm, mp, frec, gs);
         hipLaunchKernelGGL((k_density<L, kOtherWaves, STATS, BS>), grid, block, 0, st, T,
-                           posm, posl, cellv, start, rho, prs, velm, mp, gs);
+                           posm, posl, cellv, start, rho, prs, velm, mp, frec, gs);
     } else {
         hipLaunchKernelGGL((k_density<L, kAllWaves, STATS, BS>), grid, block, 0, st, T, posm,
-                           posl, cellv, start, rho, prs, velm, mp, gs);
+                           posl, cellv, start, rho, prs, velm, mp, frec, gs);
     }
 }
 
 template <bool STATS>
 static void density_any(hipStream_t st, const SceneTable& T, const float4* posm, const float4* posl,
                         const uint32_t* cellv, const uint32_t* start, float* rho, float* prs,
-                        float4* velm, float* mp, const Tuning& tu, const GatherStats& gs) {
+                        float4* velm, float* mp, float4* frec, const Tuning& tu, const GatherStats& gs) {
     if (T.ntiles == 0) return;
     const bool sp = tu.split != 0;
     switch (tu.density_list * 1000 + tu.block) {
-        case 8128: density_l<8, STATS, 128>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
-        case 16128: density_l<16, STATS, 128>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
-        case 32128: density_l<32, STATS, 128>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
-        case 8256: density_l<8, STATS, 256>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
-        case 16256: density_l<16, STATS, 256>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
-        default: density_l<32, STATS, 256>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, gs); break;
+        case 8128: density_l<8, STATS, 128>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, frec, gs); break;
+        case 16128: density_l<16, STATS, 128>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, frec, gs); break;
+        case 32128: density_l<32, STATS, 128>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, frec, gs); break;
+        case 8256: density_l<8, STATS, 256>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, frec, gs); break;
+        case 16256: density_l<16, STATS, 256>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, frec, gs); break;
+        default: density_l<32, STATS, 256>(st, T, sp, posm, posl, cellv, start, rho, prs, velm, mp, frec, gs); break;
     }
 }
 
 
-template <int L, bool SHEAR, int MODE, bool STATS, int BS>
+template <int L, bool SHEAR, int MODE, bool STATS, int BS, bool UM>
 static void force_one(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a,
                       const GatherStats& gs) {
-    hipLaunchKernelGGL((k_force<L, SHEAR, MODE, STATS, BS>), dim3(T.ntiles * (kTile / BS)), dim3(BS), 0, st,
-                       T, dt, a.posm, a.posl, a.velm, a.rho, a.prs, a.mp, a.cell, a.start, a.id, a.mat, a.body,
+    hipLaunchKernelGGL((k_force<L, SHEAR, MODE, STATS, BS, UM>), dim3(T.ntiles * (kTile / BS)), dim3(BS), 0,
+                       st, T, dt, a.posm, a.posl, a.velm, a.rho, a.prs, a.mp, a.frec, a.cell, a.start, a.id,
+                       a.mat, a.body,
                        a.posm_out, a.vel_out, a.id_out, a.mat_out, a.acc_out, a.mu_out, a.xsph_out,
                        a.fvisc_out, gs);
 }
 
-template <int L, bool SHEAR, bool STATS, int BS>
+template <int L, bool SHEAR, bool STATS, int BS, bool UM>
 static void force_l(hipStream_t st, const SceneTable& T, bool split, float dt, const ForceArgs& a,
                     const GatherStats& gs) {
     if (split) {
-        force_one<L, SHEAR, kUniformWaves, STATS, BS>(st, T, dt, a, gs);
-        force_one<L, SHEAR, kOtherWaves, STATS, BS>(st, T, dt, a, gs);
+        force_one<L, SHEAR, kUniformWaves, STATS, BS, UM>(st, T, dt, a, gs);
+        force_one<L, SHEAR, kOtherWaves, STATS, BS, UM>(st, T, dt, a, gs);
     } else {
-        force_one<L, SHEAR, kAllWaves, STATS, BS>(st, T, dt, a, gs);
+        force_one<L, SHEAR, kAllWaves, STATS, BS, UM>(st, T, dt, a, gs);
     }
 }
 
 template <bool SHEAR, bool STATS, int BS>
 static void force_list(hipStream_t st, const SceneTable& T, bool split, float dt, const ForceArgs& a,
                        int list, const GatherStats& gs) {
+    // the statistics forms read no neighbour data beyond positions: one form
+    if (!STATS && a.frec) {
+        switch (list) {
+            case 8: force_l<8, SHEAR, STATS, BS, !STATS>(st, T, split, dt, a, gs); break;
+            case 16: force_l<16, SHEAR, STATS, BS, !STATS>(st, T, split, dt, a, gs); break;
+            default: force_l<32, SHEAR, STATS, BS, !STATS>(st, T, split, dt, a, gs); break;
+        }
+        return;
+    }
     switch (list) {
-        case 8: force_l<8, SHEAR, STATS, BS>(st, T, split, dt, a, gs); break;
-        case 16: force_l<16, SHEAR, STATS, BS>(st, T, split, dt, a, gs); break;
-        default: force_l<32, SHEAR, STATS, BS>(st, T, split, dt, a, gs); break;
+        case 8: force_l<8, SHEAR, STATS, BS, false>(st, T, split, dt, a, gs); break;
+        case 16: force_l<16, SHEAR, STATS, BS, false>(st, T, split, dt, a, gs); break;
+        default: force_l<32, SHEAR, STATS, BS, false>(st, T, split, dt, a, gs); break;
     }
 }
 

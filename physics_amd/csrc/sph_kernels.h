@@ -69,6 +69,10 @@ struct ForceArgs {
     const float* rho;
     const float* prs;
     const float* mp;
+    // with every scene's masses equal (m_uni): the density pass's (x, y, z,
+    // m p / rho^2) of each sorted slot, so a neighbour takes two loads (null:
+    // x and m from posm, m p / rho^2 from mp)
+    const float4* frec = nullptr;
     const uint32_t* cell;
     const uint32_t* start;
     const uint32_t* id;
@@ -145,7 +149,7 @@ void launch_domain(hipStream_t st, const SceneTable& T, const float4* posm);
 void launch_sort(hipStream_t st, const SceneTable& T, const SortBuffers& b);
 void launch_density(hipStream_t st, const SceneTable& T, const float4* posm, const float4* posl,
                     const uint32_t* cellv, const uint32_t* start, float* rho, float* prs,
-                    float4* velm, float* mp, const Tuning& tu);
+                    float4* velm, float* mp, float4* frec, const Tuning& tu);
 void launch_force(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a, bool shear,
                   const Tuning& tu);
 // FP64 diagnostics of one scene held in caller-owned device arrays (posm:

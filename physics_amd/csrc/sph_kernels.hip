@@ -630,8 +630,8 @@ bool tuning_valid(const Tuning& t) {
 
 void launch_density(hipStream_t st, const SceneTable& T, const float4* posm, const float4* posl,
                     const uint32_t* cellv, const uint32_t* start, float* rho, float* prs,
-                    float4* velm, float* mp, const Tuning& tu) {
-    density_any<false>(st, T, posm, posl, cellv, start, rho, prs, velm, mp, tu, GatherStats{});
+                    float4* velm, float* mp, float4* frec, const Tuning& tu) {
+    density_any<false>(st, T, posm, posl, cellv, start, rho, prs, velm, mp, frec, tu, GatherStats{});
 }
 
 void launch_force(hipStream_t st, const SceneTable& T, float dt, const ForceArgs& a, bool shear,

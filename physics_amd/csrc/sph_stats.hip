@@ -13,7 +13,7 @@ void launch_gather_stats(hipStream_t st, const SceneTable& T, int pass, const Fo
                          bool shear, const Tuning& tu, const GatherStats& gs) {
     if (pass == 0)
         density_any<true>(st, T, a.posm, a.posl, a.cell, a.start, nullptr, nullptr, nullptr, nullptr,
-                          tu, gs);
+                          nullptr, tu, gs);
     else
         force_any<true>(st, T, 0.f, a, shear, tu, gs);
 }
