@@ -68,7 +68,7 @@ def world_force(disps, particles: int):
     """The 1M world's step-marking force launches (the step's form, not the
     statistics form), in order."""
     grid = -(-particles // K_TILE) * K_TILE
-    pat = re.compile(r"^k_force<\d+, (true|false), \d, false, \d+>$")
+    pat = re.compile(r"^k_force<\d+, (true|false), \d, false, \d+(, (true|false))?>$")
     return [d for d in disps if pat.match(d["name"]) and d["grid"] == grid]
 
 
