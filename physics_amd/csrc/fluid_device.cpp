@@ -331,7 +331,7 @@ void DeviceFluid::ensure_capacity(size_t n) {
         // the uniform scan loads candidates eight at a time
         posm_s_.ensure(c + kScanPad); velm_s_.ensure(c); body_s_.ensure(c);
         posl_.ensure(c + kScanPad);
-        frec_.ensure(c);
+        frec_.ensure(c + kScanPad);  // read by the force pass's scans too
         id_s_.ensure(c); mat_s_.ensure(c); cell_s_.ensure(c);
         rho_.ensure(c); prs_.ensure(c); mp_.ensure(c);
         cell_.ensure(c); rank_.ensure(c); tmp_.ensure(c); ctmp_.ensure(c); keytmp_.ensure(c);

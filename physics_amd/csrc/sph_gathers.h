@@ -433,8 +433,11 @@ __global__ __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(5))) void k_
     // the gather use it with the arrays' scalar bases (no 64-bit per-lane
     // address stays live across the gather)
     const uint32_t self_j = act ? (uint32_t)i : 0u;
+    // UM: the per-lane scan and the overflow segment read candidates' x, y, z
+    // from the records the drains load too, so a drain finds the lines its
+    // scan brought into L1
     gather<L, MODE, (MODE == kUniformWaves ? 1 : 2), LOC, true, 6, BS>(
-        act, ci, pi.x, pi.y, pi.z, self_j, d, P.inv_cell, zwin(P, d), cl, start, posm, lst, tid,
+        act, ci, pi.x, pi.y, pi.z, self_j, d, P.inv_cell, zwin(P, d), cl, start, UM ? frec : posm, lst, tid,
         [&](uint32_t j, const float4& pj) { return force_maybe(P, pi.x, pi.y, pi.z, pj); },
         // UM: two loads per neighbour (the density pass's record and velm)
         [&](uint32_t o) {
