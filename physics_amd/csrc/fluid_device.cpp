@@ -185,7 +185,7 @@ DeviceFluid::DeviceFluid(const std::vector<HostParams>& scenes) {
             scn_[s].bkey[3 + a] = host_key(Ps_[s].hi[a], Ps_[s].inv_cell, Ps_[s].cdiv);
         }
     }
-    dom_.ensure(S);
+    ensure_dom(dom_, S);
     table_total_.ensure(1);
     partial_.ensure(kScanBlocks);
     diag_out_.ensure(5 * S);
@@ -284,6 +284,12 @@ void DeviceFluid::size_tables(const float* pos3) {
         sync();
         graph_dirty_ = true;
     }
+}
+
+void DeviceFluid::ensure_dom(DevBuf<Domain>& b, size_t S) {
+    if (S <= b.cap() && b.get()) return;
+    b.ensure(S);
+    launch_domain_reset(st_, b.get(), (int)S);
 }
 
 void DeviceFluid::push_params() {
@@ -872,7 +878,7 @@ DeviceFluid::StepBufs DeviceFluid::slot_bufs(TapeSlot& t) {
     t.rho.ensure(n); t.prs.ensure(n); t.mp.ensure(n);
     t.acc.ensure(n); t.xsph.ensure(n); t.fvisc.ensure(n); t.mu.ensure(n);
     if (!compact_slots()) t.start.ensure(table_cap_ + 1);
-    t.dom.ensure(scn_.size());
+    ensure_dom(t.dom, scn_.size());
     StepBufs s;
     s.posm_s = t.posm_s.get(); s.velm_s = t.velm_s.get(); s.body_s = t.body_s.get();
     s.posl = posl_.get();
@@ -1085,7 +1091,7 @@ void DeviceFluid::neighbor_stats(uint32_t* cnt, uint64_t* sum, uint64_t* xr) {
     // sort the current state (the same binning as a step) with a domain array
     // of its own, so the last step's watchdog survives, then count
     DevBuf<Domain> dom;
-    dom.ensure(scn_.size());
+    ensure_dom(dom, scn_.size());
     StepBufs sb = own_bufs();
     sb.dom = dom.get();
     run_domain_and_sort(sb);
@@ -1111,7 +1117,7 @@ void DeviceFluid::gather_stats(int pass, uint32_t* cnt, uint64_t* sum, uint64_t*
     // last step are not touched, since the statistics kernels write only the
     // statistics
     DevBuf<Domain> dom;
-    dom.ensure(scn_.size());
+    ensure_dom(dom, scn_.size());
     StepBufs sb = own_bufs();
     sb.dom = dom.get();
     run_domain_and_sort(sb);
@@ -1200,7 +1206,7 @@ void DeviceFluid::one_scene(const char* what) const {
 }
 
 void DeviceFluid::sort_current() {
-    cdom_.ensure(scn_.size());
+    ensure_dom(cdom_, scn_.size());
     StepBufs sb = own_bufs();
     sb.dom = cdom_.get();
     run_domain_and_sort(sb);

@@ -260,6 +260,9 @@ class DeviceFluid {
         DevBuf<Domain> dom;
     };
     void init_stream();
+    // a Domain array of S scenes, reset when (re)allocated: the domain pass
+    // expects reset AABB words and leaves them reset
+    void ensure_dom(DevBuf<Domain>& b, size_t S);
     void ensure_capacity(size_t n);
     void set_layout(const uint32_t* counts);
     void size_tables(const float* pos3);

@@ -145,6 +145,7 @@ void launch_adjoint_step(hipStream_t st, const SceneTable& T, const AdjSlot& S, 
 // Every launch covers all scenes of T; T.dom / T.table_total belong to the
 // step's buffer set. Diagnostics write 5 doubles per scene; partial holds
 // 5 per tile.
+void launch_domain_reset(hipStream_t st, Domain* dom, int nscenes);
 void launch_domain(hipStream_t st, const SceneTable& T, const float4* posm);
 void launch_sort(hipStream_t st, const SceneTable& T, const SortBuffers& b);
 void launch_density(hipStream_t st, const SceneTable& T, const float4* posm, const float4* posl,

@@ -190,6 +190,9 @@ __device__ void finalize_domain(Domain* d, unsigned long long cap, const long lo
 
 // One workgroup: every scene's domain, then the scenes' table offsets as an
 // exclusive scan of (ncells + 1) in scene order, and the table's total size.
+// It also resets what the next domain pass and this step's force pass
+// accumulate into: the key AABB (consumed here) and the watchdog words (the
+// host reads them after the step), so a step needs no reset launch.
 __global__ __launch_bounds__(256) void k_domain_finalize(SceneTable T) {
     __shared__ uint32_t sh[4];
     uint32_t carry = 0;
@@ -198,8 +201,15 @@ __global__ __launch_bounds__(256) void k_domain_finalize(SceneTable T) {
         const int s = base + tid;
         uint32_t cells = 0;
         if (s < T.nscenes) {
-            finalize_domain(T.dom + s, T.info[s].cap, T.info[s].bkey, T.params[s].cdiv);
-            cells = T.dom[s].ncells + 1u;
+            Domain* dm = T.dom + s;
+            finalize_domain(dm, T.info[s].cap, T.info[s].bkey, T.params[s].cdiv);
+            cells = dm->ncells + 1u;
+            for (int a = 0; a < 3; ++a) {
+                dm->amin[a] = 0x7fffffffffffffffll;
+                dm->amax[a] = (long long)0x8000000000000000ull;
+            }
+            dm->bad_pos = 0xffffffffu;
+            dm->bad_vel = 0xffffffffu;
         }
         uint32_t x = cells;
         for (int o = 1; o < 64; o <<= 1) {
@@ -587,8 +597,13 @@ __global__ __launch_bounds__(256) void k_neighbor_stats(SceneTable T, const floa
 
 static inline int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
+void launch_domain_reset(hipStream_t st, Domain* dom, int nscenes) {
+    if (nscenes > 0) hipLaunchKernelGGL(k_domain_reset, dim3(nblk(nscenes, 256)), dim3(256), 0, st, dom, nscenes);
+}
+
+// dom must hold reset AABB words (a new array through launch_domain_reset;
+// afterwards k_domain_finalize resets them for the next pass)
 void launch_domain(hipStream_t st, const SceneTable& T, const float4* posm) {
-    hipLaunchKernelGGL(k_domain_reset, dim3(nblk(T.nscenes, 256)), dim3(256), 0, st, T.dom, T.nscenes);
     if (T.ntiles > 0) {
         // about eight workgroups per scene, at least 512: more workgroups per
         // scene only add atomic traffic on that scene's domain

@@ -94,12 +94,12 @@ def split(disps, force, a):
     def take(tag, f0, s0, s1):
         # step s of the world whose first force launch is force[f0] ends with
         # force[f0 + s - 1]; the first step of a world starts with its first
-        # domain reset after the previous world's last step
+        # AABB pass after the previous world's last step
         if s0 > 1:
             lo = force[f0 + s0 - 2]["end"]
         else:
             prev = force[f0 - 1]["end"] if f0 > 0 else -1
-            lo = min(d["start"] for d in disps if d["start"] > prev and d["name"] == "k_domain_reset") - 1
+            lo = min(d["start"] for d in disps if d["start"] > prev and d["name"] == "k_aabb") - 1
         hi = force[f0 + s1 - 1]["end"]
         out[tag] = [d for d in disps if d["start"] > lo and d["end"] <= hi and not NOT_STEP.match(d["name"])]
 
