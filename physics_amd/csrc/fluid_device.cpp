@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstddef>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -1049,15 +1050,21 @@ void DeviceFluid::last_validate_all(int32_t* status) {
         for (size_t s = 0; s < S; ++s) status[s] = 0;
         return;
     }
-    std::vector<Domain> d(S);
-    hip_check(hipMemcpyAsync(d.data(), dom_.get(), S * sizeof(Domain), hipMemcpyDeviceToHost, st_), "watchdog");
+    // only the two watchdog words of each scene's Domain
+    static_assert(offsetof(Domain, bad_vel) == offsetof(Domain, bad_pos) + sizeof(uint32_t), "adjacent words");
+    std::vector<uint32_t> w(2 * S);
+    hip_check(hipMemcpy2DAsync(w.data(), 2 * sizeof(uint32_t),
+                               reinterpret_cast<const char*>(dom_.get()) + offsetof(Domain, bad_pos),
+                               sizeof(Domain), 2 * sizeof(uint32_t), S, hipMemcpyDeviceToHost, st_),
+              "watchdog");
     sync();
     for (size_t s = 0; s < S; ++s) {
         // first particle in index order with a bad field; pos is checked before vel
-        if (d[s].bad_pos == 0xffffffffu && d[s].bad_vel == 0xffffffffu)
+        const uint32_t bp = w[2 * s], bv = w[2 * s + 1];
+        if (bp == 0xffffffffu && bv == 0xffffffffu)
             status[s] = 0;
         else
-            status[s] = d[s].bad_pos <= d[s].bad_vel ? 1 : 2;
+            status[s] = bp <= bv ? 1 : 2;
     }
 }
 

@@ -89,6 +89,7 @@ __device__ __forceinline__ void kahan_add(float& sum, float& c, float x) {
 // Workgroups are dispatched round-robin over the 8 XCDs; remap so consecutive
 // logical blocks (spatially adjacent cells) share one XCD's L2.
 __device__ __forceinline__ int xcd_block(int b, int nb) {
+    static_assert(kXcds == 8, "the remap divides by shifts");
     int per = nb >> 3, rem = nb & 7;
     int x = b & 7, k = b >> 3;
     return x * per + min(x, rem) + k;

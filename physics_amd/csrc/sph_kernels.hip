@@ -45,10 +45,11 @@ __global__ __launch_bounds__(256) void k_domain_reset(Domain* dom, int nscenes) 
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nscenes) return;
     Domain* d = dom + s;
-    for (int a = 0; a < 3; ++a) {
-        d->amin[a] = 0x7fffffffffffffffll;
-        d->amax[a] = (long long)0x8000000000000000ull;
-    }
+    for (int x = 0; x < kXcds; ++x)
+        for (int a = 0; a < 3; ++a) {
+            d->amin[x][a] = 0x7fffffffffffffffll;
+            d->amax[x][a] = (long long)0x8000000000000000ull;
+        }
     d->bad_pos = 0xffffffffu;
     d->bad_vel = 0xffffffffu;
 }
@@ -89,8 +90,9 @@ __global__ __launch_bounds__(256) void k_aabb(SceneTable T, const float4* __rest
                 hi = sh[3 + a][k] > hi ? sh[3 + a][k] : hi;
             }
             if (lo <= hi) {
-                atomicMin(&T.dom[s].amin[a], lo);
-                atomicMax(&T.dom[s].amax[a], hi);
+                const int c = (int)blockIdx.x % aabb_copies(T.nscenes);
+                atomicMin(&T.dom[s].amin[c][a], lo);
+                atomicMax(&T.dom[s].amax[c][a], hi);
             }
         }
         __syncthreads();
@@ -144,14 +146,23 @@ __global__ __launch_bounds__(256) void k_aabb(SceneTable T, const float4* __rest
 // Turns one scene's key AABB into its dense table domain, clipped to the
 // scene's table capacity. Particles outside the domain are binned to the
 // scene's overflow cell.
-__device__ void finalize_domain(Domain* d, unsigned long long cap, const long long* bk, int cdiv) {
+__device__ void finalize_domain(Domain* d, unsigned long long cap, const long long* bk, int cdiv, int ncopies) {
     long long lo[3], hi[3];
     const long long* blo = bk;
     const long long* bhi = bk + 3;
-    bool any = d->amin[0] <= d->amax[0];
+    long long mn[3], mx[3];
     for (int a = 0; a < 3; ++a) {
-        lo[a] = any ? d->amin[a] : blo[a];
-        hi[a] = any ? d->amax[a] : blo[a];
+        mn[a] = d->amin[0][a];
+        mx[a] = d->amax[0][a];
+        for (int x = 1; x < ncopies; ++x) {
+            mn[a] = d->amin[x][a] < mn[a] ? d->amin[x][a] : mn[a];
+            mx[a] = d->amax[x][a] > mx[a] ? d->amax[x][a] : mx[a];
+        }
+    }
+    bool any = mn[0] <= mx[0];
+    for (int a = 0; a < 3; ++a) {
+        lo[a] = any ? mn[a] : blo[a];
+        hi[a] = any ? mx[a] : blo[a];
     }
     // extents as unsigned differences: exact for any hi >= lo
     auto ext = [&](int a) { return ((unsigned long long)hi[a] - (unsigned long long)lo[a]) + 1ull; };
@@ -202,12 +213,14 @@ __global__ __launch_bounds__(256) void k_domain_finalize(SceneTable T) {
         uint32_t cells = 0;
         if (s < T.nscenes) {
             Domain* dm = T.dom + s;
-            finalize_domain(dm, T.info[s].cap, T.info[s].bkey, T.params[s].cdiv);
+            const int nc = aabb_copies(T.nscenes);
+            finalize_domain(dm, T.info[s].cap, T.info[s].bkey, T.params[s].cdiv, nc);
             cells = dm->ncells + 1u;
-            for (int a = 0; a < 3; ++a) {
-                dm->amin[a] = 0x7fffffffffffffffll;
-                dm->amax[a] = (long long)0x8000000000000000ull;
-            }
+            for (int x = 0; x < nc; ++x)
+                for (int a = 0; a < 3; ++a) {
+                    dm->amin[x][a] = 0x7fffffffffffffffll;
+                    dm->amax[x][a] = (long long)0x8000000000000000ull;
+                }
             dm->bad_pos = 0xffffffffu;
             dm->bad_vel = 0xffffffffu;
         }

@@ -74,6 +74,12 @@ struct SceneInfo {
 // neighbour row are one contiguous run of sorted particles. Local cell index
 // ncells is the scene's overflow segment. The scene's cells 0 .. ncells sit at
 // toff in the concatenated cell table of the step.
+// XCDs of the MI355X (workgroups are dealt to them round-robin)
+constexpr int kXcds = 8;
+
+// AABB copies in use: one per XCD for up to kXcds scenes, else one.
+__host__ __device__ inline int aabb_copies(int nscenes) { return nscenes <= kXcds ? kXcds : 1; }
+
 struct Domain {
     long long kmin[3];
     long long dims[3];
@@ -84,8 +90,11 @@ struct Domain {
     int32_t reach;
     int32_t pad0;
     // AABB of the finite keys of the current positions (filled by k_aabb).
-    long long amin[3];
-    long long amax[3];
+    // With at most kXcds scenes, one copy per XCD (workgroup b runs on XCD b
+    // mod 8): a workgroup's atomics share their words with an eighth of the
+    // grid only; larger batches spread theirs over their scenes and use copy 0.
+    long long amin[kXcds][3];
+    long long amax[kXcds][3];
     // Watchdog (subsystem.rs:74-90): smallest particle index (concatenated
     // original order) with a non-finite position / velocity after the step;
     // 0xffffffff when none.
