@@ -5,7 +5,7 @@ golden vectors, so each oracle function is pinned by checks that a plausible
 mistake fails:
   * kernels          Mueller-2003 normalisation and derivative identities by
                      quadrature / finite differences (not by retyping formulas)
-  * lattice_mass     survey values + the interior-density identity rho = rho0
+  * lattice_mass     three values at h = 0.2 + the interior-density identity rho = rho0
   * fill_box         closed-form counts of the reference's x += d loop
   * neighbour grid   O(N^2) brute force with the reference predicate, exact sets
   * density          brute force, single-particle closed form, lattice interior
@@ -86,7 +86,7 @@ def test_viscosity_laplacian_is_laplacian_of_viscosity_kernel(h):
         assert O.kernels(h, rr)[1][2] == pytest.approx(d2 + 2 * d1 / rr, rel=1e-5)
 
 
-def test_lattice_mass_survey_values_and_interior_density():
+def test_lattice_mass_values_and_interior_density():
     w = O.World()
     assert w.lattice_mass(0.1) == pytest.approx(0.99031946198009, rel=1e-13)
     assert w.lattice_mass(0.3) == pytest.approx(5.10633472583484, rel=1e-13)
@@ -561,7 +561,7 @@ def _checks():
     run("two_particle", lambda: [test_two_particle_forces_closed_form(r) for r in (0.05, 0.12)])
     run("antisym", test_pressure_forces_are_pairwise_antisymmetric)
     run("density", test_density_matches_brute_force_and_single_particle)
-    run("interior", test_lattice_mass_survey_values_and_interior_density)
+    run("interior", test_lattice_mass_values_and_interior_density)
     run("c1", test_readme_dam_break_is_closed_form)
     run("material", test_ref_material_tag_distinguishes_viscosity)
     run("power_law_closed_form", lambda: [test_power_law_two_particle_closed_form(n, r)
